@@ -1,0 +1,606 @@
+// couplb_hip.hip -- device context and C ABI (include/couplb_hip.h).
+#include <cstring>
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include <hip/hip_runtime.h>
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include "../../include/couplb_hip.h"
+#include "ibm.hpp"
+#include "kernels.hpp"
+
+using namespace couplb;
+
+namespace {
+
+struct SlotHost {
+  int* rowstart = nullptr;
+  MarkerRec* rec = nullptr;
+  int n = 0;
+};
+
+}  // namespace
+
+struct couplb_ctx {
+  couplb_params p{};
+  int D = 3, Q = 19;
+  Geo geo{};
+  Walls walls{};
+  size_t N = 0, pitch = 0, elem = 8;
+  void* f[2] = {nullptr, nullptr};
+  int cur = 0;           // buffer holding the state
+  int local = 1;         // state is post-stream (create / import)
+  double* rho = nullptr; // collision density (interior layout)
+  int rho_fresh = 0;     // density refreshed since the last step (P1)
+  int has_moving = 0;
+  double base = 0.0;
+  float scale = 1.0f;
+  int cycle = 0;         // state counter for the u stamps
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {};
+  double last_ms[2] = {0, 0};
+  std::string err;
+  // macroscopic staging
+  double* mac = nullptr;      // 4*N
+  double* partial = nullptr;  // 9*ny*nz
+  double* red = nullptr;      // 6 + 3*ny
+  // IBM
+  double *ux = nullptr, *uy = nullptr, *uz = nullptr;
+  int* stamp = nullptr;
+  int cap = 0;
+  unsigned *keys = nullptr, *skeys = nullptr;
+  int *vals = nullptr, *perm = nullptr;
+  int* rowstart_cur = nullptr;
+  Base* bases = nullptr;
+  void* sort_tmp = nullptr;
+  size_t sort_tmp_bytes = 0;
+  double *dx = nullptr, *dv = nullptr, *dm = nullptr, *df = nullptr;
+  int order_n = -1;       // marker count of the cached ordering
+  int ugather_cycle = -1; // state counter of the last velocity gather
+  std::vector<SlotHost> slots;
+  int nslots = 0;
+  double* staging = nullptr;  // ghost-inclusive FP64 state for import/export
+  size_t staging_len = 0;
+};
+
+namespace {
+
+int fail(couplb_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPCHK(ctx, call)                                                            \
+  do {                                                                               \
+    hipError_t e_ = (call);                                                          \
+    if (e_ != hipSuccess)                                                            \
+      return fail(ctx, COUPLB_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+size_t round_up(size_t a, size_t b) { return (a + b - 1) / b * b; }
+
+int storage_bytes(int s) { return s == COUPLB_FP64 ? 8 : s == COUPLB_FP32 ? 4 : 2; }
+
+dim3 row_block(int nx) {
+  const int bx = nx > 128 ? 256 : (nx > 64 ? 128 : 64);
+  return dim3(bx, 1, 1);
+}
+dim3 row_grid(const Geo& g, int bx) {
+  return dim3((unsigned)((g.nx + bx - 1) / bx), (unsigned)g.ny, (unsigned)(g.D == 3 ? g.nz : 1));
+}
+
+PullCtx make_pull(couplb_ctx* c) {
+  PullCtx pc;
+  pc.src = c->f[c->cur];
+  pc.pitch = c->pitch;
+  pc.g = c->geo;
+  pc.w = c->walls;
+  pc.rho_coll = c->rho;
+  pc.rho_floor = std::max(0.1 * c->p.rho0, RHO_MIN);  // couplb_streaming.h:86
+  pc.local = c->local;
+  pc.base = (c->p.storage == COUPLB_FP64) ? 0.0 : c->p.rho0;
+  pc.rho0 = c->p.rho0;
+  pc.scale = c->scale;
+  pc.inv_scale = 1.0f / c->scale;
+  return pc;
+}
+
+// Dispatch over lattice x storage.
+template <template <int, typename> class F, typename... Args>
+void dispatch(couplb_ctx* c, Args&&... args) {
+  const int s = c->p.storage;
+  if (c->D == 3) {
+    if (s == COUPLB_FP64) F<3, double>::run(c, args...);
+    else if (s == COUPLB_FP32) F<3, float>::run(c, args...);
+    else if (s == COUPLB_FP16) F<3, _Float16>::run(c, args...);
+    else F<3, bf16_t>::run(c, args...);
+  } else {
+    if (s == COUPLB_FP64) F<2, double>::run(c, args...);
+    else if (s == COUPLB_FP32) F<2, float>::run(c, args...);
+    else if (s == COUPLB_FP16) F<2, _Float16>::run(c, args...);
+    else F<2, bf16_t>::run(c, args...);
+  }
+}
+
+template <int D, typename S> struct LaunchStep {
+  static void run(couplb_ctx* c, const StepArgs& a) {
+    const dim3 b = row_block(c->geo.nx);
+    hipLaunchKernelGGL((k_step<D, S>), row_grid(c->geo, b.x), b, 0, c->stream, a);
+  }
+};
+template <int D, typename S> struct LaunchMacro {
+  static void run(couplb_ctx* c, const MacroArgs& a) {
+    const dim3 b = row_block(c->geo.nx);
+    hipLaunchKernelGGL((k_macro<D, S>), dim3(1, c->geo.ny, c->D == 3 ? c->geo.nz : 1), b, 0,
+                       c->stream, a);
+  }
+};
+template <int D, typename S> struct LaunchImport {
+  static void run(couplb_ctx* c, const double* fin, void* dst) {
+    const dim3 b = row_block(c->geo.nx);
+    hipLaunchKernelGGL((k_import<D, S>), row_grid(c->geo, b.x), b, 0, c->stream, fin, dst,
+                       c->pitch, c->geo, c->p.rho0, c->scale);
+  }
+};
+template <int D, typename S> struct LaunchExport {
+  static void run(couplb_ctx* c, const PullCtx& pc, double* fout) {
+    const int gx = c->geo.nx + 2, gy = c->geo.ny + 2, gz = (D == 3) ? c->geo.nz + 2 : 1;
+    const dim3 b = row_block(gx);
+    const dim3 grid((unsigned)((gx + b.x - 1) / b.x), (unsigned)gy, (unsigned)gz);
+    hipLaunchKernelGGL((k_export<D, S>), grid, b, 0, c->stream, pc, fout);
+  }
+};
+template <int D, typename S> struct LaunchUgather {
+  static void run(couplb_ctx* c, const PullCtx& pc, const BaseSlot& bs) {
+    const dim3 b = row_block(c->geo.nx);
+    hipLaunchKernelGGL((k_ugather<D, S>), row_grid(c->geo, b.x), b, 0, c->stream, pc, bs, c->ux,
+                       c->uy, c->uz, c->stamp, c->cycle);
+  }
+};
+
+int ensure_macro_buffers(couplb_ctx* c) {
+  if (!c->mac) HIPCHK(c, hipMalloc(&c->mac, sizeof(double) * 4 * c->N));
+  if (!c->partial)
+    HIPCHK(c, hipMalloc(&c->partial, sizeof(double) * 9 * (size_t)c->geo.ny * std::max(1, c->geo.nz)));
+  if (!c->red) HIPCHK(c, hipMalloc(&c->red, sizeof(double) * (6 + 3 * (size_t)c->geo.ny)));
+  return COUPLB_OK;
+}
+
+int run_macro(couplb_ctx* c, bool want_fields) {
+  int rc = ensure_macro_buffers(c);
+  if (rc) return rc;
+  MacroArgs a;
+  a.pc = make_pull(c);
+  a.rho = want_fields ? c->mac : nullptr;
+  a.ux = want_fields ? c->mac + c->N : nullptr;
+  a.uy = want_fields ? c->mac + 2 * c->N : nullptr;
+  a.uz = want_fields ? c->mac + 3 * c->N : nullptr;
+  a.partial = c->partial;
+  dispatch<LaunchMacro>(c, a);
+  HIPCHK(c, hipGetLastError());
+  hipLaunchKernelGGL(k_macro_reduce, dim3(1), dim3(256), 0, c->stream, c->partial, c->geo.ny,
+                     c->D == 3 ? c->geo.nz : 1, c->red, c->red + 6);
+  HIPCHK(c, hipGetLastError());
+  c->rho_fresh = 1;
+  return COUPLB_OK;
+}
+
+unsigned key_bits(int nrows) {
+  unsigned b = 1;
+  while ((1u << b) <= (unsigned)nrows) b++;
+  return b;
+}
+
+int ensure_ibm(couplb_ctx* c, int n) {
+  if (!c->ux) {
+    HIPCHK(c, hipMalloc(&c->ux, sizeof(double) * c->N));
+    HIPCHK(c, hipMalloc(&c->uy, sizeof(double) * c->N));
+    HIPCHK(c, hipMalloc(&c->uz, sizeof(double) * c->N));
+    HIPCHK(c, hipMalloc(&c->stamp, sizeof(int) * c->N));
+    HIPCHK(c, hipMemsetAsync(c->stamp, 0xFF, sizeof(int) * c->N, c->stream));
+    HIPCHK(c, hipMalloc(&c->rowstart_cur, sizeof(int) * (c->geo.nrows + 1)));
+  }
+  if (n <= c->cap) return COUPLB_OK;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const int cap = std::max(n, 2 * c->cap);
+  (void)hipFree(c->keys); (void)hipFree(c->skeys); (void)hipFree(c->vals); (void)hipFree(c->perm); (void)hipFree(c->bases);
+  (void)hipFree(c->sort_tmp); (void)hipFree(c->dx); (void)hipFree(c->dv); (void)hipFree(c->dm); (void)hipFree(c->df);
+  HIPCHK(c, hipMalloc(&c->keys, sizeof(unsigned) * cap));
+  HIPCHK(c, hipMalloc(&c->skeys, sizeof(unsigned) * cap));
+  HIPCHK(c, hipMalloc(&c->vals, sizeof(int) * cap));
+  HIPCHK(c, hipMalloc(&c->perm, sizeof(int) * cap));
+  HIPCHK(c, hipMalloc(&c->bases, sizeof(Base) * cap));
+  HIPCHK(c, hipMalloc(&c->dx, sizeof(double) * 3 * cap));
+  HIPCHK(c, hipMalloc(&c->dv, sizeof(double) * 3 * cap));
+  HIPCHK(c, hipMalloc(&c->dm, sizeof(double) * cap));
+  HIPCHK(c, hipMalloc(&c->df, sizeof(double) * 3 * cap));
+  size_t bytes = 0;
+  HIPCHK(c, rocprim::radix_sort_pairs(nullptr, bytes, c->keys, c->skeys, c->vals, c->perm,
+                                      (size_t)cap, 0u, key_bits(c->geo.nrows), c->stream));
+  c->sort_tmp_bytes = bytes;
+  HIPCHK(c, hipMalloc(&c->sort_tmp, bytes));
+  for (auto& s : c->slots) {
+    (void)hipFree(s.rec);
+    s.rec = nullptr;
+    HIPCHK(c, hipMalloc(&s.rec, sizeof(MarkerRec) * cap));
+  }
+  c->cap = cap;
+  c->order_n = -1;
+  return COUPLB_OK;
+}
+
+int ensure_slot(couplb_ctx* c, int s) {
+  while ((int)c->slots.size() <= s) {
+    if ((int)c->slots.size() >= MAX_SLOTS)
+      return fail(c, COUPLB_ESTATE, "more than 16 IBM sub-steps between two LB steps");
+    SlotHost h;
+    HIPCHK(c, hipMalloc(&h.rowstart, sizeof(int) * (c->geo.nrows + 1)));
+    HIPCHK(c, hipMalloc(&h.rec, sizeof(MarkerRec) * std::max(1, c->cap)));
+    c->slots.push_back(h);
+  }
+  return COUPLB_OK;
+}
+
+int ibm_impl(couplb_ctx* c, int n, const double* x, const double* v, const double* m,
+             double* f_dev, int fmode, const couplb_ibm_args* a) {
+  if (n < 0 || !a) return fail(c, COUPLB_EINVAL, "bad IBM arguments");
+  if (n == 0) return COUPLB_OK;
+  int rc = ensure_ibm(c, n);
+  if (rc) return rc;
+  const int s = c->nslots;
+  rc = ensure_slot(c, s);
+  if (rc) return rc;
+  if (c->slots[s].rec == nullptr || c->cap < n) return fail(c, COUPLB_ENOMEM, "slot capacity");
+  IbmParams ip;
+  for (int d = 0; d < 3; d++) ip.lo[d] = a->lo[d];
+  ip.inv_dx = a->inv_dx;
+  ip.xi_over_dtLB = a->xi_over_dtLB;
+  ip.vel_scale = a->vel_scale;
+  ip.inv_N = a->inv_N;
+  ip.force_scale = a->force_scale;
+  ip.kern = c->p.kernel;
+  const int tb = 256, nb = (n + tb - 1) / tb;
+  HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+  const bool reorder = !(a->positions_unchanged && c->order_n == n);
+  if (reorder) {
+    hipLaunchKernelGGL(k_keys, dim3(nb), dim3(tb), 0, c->stream, n, x, ip, c->geo, c->keys, c->vals);
+    HIPCHK(c, hipGetLastError());
+    size_t bytes = c->sort_tmp_bytes;
+    HIPCHK(c, rocprim::radix_sort_pairs(c->sort_tmp, bytes, c->keys, c->skeys, c->vals, c->perm,
+                                        (size_t)n, 0u, key_bits(c->geo.nrows), c->stream));
+    hipLaunchKernelGGL(k_rowstart, dim3((c->geo.nrows + 1 + tb - 1) / tb), dim3(tb), 0, c->stream,
+                       c->skeys, n, c->geo.nrows, c->rowstart_cur);
+    hipLaunchKernelGGL(k_bases, dim3(nb), dim3(tb), 0, c->stream, n, c->perm, x, ip, c->geo, c->bases);
+    HIPCHK(c, hipGetLastError());
+    c->order_n = n;
+  }
+  if (reorder || c->ugather_cycle != c->cycle) {
+    BaseSlot bs{c->rowstart_cur, c->bases};
+    PullCtx pc = make_pull(c);
+    dispatch<LaunchUgather>(c, pc, bs);
+    HIPCHK(c, hipGetLastError());
+    c->ugather_cycle = c->cycle;
+  }
+  SlotHost& sl = c->slots[s];
+  if (c->D == 3)
+    hipLaunchKernelGGL((k_marker<3>), dim3(nb), dim3(tb), 0, c->stream, n, c->perm, x, v, m, f_dev,
+                       fmode, ip, c->geo, c->walls, c->ux, c->uy, c->uz, sl.rec);
+  else
+    hipLaunchKernelGGL((k_marker<2>), dim3(nb), dim3(tb), 0, c->stream, n, c->perm, x, v, m, f_dev,
+                       fmode, ip, c->geo, c->walls, c->ux, c->uy, c->uz, sl.rec);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(sl.rowstart, c->rowstart_cur, sizeof(int) * (c->geo.nrows + 1),
+                           hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+  sl.n = n;
+  c->nslots = s + 1;
+  c->rho_fresh = 1;  // compute_macroscopic(false) at fix_couplb.cpp:758
+  return COUPLB_OK;
+}
+
+int step_impl(couplb_ctx* c, const double g[3], int flags, bool timed) {
+  StepArgs a;
+  a.pc = make_pull(c);
+  const int dst = c->cur ^ 1;
+  a.dst = c->f[dst];
+  a.omega = 1.0 / c->p.tau;
+  const double ZERO_TOL = 1e-15;  // fix_couplb.cpp:693-695
+  const bool grav = g && !(std::fabs(g[0]) < ZERO_TOL && std::fabs(g[1]) < ZERO_TOL &&
+                           std::fabs(g[2]) < ZERO_TOL);
+  for (int d = 0; d < 3; d++) a.g[d] = grav ? g[d] : 0.0;
+  a.grav_mode = grav ? (c->rho_fresh ? 1 : 2) : 0;
+  a.write_rho = (grav && !(flags & COUPLB_STEP_RHO_REFRESHED_NEXT)) ? 1 : 0;
+  a.has_moving = c->has_moving;
+  a.rho_out = c->rho;
+  a.nslots = c->nslots;
+  for (int s = 0; s < c->nslots; s++) {
+    a.slots[s].rowstart = c->slots[s].rowstart;
+    a.slots[s].rec = c->slots[s].rec;
+  }
+  if (timed) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+  dispatch<LaunchStep>(c, a);
+  HIPCHK(c, hipGetLastError());
+  if (timed) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+  c->cur = dst;
+  c->local = 0;
+  c->nslots = 0;
+  c->rho_fresh = 0;
+  c->cycle++;
+  return COUPLB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* couplb_version(void) { return "couplb-mi355x 0.1 (gfx950)"; }
+
+const char* couplb_last_error(const couplb_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int couplb_create(const couplb_params* p, couplb_ctx** out) {
+  if (!p || !out) return COUPLB_EINVAL;
+  *out = nullptr;
+  auto* c = new couplb_ctx();
+  c->p = *p;
+  auto bad = [&](const char* m) { c->err = m; *out = c; return COUPLB_EINVAL; };
+  if (p->dim != 2 && p->dim != 3) return bad("dim must be 2 or 3");
+  const int nz = p->dim == 3 ? p->nz : 1;
+  if (p->nx < 2 || p->ny < 2 || (p->dim == 3 && p->nz < 2) || (p->dim == 2 && p->nz != 1))
+    return bad("grid extents must be >= 2 (nz = 1 in 2D)");
+  if (!(p->tau > 0.5)) return bad("tau <= 0.5 (unstable)");
+  if (p->kernel != COUPLB_ROMA && p->kernel != COUPLB_PESKIN4) return bad("unknown IBM kernel");
+  if (p->storage < COUPLB_FP64 || p->storage > COUPLB_BF16) return bad("unknown storage");
+  if (!(p->rho0 > 0)) return bad("rho0 must be positive");
+  for (int d = 0; d < 3; d++)
+    for (int s = 0; s < 2; s++)
+      if (p->wall[d][s] < 0 || p->wall[d][s] > 4) return bad("wall types must be in [0,4]");
+  if (p->dim == 2 && (p->wall[2][0] || p->wall[2][1])) return bad("2D grid with z walls");
+  if ((size_t)p->nx * p->ny * nz >= (1ull << 32)) return bad("grid too large for 32-bit node index");
+  c->D = p->dim;
+  c->Q = p->dim == 3 ? 19 : 9;
+  c->geo.nx = p->nx; c->geo.ny = p->ny; c->geo.nz = nz; c->geo.D = p->dim;
+  c->geo.nry = p->ny + 5;
+  c->geo.nrows = c->geo.nry * (p->dim == 3 ? nz + 5 : 1);
+  for (int d = 0; d < 3; d++)
+    for (int s = 0; s < 2; s++) {
+      c->walls.type[d][s] = (d < p->dim) ? p->wall[d][s] : 0;
+      for (int k = 0; k < 3; k++) c->walls.vel[d][s][k] = p->face_vel[d][s][k];
+      if (d < p->dim && p->wall[d][s] == 2) c->has_moving = 1;
+    }
+  c->N = (size_t)p->nx * p->ny * nz;
+  c->pitch = round_up(c->N, 512);
+  c->elem = storage_bytes(p->storage);
+  c->scale = (p->storage == COUPLB_FP16) ? 1024.0f : 1.0f;
+  c->base = (p->storage == COUPLB_FP64) ? 0.0 : p->rho0;
+  int dev = p->device;
+  if (hipSetDevice(dev) != hipSuccess) return bad("hipSetDevice failed");
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+    return bad("hipStreamCreate failed");
+  for (auto& e : c->ev)
+    if (hipEventCreate(&e) != hipSuccess) return bad("hipEventCreate failed");
+  const size_t fbytes = c->elem * c->pitch * c->Q;
+  for (int b = 0; b < 2; b++) {
+    if (hipMalloc(&c->f[b], fbytes) != hipSuccess) {
+      c->err = "hipMalloc of populations failed";
+      *out = c;
+      return COUPLB_ENOMEM;
+    }
+  }
+  if (hipMalloc(&c->rho, sizeof(double) * c->N) != hipSuccess) {
+    c->err = "hipMalloc of density failed";
+    *out = c;
+    return COUPLB_ENOMEM;
+  }
+  // init_equilibrium(rho_lb, 0, 0, 0) (couplb_lattice.h:145-154): f = w_q
+  // rho0 (zero deviation); density array = rho0 (fix_couplb.cpp:504).
+  {
+    const int gx = p->nx + 2, gy = p->ny + 2, gz = p->dim == 3 ? nz + 2 : 1;
+    const size_t nt = (size_t)gx * gy * gz;
+    std::vector<double> h((size_t)c->Q * nt);
+    const double* w3 = nullptr;
+    static const double W3[19] = {1.0/3.0, 1.0/18.0, 1.0/18.0, 1.0/18.0, 1.0/18.0, 1.0/18.0, 1.0/18.0,
+                                  1.0/36.0, 1.0/36.0, 1.0/36.0, 1.0/36.0, 1.0/36.0, 1.0/36.0,
+                                  1.0/36.0, 1.0/36.0, 1.0/36.0, 1.0/36.0, 1.0/36.0, 1.0/36.0};
+    static const double W2[9] = {4.0/9.0, 1.0/9.0, 1.0/9.0, 1.0/9.0, 1.0/9.0,
+                                 1.0/36.0, 1.0/36.0, 1.0/36.0, 1.0/36.0};
+    w3 = p->dim == 3 ? W3 : W2;
+    for (int q = 0; q < c->Q; q++)
+      for (size_t n = 0; n < nt; n++) h[q * nt + n] = w3[q] * p->rho0 * (1.0 + 0.0);
+    *out = c;
+    int rc = couplb_set_state_f(c, h.data());
+    if (rc) return rc;
+    std::vector<double> r(c->N, p->rho0);
+    if (hipMemcpy(c->rho, r.data(), sizeof(double) * c->N, hipMemcpyHostToDevice) != hipSuccess)
+      return fail(c, COUPLB_EDEVICE, "initial density upload failed");
+    c->rho_fresh = 0;  // the first step's gravity reads the array (= rho0)
+  }
+  c->err.clear();
+  return COUPLB_OK;
+}
+
+int couplb_destroy(couplb_ctx* c) {
+  if (!c) return COUPLB_OK;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(c->f[0]); (void)hipFree(c->f[1]); (void)hipFree(c->rho);
+  (void)hipFree(c->mac); (void)hipFree(c->partial); (void)hipFree(c->red);
+  (void)hipFree(c->ux); (void)hipFree(c->uy); (void)hipFree(c->uz); (void)hipFree(c->stamp);
+  (void)hipFree(c->keys); (void)hipFree(c->skeys); (void)hipFree(c->vals); (void)hipFree(c->perm);
+  (void)hipFree(c->rowstart_cur); (void)hipFree(c->bases); (void)hipFree(c->sort_tmp);
+  (void)hipFree(c->dx); (void)hipFree(c->dv); (void)hipFree(c->dm); (void)hipFree(c->df);
+  (void)hipFree(c->staging);
+  for (auto& s : c->slots) { (void)hipFree(s.rowstart); (void)hipFree(s.rec); }
+  for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return COUPLB_OK;
+}
+
+long couplb_state_len(const couplb_ctx* c) {
+  if (!c) return 0;
+  const long gx = c->geo.nx + 2, gy = c->geo.ny + 2, gz = c->D == 3 ? c->geo.nz + 2 : 1;
+  return (long)c->Q * gx * gy * gz;
+}
+
+static int ensure_staging(couplb_ctx* c) {
+  const size_t len = (size_t)couplb_state_len(c);
+  if (c->staging_len < len) {
+    (void)hipFree(c->staging);
+    c->staging = nullptr;
+    HIPCHK(c, hipMalloc(&c->staging, sizeof(double) * len));
+    c->staging_len = len;
+  }
+  return COUPLB_OK;
+}
+
+int couplb_set_state_f(couplb_ctx* c, const double* fh) {
+  if (!c || !fh) return fail(c, COUPLB_EINVAL, "null argument");
+  int rc = ensure_staging(c);
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpyAsync(c->staging, fh, sizeof(double) * couplb_state_len(c),
+                           hipMemcpyHostToDevice, c->stream));
+  c->cur = 0;
+  dispatch<LaunchImport>(c, (const double*)c->staging, c->f[0]);
+  HIPCHK(c, hipGetLastError());
+  c->local = 1;
+  c->nslots = 0;
+  c->cycle++;
+  // read_checkpoint ends with compute_macroscopic(false) (couplb_io.h:817):
+  // the density array then holds the restored density.
+  rc = run_macro(c, true);
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpyAsync(c->rho, c->mac, sizeof(double) * c->N, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return COUPLB_OK;
+}
+
+int couplb_get_state_f(couplb_ctx* c, double* fh) {
+  if (!c || !fh) return fail(c, COUPLB_EINVAL, "null argument");
+  int rc = ensure_staging(c);
+  if (rc) return rc;
+  PullCtx pc = make_pull(c);
+  dispatch<LaunchExport>(c, pc, c->staging);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(fh, c->staging, sizeof(double) * couplb_state_len(c),
+                           hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return COUPLB_OK;
+}
+
+int couplb_lbm_step(couplb_ctx* c, const double g[3], int flags) {
+  if (!c) return COUPLB_EINVAL;
+  int rc = step_impl(c, g, flags, true);
+  if (rc) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  float ms = 0;
+  HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+  c->last_ms[0] = ms;
+  return COUPLB_OK;
+}
+
+int couplb_lbm_steps_async(couplb_ctx* c, const double g[3], int flags, int nsteps) {
+  if (!c) return COUPLB_EINVAL;
+  for (int s = 0; s < nsteps; s++) {
+    int rc = step_impl(c, g, flags, false);
+    if (rc) return rc;
+  }
+  return COUPLB_OK;
+}
+
+int couplb_ibm_substep_device(couplb_ctx* c, int n, const double* x, const double* v,
+                              const double* m, double* f, const couplb_ibm_args* a) {
+  if (!c) return COUPLB_EINVAL;
+  int rc = ibm_impl(c, n, x, v, m, f, 0, a);
+  if (rc) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  float ms = 0;
+  HIPCHK(c, hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
+  c->last_ms[1] = ms;
+  return COUPLB_OK;
+}
+
+int couplb_ibm_substep(couplb_ctx* c, int n, const double* x, const double* v, const double* m,
+                       double* f, const couplb_ibm_args* a) {
+  if (!c) return COUPLB_EINVAL;
+  if (n <= 0) return COUPLB_OK;
+  int rc = ensure_ibm(c, n);
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpyAsync(c->dx, x, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->dv, v, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->dm, m, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  rc = ibm_impl(c, n, c->dx, c->dv, c->dm, c->df, 1, a);
+  if (rc) return rc;
+  std::vector<double> F((size_t)3 * n);
+  HIPCHK(c, hipMemcpyAsync(F.data(), c->df, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (size_t i = 0; i < (size_t)3 * n; i++) f[i] += F[i];  // f[i] += F (fix_couplb.cpp:783-785)
+  float ms = 0;
+  HIPCHK(c, hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
+  c->last_ms[1] = ms;
+  return COUPLB_OK;
+}
+
+int couplb_refresh_macro(couplb_ctx* c) {
+  if (!c) return COUPLB_EINVAL;
+  c->rho_fresh = 1;
+  return COUPLB_OK;
+}
+
+int couplb_get_macro(couplb_ctx* c, double* rho, double* ux, double* uy, double* uz) {
+  if (!c) return COUPLB_EINVAL;
+  int rc = run_macro(c, true);
+  if (rc) return rc;
+  double* dsts[4] = {rho, ux, uy, uz};
+  for (int f = 0; f < 4; f++)
+    if (dsts[f])
+      HIPCHK(c, hipMemcpyAsync(dsts[f], c->mac + f * c->N, sizeof(double) * c->N,
+                               hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return COUPLB_OK;
+}
+
+int couplb_diagnostics(couplb_ctx* c, double out[6]) {
+  if (!c || !out) return COUPLB_EINVAL;
+  int rc = run_macro(c, false);
+  if (rc) return rc;
+  double h[6];
+  HIPCHK(c, hipMemcpyAsync(h, c->red, sizeof(double) * 6, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  out[0] = h[0]; out[1] = h[1]; out[2] = h[2]; out[3] = h[3];
+  out[4] = std::sqrt(h[4]);
+  out[5] = h[5];
+  return COUPLB_OK;
+}
+
+int couplb_profile_y(couplb_ctx* c, double* rs, double* us, double* vs) {
+  if (!c) return COUPLB_EINVAL;
+  int rc = run_macro(c, false);
+  if (rc) return rc;
+  std::vector<double> h(3 * (size_t)c->geo.ny);
+  HIPCHK(c, hipMemcpyAsync(h.data(), c->red + 6, sizeof(double) * h.size(), hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int j = 0; j < c->geo.ny; j++) {
+    if (rs) rs[j] = h[3 * j];
+    if (us) us[j] = h[3 * j + 1];
+    if (vs) vs[j] = h[3 * j + 2];
+  }
+  return COUPLB_OK;
+}
+
+int couplb_synchronize(couplb_ctx* c) {
+  if (!c) return COUPLB_EINVAL;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return COUPLB_OK;
+}
+
+void* couplb_stream(couplb_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+double couplb_last_kernel_ms(const couplb_ctx* c, int which) {
+  return (c && which >= 0 && which < 2) ? c->last_ms[which] : -1.0;
+}
+
+}  // extern "C"

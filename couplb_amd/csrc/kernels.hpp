@@ -1,0 +1,499 @@
+// kernels.hpp -- HIP kernels of the LB/IBM step for CDNA4 (gfx950).
+//
+// Device layout (interior nodes only, no ghost planes):
+//   populations  f[q*pitch + node], node = i + nx*(j + ny*k), x fastest;
+//                two buffers, ping-ponged by the fused step.
+//   The stored state is POST-COLLISION (f* of the last step) unless the
+//   context is in "local" mode (right after create/import), when the buffer
+//   holds the POST-STREAM populations the reference keeps between steps.
+//   Periodic faces are index wraps; wall faces are coordinate tests; the
+//   reference's ghost planes (couplb_lattice.h:82-104) are never stored.
+#pragma once
+#include "lattice.hpp"
+
+namespace couplb {
+
+constexpr int MAX_SLOTS = 16;
+constexpr double RHO_MIN = 1e-10;     // couplb_lattice.h:14
+constexpr double DELTA_TOL = 1e-12;   // couplb_lattice.h:15
+
+// Face configuration: type per face and the velocity of type-2 faces.
+struct Walls {
+  int type[3][2];
+  double vel[3][2][3];
+};
+
+// One IBM marker contribution record, written by k_marker and read by the
+// fused step's force gather.  The weights are the 1-D delta values at the
+// four stencil offsets -1..+2 of the base cell (i0, j0, k0) in the
+// reference's ghost-inclusive coordinates (couplb_ibm.h:57-73).
+struct MarkerRec {
+  double wx[4], wy[4], wz[4];
+  double sc;             // 1 / sum of fluid-node weights (couplb_ibm.h:123)
+  double fx, fy, fz;     // -F*inv_N/force_scale (fix_couplb.cpp:787-792)
+  int i0, j0, k0, valid;
+};
+
+struct Slot {
+  const int* rowstart;   // [nrows+1] into rec
+  const MarkerRec* rec;  // sorted by base row
+};
+
+struct Geo {
+  int nx, ny, nz;        // interior extents
+  int D;
+  int nry;               // row-key stride in y: gy + 3 = ny + 5
+  int nrows;             // number of row keys
+};
+
+// ---------------------------------------------------------------------------
+// Ghost-inclusive node classification (couplb_boundary.h:12-35 with the
+// x -> y -> z override order): returns the type of ghost-inclusive node
+// (a, b, c); fills the wall dimension and side when it is a wall.
+// ---------------------------------------------------------------------------
+__device__ inline int node_type(const Walls& w, const Geo& g, int a, int b, int c,
+                                int& wd, int& ws) {
+  int t = 0;
+  wd = -1; ws = 0;
+  if (a == 0 && w.type[0][0] > 0) { t = w.type[0][0]; wd = 0; ws = 0; }
+  if (a == g.nx + 1 && w.type[0][1] > 0) { t = w.type[0][1]; wd = 0; ws = 1; }
+  if (b == 0 && w.type[1][0] > 0) { t = w.type[1][0]; wd = 1; ws = 0; }
+  if (b == g.ny + 1 && w.type[1][1] > 0) { t = w.type[1][1]; wd = 1; ws = 1; }
+  if (g.D == 3) {
+    if (c == 0 && w.type[2][0] > 0) { t = w.type[2][0]; wd = 2; ws = 0; }
+    if (c == g.nz + 1 && w.type[2][1] > 0) { t = w.type[2][1]; wd = 2; ws = 1; }
+  }
+  return t;
+}
+
+// ghost-inclusive index (1..n interior, 0 / n+1 ghosts) -> interior 0-based
+// index of the periodic image
+__device__ inline int wrap_ghost(int a, int n) {
+  return a == 0 ? n - 1 : (a == n + 1 ? 0 : a - 1);
+}
+
+// Images of interior coordinate I (ghost-inclusive 1..n) in the ghost-
+// inclusive frame: itself, plus the ghost plane that the self-periodic fold
+// (couplb_streaming.h:217-225: ghost_lo -> hi face, ghost_hi -> lo face)
+// maps onto it.  Faces carrying a wall never receive spread force.
+__device__ inline int images(int I, int n, int tlo, int thi, int out[3]) {
+  int c = 0;
+  out[c++] = I;
+  if (I == n && tlo == 0) out[c++] = 0;
+  if (I == 1 && thi == 0) out[c++] = n + 1;
+  return c;
+}
+
+// ---------------------------------------------------------------------------
+// Visit every marker record of one slot whose 4^D stencil contains an image
+// of interior node row (J, K) (ghost-inclusive).  fn(rec, u, v) receives the
+// record and the y/z stencil offsets of that image; the caller tests x.
+// Rows are visited in a fixed order, so any reduction done by fn is
+// deterministic.
+// ---------------------------------------------------------------------------
+template <int D, typename SlotT, typename Fn>
+__device__ inline void visit_row(const SlotT& s, const Geo& g, const Walls& w,
+                                 int J, int K, Fn&& fn) {
+  int by[3], bz[3];
+  const int nb = images(J, g.ny, w.type[1][0], w.type[1][1], by);
+  int nc = 1;
+  bz[0] = 0;
+  if (D == 3) nc = images(K, g.nz, w.type[2][0], w.type[2][1], bz);
+  for (int ib = 0; ib < nb; ib++) {
+    for (int ic = 0; ic < nc; ic++) {
+      const int b = by[ib], c = bz[ic];
+      const int klo = (D == 3) ? c - 2 : 0, khi = (D == 3) ? c + 1 : 0;
+      for (int k0 = klo; k0 <= khi; k0++) {
+        for (int j0 = b - 2; j0 <= b + 1; j0++) {
+          const int row = (j0 + 2) + g.nry * ((D == 3) ? (k0 + 2) : 0);
+          const int r0 = s.rowstart[row], r1 = s.rowstart[row + 1];
+          for (int r = r0; r < r1; r++) {
+            const auto& m = s.rec[r];
+            if (!m.valid) continue;
+            fn(m, b - j0 + 1, (D == 3) ? (c - k0 + 1) : 0);
+          }
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Pull of the 19 (9) populations arriving at interior node (i, j, k).
+// Replaces Streaming::stream (couplb_streaming.h:93-132) applied to the
+// post-collision populations src.  Rules per source node type:
+//   0 (fluid or periodic image) f[q][s]; 1 no-slip f[opp q][n];
+//   2 moving   f[opp q][n] + 2 w rho_w (e_q . u_wall)/cs^2,
+//              rho_w = max(rho_coll[n], rho_floor);
+//   3 free-slip f[reflect_d q][n]; 4 open f[q][n].
+// ---------------------------------------------------------------------------
+struct PullCtx {
+  const void* src;
+  size_t pitch;
+  Geo g;
+  Walls w;
+  const double* rho_coll;  // density of the collision that made src
+  double rho_floor;
+  int local;               // src already holds post-stream populations
+  double base;             // stored offset per unit weight: rho0 for the
+                           // deviation formats, 0 for FP64
+  double rho0;
+  float scale, inv_scale;  // storage scale of reduced formats
+};
+
+template <int D, typename S>
+__device__ inline void pull_node(const PullCtx& pc, int i, int j, int k,
+                                 typename Store<S>::T (&f)[Lat<D>::Q]) {
+  using L = Lat<D>;
+  using T = typename Store<S>::T;
+  constexpr int Q = L::Q;
+  const S* src = static_cast<const S*>(pc.src);
+  const int nx = pc.g.nx, ny = pc.g.ny, nz = pc.g.nz;
+  const unsigned node = (unsigned)i + (unsigned)nx * ((unsigned)j + (unsigned)ny * (unsigned)k);
+  if (pc.local) {
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+      f[q] = Store<S>::load(src + q * pc.pitch + node);
+      if (Store<S>::deviation) f[q] *= (T)pc.inv_scale;
+    }
+    return;
+  }
+  // block-uniform y/z neighbour rows (wrapped) and wall flags
+  const int jm = (j == 0) ? ny - 1 : j - 1;       // source row for ey = +1
+  const int jp = (j == ny - 1) ? 0 : j + 1;       // source row for ey = -1
+  const bool ylo = (j == 0) && pc.w.type[1][0] > 0;
+  const bool yhi = (j == ny - 1) && pc.w.type[1][1] > 0;
+  int km = 0, kp = 0;
+  bool zlo = false, zhi = false;
+  if (D == 3) {
+    km = (k == 0) ? nz - 1 : k - 1;
+    kp = (k == nz - 1) ? 0 : k + 1;
+    zlo = (k == 0) && pc.w.type[2][0] > 0;
+    zhi = (k == nz - 1) && pc.w.type[2][1] > 0;
+  }
+  const int im = (i == 0) ? nx - 1 : i - 1;
+  const int ip = (i == nx - 1) ? 0 : i + 1;
+  const bool xlo = (i == 0) && pc.w.type[0][0] > 0;
+  const bool xhi = (i == nx - 1) && pc.w.type[0][1] > 0;
+  const bool any_moving = (xlo && pc.w.type[0][0] == 2) || (xhi && pc.w.type[0][1] == 2) ||
+                          (ylo && pc.w.type[1][0] == 2) || (yhi && pc.w.type[1][1] == 2) ||
+                          (zlo && pc.w.type[2][0] == 2) || (zhi && pc.w.type[2][1] == 2);
+  T rw = 0;
+  if (any_moving) {
+    const double r = pc.rho_coll[node];
+    rw = (T)((r < pc.rho_floor) ? pc.rho_floor : r);
+  }
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    const int ex = L::ex(q), ey = L::ey(q), ez = L::ez(q);
+    // wall lookup with z > y > x priority (later set_walls_* override)
+    int wt = 0, wd = 0, wsd = 0;
+    if (ex == 1 && xlo) { wt = pc.w.type[0][0]; wd = 0; wsd = 0; }
+    if (ex == -1 && xhi) { wt = pc.w.type[0][1]; wd = 0; wsd = 1; }
+    if (ey == 1 && ylo) { wt = pc.w.type[1][0]; wd = 1; wsd = 0; }
+    if (ey == -1 && yhi) { wt = pc.w.type[1][1]; wd = 1; wsd = 1; }
+    if (D == 3) {
+      if (ez == 1 && zlo) { wt = pc.w.type[2][0]; wd = 2; wsd = 0; }
+      if (ez == -1 && zhi) { wt = pc.w.type[2][1]; wd = 2; wsd = 1; }
+    }
+    const int si = ex == 1 ? im : (ex == -1 ? ip : i);
+    const int sj = ey == 1 ? jm : (ey == -1 ? jp : j);
+    const int sk = (D == 3) ? (ez == 1 ? km : (ez == -1 ? kp : k)) : 0;
+    if (wt == 0) {
+      const unsigned addr = (unsigned)si + (unsigned)nx * ((unsigned)sj + (unsigned)ny * (unsigned)sk);
+      f[q] = Store<S>::load(src + q * pc.pitch + addr);
+    } else {
+      int qs = q;
+      if (wt == 1 || wt == 2) qs = L::opp(q);
+      else if (wt == 3) qs = (wd == 0) ? L::reflect(0, q) : (wd == 1 ? L::reflect(1, q) : L::reflect(2, q));
+      f[q] = Store<S>::load(src + qs * pc.pitch + node);
+      if (wt == 2) {
+        const double* uw = pc.w.vel[wd][wsd];
+        const T eu = (T)(ex * uw[0] + ey * uw[1] + ez * uw[2]);
+        // 2 w rho_w eu / cs^2 with cs^2 = 1/3; stored values carry `scale`
+        T mv = (T)(6.0 * L::w(L::opp(q))) * rw * eu;
+        if (Store<S>::deviation) mv *= (T)pc.scale;
+        f[q] += mv;
+      }
+    }
+    if (Store<S>::deviation) f[q] *= (T)pc.inv_scale;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// The fused step: pull (stream of the previous step) -> IBM force gather ->
+// gravity -> macroscopic with Guo half-force -> BGK + Guo collision -> store.
+// Replaces do_lbm_step (fix_couplb.cpp:659-669): apply_external_force
+// (680-708), compute_macroscopic(true) (couplb_lattice.h:155-202),
+// BGK::collide (couplb_collision.h:18-62), exchange + stream
+// (couplb_streaming.h:82-139) and clear_forces (couplb_lattice.h:243-247).
+// ---------------------------------------------------------------------------
+struct StepArgs {
+  PullCtx pc;
+  void* dst;
+  double omega;
+  double g[3];
+  int grav_mode;        // 0 none, 1 current density, 2 density array (stale)
+  int write_rho;        // 1: store the collision density at every node
+  int has_moving;       // store the density at nodes next to type-2 faces
+  double* rho_out;      // may alias pc.rho_coll (same node read then written)
+  int nslots;
+  Slot slots[MAX_SLOTS];
+};
+
+template <int D, typename S>
+__global__ void __launch_bounds__(256) k_step(StepArgs a) {
+  using L = Lat<D>;
+  using T = typename Store<S>::T;
+  constexpr int Q = L::Q;
+  const Geo& g = a.pc.g;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y, k = blockIdx.z;
+  if (i >= g.nx) return;
+  const unsigned node = (unsigned)i + (unsigned)g.nx * ((unsigned)j + (unsigned)g.ny * (unsigned)k);
+
+  // IBM force (spread by the marker records of the last cycle)
+  T Fx = 0, Fy = 0, Fz = 0;
+  if (a.nslots > 0) {
+    double ax = 0, ay = 0, az = 0;
+    int xa[3];
+    const int na = images(i + 1, g.nx, a.pc.w.type[0][0], a.pc.w.type[0][1], xa);
+    for (int s = 0; s < a.nslots; s++) {
+      visit_row<D>(a.slots[s], g, a.pc.w, j + 1, k + 1,
+                   [&](const MarkerRec& m, int u, int v) {
+#pragma clang fp contract(off)
+                     const double wyz_u = m.wy[u];
+                     const double wz_v = (D == 3) ? m.wz[v] : 1.0;
+                     for (int ia = 0; ia < na; ia++) {
+                       const int t = xa[ia] - m.i0 + 1;
+                       if (t < 0 || t > 3) continue;
+                       const double d = (D == 3) ? (m.wx[t] * wyz_u) * wz_v : m.wx[t] * wyz_u;
+                       const double wgt = d * m.sc;
+                       ax += m.fx * wgt;
+                       ay += m.fy * wgt;
+                       az += m.fz * wgt;
+                     }
+                   });
+    }
+    Fx = (T)ax; Fy = (T)ay; Fz = (T)az;
+  }
+
+  T f[Q];
+  pull_node<D, S>(a.pc, i, j, k, f);
+
+  // density (deviation form for the reduced formats: A = rho - base)
+  T A = 0, jx = 0, jy = 0, jz = 0;
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    A += f[q];
+    if (L::ex(q) == 1) jx += f[q]; else if (L::ex(q) == -1) jx -= f[q];
+    if (L::ey(q) == 1) jy += f[q]; else if (L::ey(q) == -1) jy -= f[q];
+    if (D == 3) { if (L::ez(q) == 1) jz += f[q]; else if (L::ez(q) == -1) jz -= f[q]; }
+  }
+  const T base = (T)a.pc.base;
+  T rho = A + base;
+  if (!(rho > (T)RHO_MIN)) {  // also catches NaN (couplb_lattice.h:178-186)
+    rho = (T)RHO_MIN;
+    A = rho - base;
+  }
+  // gravity: F += g * rho_array (fix_couplb.cpp:706)
+  if (a.grav_mode) {
+    const T rg = (a.grav_mode == 1) ? rho : (T)a.pc.rho_coll[node];
+    Fx += (T)a.g[0] * rg;
+    Fy += (T)a.g[1] * rg;
+    Fz += (T)a.g[2] * rg;
+  }
+  if (a.write_rho) a.rho_out[node] = (double)rho;
+  else if (a.has_moving) {
+    const bool wall_adj =
+        (i == 0 && a.pc.w.type[0][0] == 2) || (i == g.nx - 1 && a.pc.w.type[0][1] == 2) ||
+        (j == 0 && a.pc.w.type[1][0] == 2) || (j == g.ny - 1 && a.pc.w.type[1][1] == 2) ||
+        (D == 3 && ((k == 0 && a.pc.w.type[2][0] == 2) || (k == g.nz - 1 && a.pc.w.type[2][1] == 2)));
+    if (wall_adj) a.rho_out[node] = (double)rho;
+  }
+  const T inv_r = (T)1 / rho;
+  const T ux = (jx + (T)0.5 * Fx) * inv_r;
+  const T uy = (jy + (T)0.5 * Fy) * inv_r;
+  const T uz = (jz + (T)0.5 * Fz) * inv_r;
+  const T om = (T)a.omega;
+  const T hom = (T)1 - (T)0.5 * om;
+  const T uu = ux * ux + uy * uy + uz * uz;
+  const T uf = ux * Fx + uy * Fy + uz * Fz;
+  S* dst = static_cast<S*>(a.dst);
+  const T scale = (T)a.pc.scale;
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    const T cx = (T)L::ex(q), cy = (T)L::ey(q), cz = (T)L::ez(q);
+    const T wq = (T)L::w(q);
+    const T eu = cx * ux + cy * uy + cz * uz;
+    const T ef = cx * Fx + cy * Fy + cz * Fz;
+    // feq - w*base = w*(A + rho*(3eu + 4.5eu^2 - 1.5uu))   (couplb_lattice.h:137-144)
+    const T feq = wq * (A + rho * ((T)3 * eu + (T)4.5 * eu * eu - (T)1.5 * uu));
+    // Guo source w(1-om/2)[(e.F - u.F)/cs^2 + (e.u)(e.F)/cs^4]  (couplb_collision.h:51-52)
+    const T Si = wq * hom * ((T)3 * (ef - uf) + (T)9 * eu * ef);
+    T fn = f[q] - om * (f[q] - feq) + Si;
+    if (Store<S>::deviation) fn *= scale;
+    Store<S>::store(dst + q * a.pc.pitch + node, fn);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Macroscopic fields of the current state (compute_macroscopic(false),
+// couplb_lattice.h:155-202) with FP64 block partials for the diagnostics
+// (couplb_lattice.h:203-242) and the y-profile (fix_couplb.cpp:903-928).
+// One block per (row j, plane k); blockDim.x covers nx in chunks.
+// partial layout per (j,k): [mass, momx, momy, momz, umax2, clamps,
+//                             sum_rho, sum_ux, sum_uy]
+// ---------------------------------------------------------------------------
+struct MacroArgs {
+  PullCtx pc;
+  double* rho;  // optional outputs (interior layout), may be null
+  double* ux;
+  double* uy;
+  double* uz;
+  double* partial;  // [ny*nz][9]
+};
+
+template <int D, typename S>
+__global__ void __launch_bounds__(256) k_macro(MacroArgs a) {
+  using L = Lat<D>;
+  using T = typename Store<S>::T;
+  constexpr int Q = L::Q;
+  const Geo& g = a.pc.g;
+  const int j = blockIdx.y, k = blockIdx.z;
+  __shared__ double red[9][256];
+  double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int i = threadIdx.x; i < g.nx; i += blockDim.x) {
+    T f[Q];
+    pull_node<D, S>(a.pc, i, j, k, f);
+    double A = 0, jx = 0, jy = 0, jz = 0;
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+      const double fv = (double)f[q];
+      A += fv;
+      jx += L::ex(q) * fv;
+      jy += L::ey(q) * fv;
+      jz += L::ez(q) * fv;
+    }
+    double r = A + a.pc.base;
+    int cl = 0;
+    if (r <= RHO_MIN) { r = RHO_MIN; cl = 1; }
+    if (!isfinite(r)) { r = RHO_MIN; cl = 1; }
+    const double inv_r = 1.0 / r;
+    const double u0 = jx * inv_r, u1 = jy * inv_r, u2 = jz * inv_r;
+    const unsigned node = (unsigned)i + (unsigned)g.nx * ((unsigned)j + (unsigned)g.ny * (unsigned)k);
+    if (a.rho) { a.rho[node] = r; a.ux[node] = u0; a.uy[node] = u1; a.uz[node] = u2; }
+    acc[0] += r;
+    acc[1] += r * u0;
+    acc[2] += r * u1;
+    acc[3] += r * u2;
+    const double u2s = u0 * u0 + u1 * u1 + u2 * u2;
+    acc[4] = acc[4] > u2s ? acc[4] : u2s;
+    acc[5] += cl;
+    acc[6] += r;
+    acc[7] += u0;
+    acc[8] += u1;
+  }
+  for (int c = 0; c < 9; c++) red[c][threadIdx.x] = acc[c];
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      for (int c = 0; c < 9; c++) {
+        if (c == 4) red[c][threadIdx.x] = fmax(red[c][threadIdx.x], red[c][threadIdx.x + s]);
+        else red[c][threadIdx.x] += red[c][threadIdx.x + s];
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 9) {
+    a.partial[((size_t)k * g.ny + j) * 9 + threadIdx.x] = red[threadIdx.x][0];
+  }
+}
+
+// Deterministic second stage: out[0..5] global sums, prof[3*ny] y-profile.
+// One block; fixed summation order.
+__global__ void __launch_bounds__(256) k_macro_reduce(const double* partial, int ny, int nz,
+                                                      double* out, double* prof) {
+  __shared__ double red[6][256];
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  const int nrow = ny * nz;
+  for (int r = threadIdx.x; r < nrow; r += blockDim.x) {
+    const double* p = partial + (size_t)r * 9;
+    for (int c = 0; c < 6; c++) {
+      if (c == 4) acc[c] = fmax(acc[c], p[c]);
+      else acc[c] += p[c];
+    }
+  }
+  for (int c = 0; c < 6; c++) red[c][threadIdx.x] = acc[c];
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+      for (int c = 0; c < 6; c++) {
+        if (c == 4) red[c][threadIdx.x] = fmax(red[c][threadIdx.x], red[c][threadIdx.x + s]);
+        else red[c][threadIdx.x] += red[c][threadIdx.x + s];
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) out[threadIdx.x] = red[threadIdx.x][0];
+  for (int j = threadIdx.x; j < ny; j += blockDim.x) {
+    double sr = 0, su = 0, sv = 0;
+    for (int k = 0; k < nz; k++) {
+      const double* p = partial + ((size_t)k * ny + j) * 9;
+      sr += p[6]; su += p[7]; sv += p[8];
+    }
+    prof[3 * j + 0] = sr; prof[3 * j + 1] = su; prof[3 * j + 2] = sv;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// State import/export in the reference's ghost-inclusive layout.
+// ---------------------------------------------------------------------------
+template <int D, typename S>
+__global__ void k_import(const double* fin, void* dstv, size_t pitch, Geo g, double base_w_rho0,
+                         float scale) {
+  using L = Lat<D>;
+  constexpr int Q = L::Q;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y, k = blockIdx.z;
+  if (i >= g.nx) return;
+  const int gx = g.nx + 2, gy = g.ny + 2, gz = (D == 3) ? g.nz + 2 : 1;
+  const size_t nt = (size_t)gx * gy * gz;
+  const size_t gn = (size_t)(i + 1) + (size_t)gx * ((size_t)(j + 1) + (size_t)gy * (D == 3 ? (size_t)(k + 1) : 0));
+  const unsigned node = (unsigned)i + (unsigned)g.nx * ((unsigned)j + (unsigned)g.ny * (unsigned)k);
+  S* dst = static_cast<S*>(dstv);
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    double v = fin[q * nt + gn];
+    if (Store<S>::deviation) v = (v - L::w(q) * base_w_rho0) * scale;
+    Store<S>::store(dst + q * pitch + node, (typename Store<S>::T)v);
+  }
+}
+
+template <int D, typename S>
+__global__ void k_export(PullCtx pc, double* fout) {
+  using L = Lat<D>;
+  using T = typename Store<S>::T;
+  constexpr int Q = L::Q;
+  const Geo& g = pc.g;
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;  // ghost-inclusive
+  const int b = blockIdx.y, c = blockIdx.z;
+  const int gx = g.nx + 2, gy = g.ny + 2, gz = (D == 3) ? g.nz + 2 : 1;
+  if (a >= gx) return;
+  const size_t nt = (size_t)gx * gy * gz;
+  const size_t gn = (size_t)a + (size_t)gx * ((size_t)b + (size_t)gy * (size_t)c);
+  int wd, wsd;
+  const int t = node_type(pc.w, g, a, b, (D == 3) ? c : 1, wd, wsd);
+  if (t != 0) {
+#pragma unroll
+    for (int q = 0; q < Q; q++) fout[q * nt + gn] = L::w(q) * pc.rho0;
+    return;
+  }
+  const int i = wrap_ghost(a, g.nx), j = wrap_ghost(b, g.ny);
+  const int k = (D == 3) ? wrap_ghost(c, g.nz) : 0;
+  T f[Q];
+  pull_node<D, S>(pc, i, j, k, f);
+#pragma unroll
+  for (int q = 0; q < Q; q++) fout[q * nt + gn] = (double)f[q] + L::w(q) * pc.base;
+}
+
+}  // namespace couplb

@@ -1,0 +1,122 @@
+// lattice.hpp -- D2Q9 / D3Q19 velocity sets as compile-time functions, and
+// the population storage formats.
+//
+// The velocity order, weights, opposite and free-slip reflection tables are
+// the reference's (CoupLB/couplb_lattice.h:24-75): populations are exchanged
+// with the host in that index order, so the tables have to be identical.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace couplb {
+
+template <int D> struct Lat;
+
+template <> struct Lat<2> {
+  static constexpr int Q = 9;
+  __host__ __device__ static constexpr int ex(int q) {
+    constexpr int t[9] = {0, 1, -1, 0, 0, 1, -1, 1, -1};
+    return t[q];
+  }
+  __host__ __device__ static constexpr int ey(int q) {
+    constexpr int t[9] = {0, 0, 0, 1, -1, 1, -1, -1, 1};
+    return t[q];
+  }
+  __host__ __device__ static constexpr int ez(int) { return 0; }
+  __host__ __device__ static constexpr double w(int q) {
+    return q == 0 ? 4.0 / 9.0 : (q < 5 ? 1.0 / 9.0 : 1.0 / 36.0);
+  }
+  __host__ __device__ static constexpr int opp(int q) {
+    constexpr int t[9] = {0, 2, 1, 4, 3, 6, 5, 8, 7};
+    return t[q];
+  }
+  // index with component d of e_q negated (free-slip specular reflection)
+  __host__ __device__ static constexpr int reflect(int d, int q) {
+    constexpr int t[3][9] = {{0, 2, 1, 3, 4, 8, 7, 6, 5},
+                             {0, 1, 2, 4, 3, 7, 8, 5, 6},
+                             {0, 1, 2, 3, 4, 5, 6, 7, 8}};
+    return t[d][q];
+  }
+};
+
+template <> struct Lat<3> {
+  static constexpr int Q = 19;
+  __host__ __device__ static constexpr int ex(int q) {
+    constexpr int t[19] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
+    return t[q];
+  }
+  __host__ __device__ static constexpr int ey(int q) {
+    constexpr int t[19] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1};
+    return t[q];
+  }
+  __host__ __device__ static constexpr int ez(int q) {
+    constexpr int t[19] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1};
+    return t[q];
+  }
+  __host__ __device__ static constexpr double w(int q) {
+    return q == 0 ? 1.0 / 3.0 : (q < 7 ? 1.0 / 18.0 : 1.0 / 36.0);
+  }
+  __host__ __device__ static constexpr int opp(int q) {
+    constexpr int t[19] = {0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17};
+    return t[q];
+  }
+  __host__ __device__ static constexpr int reflect(int d, int q) {
+    constexpr int t[3][19] = {
+        {0, 2, 1, 3, 4, 5, 6, 10, 9, 8, 7, 14, 13, 12, 11, 15, 16, 17, 18},
+        {0, 1, 2, 4, 3, 5, 6, 9, 10, 7, 8, 11, 12, 13, 14, 18, 17, 16, 15},
+        {0, 1, 2, 3, 4, 6, 5, 7, 8, 9, 10, 13, 14, 11, 12, 17, 18, 15, 16}};
+    return t[d][q];
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Storage formats.  FP64 stores f_i itself (the reference's representation).
+// The reduced formats store the deviation (f_i - w_i*rho0)*scale and compute
+// in float; scale is a power of two (exact) that lifts small deviations out
+// of the half-precision subnormal range.
+// ---------------------------------------------------------------------------
+struct bf16_t { uint16_t bits; };
+
+template <typename S> struct Store;
+
+template <> struct Store<double> {
+  using T = double;                    // arithmetic type
+  static constexpr bool deviation = false;
+  __device__ static inline T load(const double* p) { return __builtin_nontemporal_load(p); }
+  __device__ static inline void store(double* p, T v) { __builtin_nontemporal_store(v, p); }
+};
+
+template <> struct Store<float> {
+  using T = float;
+  static constexpr bool deviation = true;
+  __device__ static inline T load(const float* p) { return __builtin_nontemporal_load(p); }
+  __device__ static inline void store(float* p, T v) { __builtin_nontemporal_store(v, p); }
+};
+
+template <> struct Store<_Float16> {
+  using T = float;
+  static constexpr bool deviation = true;
+  __device__ static inline T load(const _Float16* p) {
+    return (float)__builtin_nontemporal_load(p);
+  }
+  __device__ static inline void store(_Float16* p, T v) {
+    __builtin_nontemporal_store((_Float16)v, p);
+  }
+};
+
+template <> struct Store<bf16_t> {
+  using T = float;
+  static constexpr bool deviation = true;
+  __device__ static inline T load(const bf16_t* p) {
+    const uint16_t b = __builtin_nontemporal_load(&p->bits);
+    return __uint_as_float((uint32_t)b << 16);
+  }
+  // round to nearest even; inputs are finite in normal operation
+  __device__ static inline void store(bf16_t* p, T v) {
+    const uint32_t u = __float_as_uint(v);
+    const uint32_t r = u + 0x7FFFu + ((u >> 16) & 1u);
+    __builtin_nontemporal_store((uint16_t)(r >> 16), &p->bits);
+  }
+};
+
+}  // namespace couplb

@@ -1,0 +1,95 @@
+// fix_couplb_core.hpp -- host-side FixCoupLB on the MI355X device context.
+//
+// The class keeps the reference fix's interface and behaviour
+// (CoupLB/fix_couplb.h:22-30, fix_couplb.cpp) -- same argument syntax,
+// validation messages, unit conversion, callback order and sub-stepping --
+// while every grid operation goes through the C ABI in couplb_hip.h.
+#pragma once
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/couplb_fix.h"
+#include "../../include/couplb_hip.h"
+
+namespace couplb_host {
+
+// error->all / error->one (LAMMPS aborts; here the C boundary reports it)
+struct fix_error : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+class FixCoupLBCore {
+ public:
+  FixCoupLBCore(int narg, const char* const* arg, const couplb_domain& dom);
+  ~FixCoupLBCore();
+  static int setmask();
+  void init();
+  void setup(const couplb_atoms& a);
+  void post_force(const couplb_atoms& a);
+  int end_of_step(long step);
+
+  void set_variable_callback(couplb_var_eval fn, void* user) { var_fn = fn; var_user = user; }
+  double scalar(const std::string& name) const;
+  const double* last_diag() const { return diag; }
+  bool last_profile(double* r, double* u, double* v) const;
+  couplb_ctx* context() { return ctx; }
+  const std::string& log() const { return screen; }
+
+ private:
+  enum { NONE_G, CONSTANT_G, EQUAL_G };
+  couplb_domain dom;
+  int Nx, Ny, Nz;
+  double nu, rho;
+  int md_per_lb = 1, md_sub_count = 0;
+  double xi_ibm = 1.0;
+  double gext[3] = {0, 0, 0};
+  int gstyle[3] = {NONE_G, NONE_G, NONE_G};
+  std::string gvar[3];
+  int wall[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+  double wall_vel[3] = {0, 0, 0};
+  bool face_vel_set[3][2] = {{false, false}, {false, false}, {false, false}};
+  double face_vel[3][2][3] = {};
+  int output_every = 0;
+  std::string output_file = "couplb_profile.dat";
+  int check_every = 0;
+  int vtk_every = 0;
+  int checkpoint_every = 0;
+  std::string checkpoint_prefix = "couplb_ckpt";
+  std::string restart_prefix;
+  bool do_restart = false;
+  int kernel = COUPLB_ROMA;
+  int storage = COUPLB_FP64;
+  bool is3d = true;
+
+  double dx = 0, inv_dx = 0, dt_LBM = 0, vel_scale = 0, nu_lb = 0, tau = 0, force_scale = 0;
+  long next_output = 0, next_check = -1, next_checkpoint = -1, next_vtk = -1;
+  long lbm_step_count = 0;
+  bool ibm_has_particles = false;
+  couplb_ctx* ctx = nullptr;
+  couplb_var_eval var_fn = nullptr;
+  void* var_user = nullptr;
+
+  double diag[6] = {0, 0, 0, 0, 0, 0};
+  std::vector<double> prof_r, prof_u, prof_v;
+  bool have_profile = false;
+  std::string screen;
+
+  std::vector<double> px, pv, pm, pf;
+  std::vector<int> pidx;
+
+  void say(const char* fmt, ...);
+  void check(int rc, const char* what);
+  void update_gravity_variables();
+  void lbm_step(int ng);
+  int count_group(const couplb_atoms& a) const;
+  void ibm_sub_coupling(const couplb_atoms& a, bool first);
+  void check_stability_precomputed(long step);
+  void write_profile(long step);
+  void write_checkpoint(long step);
+  void read_checkpoint();
+  bool refresh_follows_next_step() const;
+};
+
+}  // namespace couplb_host

@@ -1,0 +1,154 @@
+/*
+ * couplb_hip.h -- C ABI of the MI355X LB/IBM step (libcouplb_hip.so).
+ *
+ * This is the drop-in boundary under the LAMMPS fix: FixCoupLB's host code
+ * (couplb_amd/host/fix_couplb_core.cpp) calls these functions instead of the
+ * reference's header templates.  Each entry point names the reference code
+ * it replaces.  Plain C types only; every function returns COUPLB_OK (0) or a
+ * negative COUPLB_E* code and leaves a message for couplb_last_error().
+ *
+ * Units are lattice units (dx = dt_LB = 1), as inside the reference's Grid.
+ * One context owns one grid on one GPU and one HIP stream; calls are
+ * synchronous at return unless the name says _async.
+ */
+#ifndef COUPLB_HIP_H
+#define COUPLB_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define COUPLB_OK 0
+#define COUPLB_EINVAL -1   /* bad argument or configuration */
+#define COUPLB_EDEVICE -2  /* HIP runtime error */
+#define COUPLB_ENOMEM -3   /* device allocation failed */
+#define COUPLB_ESTATE -4   /* call not valid in the current state */
+
+/* population storage (north-star precision variants) */
+#define COUPLB_FP64 0 /* f_i, the reference's representation */
+#define COUPLB_FP32 1 /* f_i - w_i*rho0 in float, float arithmetic */
+#define COUPLB_FP16 2 /* f_i - w_i*rho0 in IEEE half, float arithmetic */
+#define COUPLB_BF16 3 /* f_i - w_i*rho0 in bfloat16, float arithmetic */
+
+/* IBM delta kernels, couplb_ibm.h:11-33 / fix_couplb.cpp:175-180 */
+#define COUPLB_ROMA 0
+#define COUPLB_PESKIN4 1
+
+/* face types, fix_couplb.cpp:49-51 */
+#define COUPLB_PERIODIC 0
+#define COUPLB_NOSLIP 1
+#define COUPLB_MOVING 2
+#define COUPLB_FREESLIP 3
+#define COUPLB_OPEN 4
+
+typedef struct couplb_ctx couplb_ctx;
+
+/* Grid configuration.  Replaces Grid<L>::allocate + init_equilibrium
+ * (couplb_lattice.h:105-154), BGK<L>::set_tau (couplb_collision.h:16),
+ * Boundary<L>::set_walls_x/y/z + set_wall_velocity (couplb_boundary.h:12-38)
+ * and Streaming<L>::allocate_buffers/precompute_wall_flags
+ * (couplb_streaming.h:54-80). */
+typedef struct couplb_params {
+  int dim;                /* 2 -> D2Q9, 3 -> D3Q19 */
+  int nx, ny, nz;         /* interior nodes; nz = 1 when dim = 2 */
+  double tau;             /* BGK relaxation time, > 0.5 */
+  int wall[3][2];         /* COUPLB_PERIODIC..COUPLB_OPEN for (lo, hi) of x, y, z */
+  double face_vel[3][2][3]; /* velocity of each type-2 face; the reference
+                               gives every face the same one */
+  int kernel;             /* COUPLB_ROMA or COUPLB_PESKIN4 */
+  int storage;            /* COUPLB_FP64 .. COUPLB_BF16 */
+  double rho0;            /* lattice reference density (fix_couplb.h:41: 1.0) */
+  int md_per_lb;          /* IBM sub-steps per LB step (sizes record slots) */
+  int device;             /* HIP device ordinal */
+} couplb_params;
+
+int couplb_create(const couplb_params* p, couplb_ctx** out);
+int couplb_destroy(couplb_ctx* ctx);
+const char* couplb_last_error(const couplb_ctx* ctx);
+const char* couplb_version(void);
+
+/* Population state in the reference's ghost-inclusive layout
+ * f[q*ntotal + n], n = i + gx*(j + gy*k), gx = nx+2, gy = ny+2,
+ * gz = nz+2 (3D) or 1 (2D) -- exactly the array couplb_io.h:653-828 writes.
+ * set: replaces read_checkpoint's `f = file; f_buf = f;
+ * compute_macroscopic(false)` (couplb_io.h:802-817).  Ghost entries are
+ * ignored on input.  get: returns the post-stream populations the reference
+ * holds between steps; periodic ghosts carry their images, wall ghosts the
+ * rest equilibrium w_q*rho0. */
+int couplb_set_state_f(couplb_ctx* ctx, const double* f_host);
+int couplb_get_state_f(couplb_ctx* ctx, double* f_host);
+long couplb_state_len(const couplb_ctx* ctx); /* Q*ntotal */
+
+/* One LB step: replaces do_lbm_step (fix_couplb.cpp:659-669), i.e.
+ * apply_external_force -> compute_macroscopic(true) -> BGK::collide ->
+ * Streaming::exchange -> Streaming::stream -> clear_forces, fused into one
+ * pull kernel.  g_lb is the lattice-unit body acceleration
+ * (fix_couplb.cpp:688-691); pass {0,0,0} for none.  IBM forces are the
+ * records of every couplb_ibm_substep since the previous step.
+ * flags: COUPLB_STEP_RHO_REFRESHED_NEXT -- the caller guarantees a
+ * compute_macroscopic(false)-equivalent call (an IBM sub-step with markers,
+ * couplb_refresh_macro, couplb_diagnostics) before the next step, so the
+ * density array need not be kept for the next step's gravity. */
+#define COUPLB_STEP_RHO_REFRESHED_NEXT 1
+int couplb_lbm_step(couplb_ctx* ctx, const double g_lb[3], int flags);
+int couplb_lbm_steps_async(couplb_ctx* ctx, const double g_lb[3], int flags, int nsteps);
+
+/* IBM sub-step: replaces the loop body of do_ibm_sub_coupling
+ * (fix_couplb.cpp:757-797): fluid velocity at the marker (IBM::interpolate,
+ * couplb_ibm.h:57-90), penalty force
+ *   F = m * xi_over_dtLB * (u_f * vel_scale - v)        (779-781)
+ * added into f, and the reaction -F*inv_N/force_scale spread onto the grid
+ * (IBM::spread, couplb_ibm.h:92-133) with the self-periodic fold
+ * (couplb_streaming.h:217-225) applied as an index wrap.  The fluid velocity
+ * is compute_macroscopic(false) of the current state, as at 757-760; it is
+ * computed once per state and reused across sub-steps.
+ * Arrays are [n][3] (x, v, f) and [n] (m) -- LAMMPS' own layout. */
+typedef struct couplb_ibm_args {
+  double lo[3];          /* domain->boxlo (fix_couplb.cpp:282) */
+  double inv_dx;         /* 1/dx */
+  double xi_over_dtLB;   /* xi_ibm / dt_LBM (fix_couplb.cpp:763) */
+  double vel_scale;      /* dx / dt_LBM */
+  double inv_N;          /* 1 / md_per_lb */
+  double force_scale;    /* rho*dx^(D+1)/dt_LBM^2 */
+  int positions_unchanged; /* 1: x equals the previous sub-step's x, so the
+                              marker ordering may be reused */
+} couplb_ibm_args;
+
+/* host arrays (copied over PCIe each call) */
+int couplb_ibm_substep(couplb_ctx* ctx, int n, const double* x, const double* v,
+                       const double* m, double* f_inout, const couplb_ibm_args* a);
+/* device arrays (already resident in HBM); f_inout is device memory */
+int couplb_ibm_substep_device(couplb_ctx* ctx, int n, const double* x, const double* v,
+                              const double* m, double* f_inout, const couplb_ibm_args* a);
+
+/* compute_macroscopic(false) semantics without copying anything out: marks
+ * the density as refreshed for the next step's gravity (fix_couplb.cpp:706,
+ * couplb_lattice.h:155-202). */
+int couplb_refresh_macro(couplb_ctx* ctx);
+
+/* Interior macroscopic fields (rho, u = j/rho) of the current state, each an
+ * nx*ny*nz array with x fastest.  Also a refresh. */
+int couplb_get_macro(couplb_ctx* ctx, double* rho, double* ux, double* uy, double* uz);
+
+/* FP64 global diagnostics: replaces Grid::max_velocity and
+ * Grid::compute_diagnostics (couplb_lattice.h:203-242) used by
+ * check_stability_precomputed (fix_couplb.cpp:818-844).
+ * out = {mass, momx, momy, momz, umax, clamp_count}.  Also a refresh. */
+int couplb_diagnostics(couplb_ctx* ctx, double out[6]);
+
+/* y-binned sums over x and z of rho, ux, uy (write_profile,
+ * fix_couplb.cpp:903-928); each array has ny entries.  Also a refresh. */
+int couplb_profile_y(couplb_ctx* ctx, double* rho_sum, double* ux_sum, double* uy_sum);
+
+int couplb_synchronize(couplb_ctx* ctx);
+/* the context's hipStream_t, for callers that time with HIP events */
+void* couplb_stream(couplb_ctx* ctx);
+
+/* Timing of the last call's kernels (milliseconds, HIP events on the
+ * context stream): which = 0 LB kernel, 1 IBM sub-step total. */
+double couplb_last_kernel_ms(const couplb_ctx* ctx, int which);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,246 @@
+#!/usr/bin/env python3
+"""D3Q19 LB step with immersed-boundary coupling on one MI355X.
+
+Workload (synthetic, seeded): D3Q19 256^3, fully periodic, nu = 0.1
+(tau = 0.8), FP64 populations, uniform initial velocity (0.02, 0, 0) with a
+constant body force g_x = 1e-6; one rigid sphere of diameter 24 dx at the box
+centre (1810 Fibonacci-lattice markers, held fixed, v = 0) plus 100000 point
+particles at uniform random positions with Gaussian velocities
+(sigma = 1e-3), all of mass 1; Peskin4 kernel, xi_ibm = 1, md_per_lb = 4.
+Positions are prescribed (held fixed), velocities constant.
+
+A step = one LB step (the fused pull kernel) + md_per_lb IBM sub-steps, with
+the marker arrays resident in HBM.  value = MLUPS = 256^3 * steps / wall
+seconds / 1e6.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+Q = 19
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def fibonacci_sphere(n, radius, centre):
+    i = np.arange(n) + 0.5
+    z = 1.0 - 2.0 * i / n
+    r = np.sqrt(1.0 - z * z)
+    phi = i * math.pi * (3.0 - math.sqrt(5.0))
+    return np.stack([r * np.cos(phi), r * np.sin(phi), z], 1) * radius + np.asarray(centre)
+
+
+def make_markers(N, n_particles, seed=2025):
+    D = 24.0
+    ns = int(round(math.pi * D * D))  # ~1 dx marker spacing: 1810
+    sphere = fibonacci_sphere(ns, D / 2, (N / 2, N / 2, N / 2))
+    rng = np.random.default_rng(seed)
+    px = rng.uniform(0.0, float(N), (n_particles, 3))
+    pv = rng.normal(0.0, 1e-3, (n_particles, 3))
+    x = np.concatenate([sphere, px]).astype(np.float64)
+    v = np.concatenate([np.zeros_like(sphere), pv]).astype(np.float64)
+    m = np.ones(len(x))
+    return x, v, m, ns
+
+
+def cpu_baseline(N, nz_sample, steps, md_per_lb):
+    """Reference-numerics LB/IBM cycle on host cores (bounded sample)."""
+    import oracle
+    kind = "reference"
+    try:
+        oracle.ops("ref")
+        numerics = "ref"
+    except OSError:
+        numerics, kind = "oracle", "port"
+    x, v, m, _ = make_markers(N, 100000)
+    keep = x[:, 2] < nz_sample                  # markers inside the slab
+    x, v, m = x[keep], v[keep], m[keep]
+    fx = oracle.Fix(N, N, nz_sample, 0.1, 1.0, md_per_lb=md_per_lb, dt=1.0 / md_per_lb,
+                    gravity=(1e-6, 0.0, 0.0), kernel=oracle.PESKIN4, numerics=numerics)
+    n = len(m)
+    atoms = dict(x=x, v=v, f=np.zeros((n, 3)), mask=np.ones(n, np.int32), mass=m)
+    fx.setup(atoms)
+    t0 = time.perf_counter()
+    for _ in range(steps * md_per_lb):
+        atoms["f"][:] = 0
+        fx.post_force(atoms)
+    dt = time.perf_counter() - t0
+    threads = oracle.omp_threads()
+    nodes = N * N * nz_sample
+    return dict(value=nodes * steps / dt / 1e6, unit="MLUPS", cores=threads, kind=kind,
+                sample=(f"D3Q19 {N}x{N}x{nz_sample} slab of the same workload, {n} markers "
+                        f"(same density), md_per_lb={md_per_lb}, {steps} LB steps, FP64, "
+                        f"reference pass structure (numerics: {numerics}), "
+                        f"OMP_NUM_THREADS={threads}, {dt:.1f} s"))
+
+
+def copy_calibration(mb, reps):
+    """Plain device-to-device copy of known size: the reference point for
+    rocprofv3 FETCH_SIZE / WRITE_SIZE and for achievable HBM bandwidth."""
+    import torch
+    n = mb * (1 << 20) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    ev1.record()
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / reps
+    print(json.dumps({"copy_mb": mb, "bytes_per_copy": 2 * n * 8, "avg_ms": ms,
+                      "copy_gbs": 2 * n * 8 / (ms * 1e-3) / 1e9}), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--n", type=int, default=256, help="grid edge (interior nodes)")
+    ap.add_argument("--particles", type=int, default=100000)
+    ap.add_argument("--storage", default="fp64", choices=["fp64", "fp32", "fp16", "bf16"])
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-nz", type=int, default=32)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
+                    help="time a plain device copy of MB megabytes (PMC byte-counter scale)")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
+              f"(asked for --gpus {args.gpus}, WORLD_SIZE={world})", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import couplb_amd
+
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration, args.steps)
+        return
+    N, md = args.n, 4
+    log(f"bench: D3Q19 {N}^3 {args.storage} IBM md_per_lb={md} particles={args.particles}")
+    dev = couplb_amd.Device(3, N, N, N, tau=0.8, kernel=couplb_amd.PESKIN4,
+                            storage=args.storage, md_per_lb=md)
+    dev.init_equilibrium(1.0, (0.02, 0.0, 0.0))
+    x, v, m, nsphere = make_markers(N, args.particles)
+    n = len(m)
+    cuda = torch.device("cuda:0")
+    tx = torch.from_numpy(x).to(cuda)
+    tv = torch.from_numpy(v).to(cuda)
+    tm = torch.from_numpy(m).to(cuda)
+    tf = torch.zeros((n, 3), dtype=torch.float64, device=cuda)
+    torch.cuda.synchronize()
+    ibm_kw = dict(lo=(0.0, 0.0, 0.0), inv_dx=1.0, xi_over_dtLB=1.0, vel_scale=1.0,
+                  inv_N=1.0 / md, force_scale=1.0)
+    g = (1e-6, 0.0, 0.0)
+
+    def ibm(first=False):
+        dev.ibm_device(n, tx.data_ptr(), tv.data_ptr(), tm.data_ptr(), tf.data_ptr(),
+                       positions_unchanged=not first, **ibm_kw)
+
+    ibm(first=True)  # FixCoupLB::setup primes the IBM force (fix_couplb.cpp:558)
+    lb_ms, ibm_ms, spread_ms = [], [], []
+
+    def one_step():
+        dev.step(g, couplb_amd.RHO_REFRESHED_NEXT)
+        lb_ms.append(dev.last_ms(0))
+        spread_ms.append(dev.last_ms(2))
+        for _ in range(md):
+            ibm()
+            ibm_ms.append(dev.last_ms(1))
+
+    for _ in range(args.warmup):
+        one_step()
+    lb_ms.clear()
+    ibm_ms.clear()
+    spread_ms.clear()
+    dev.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        one_step()
+        if (s + 1) % 10 == 0:
+            log(f"bench: step {s + 1}/{args.steps}")
+    dev.sync()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    nodes = N ** 3
+    mlups = nodes * args.steps / wall / 1e6
+    s_bytes = {"fp64": 8, "fp32": 4, "fp16": 2, "bf16": 2}[args.storage]
+    b_lup = 2 * Q * s_bytes
+    lb_avg = float(np.mean(lb_ms))
+    ibm_avg = float(np.mean(ibm_ms)) if ibm_ms else 0.0
+    spread_avg = float(np.mean(spread_ms)) if spread_ms else 0.0
+    achieved = b_lup * nodes / (lb_avg * 1e-3) / 1e9
+    # IBM time per marker sub-step: the sub-step kernels plus this sub-step's
+    # share of the force spreading that precedes the next LB kernel
+    markers_per_s = n / ((ibm_avg + spread_avg / md) * 1e-3) if n else 0.0
+    diag = dev.diagnostics()
+
+    # LB-only throughput of the same grid (no markers), for reference
+    dev2_steps = max(10, args.steps)
+    dev.sync()
+    t1 = time.perf_counter()
+    dev.steps_async(dev2_steps, g, 0)
+    dev.sync()
+    lb_only = nodes * dev2_steps / (time.perf_counter() - t1) / 1e6
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("bench: cpu baseline")
+        cpu = cpu_baseline(N, args.cpu_nz, args.cpu_steps, md)
+
+    dtype = {"fp64": "f64", "fp32": "f32 storage/f32 math", "fp16": "f16 storage/f32 math",
+             "bf16": "bf16 storage/f32 math"}[args.storage]
+    out = {
+        "metric": "D3Q19 MLUPS & %HBM peak at 256^3 w/ IBM; markers/s; err vs FP64 oracle",
+        "value": round(mlups, 2),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": dtype,
+        "data": "synthetic (seeded markers, uniform-flow initial state)",
+        "config": {"workload": f"D3Q19 {N}^3 periodic + IBM: Fibonacci sphere D=24 ({nsphere} markers, "
+                               f"fixed) + {args.particles} point particles, Peskin4, xi=1, "
+                               f"md_per_lb={md}, u0=0.02, g_x=1e-6, nu=0.1",
+                   "grid": [N, N, N], "markers": n, "md_per_lb": md, "storage": args.storage},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": f"k_step<3,{args.storage}> (fused pull + IBM force gather)",
+                     "bytes_per_lup": b_lup, "avg_kernel_ms": round(lb_avg, 4)},
+        "ibm": {"markers_per_s": round(markers_per_s, 1), "avg_substep_ms": round(ibm_avg, 4),
+                "avg_spread_ms_per_lb_step": round(spread_avg, 4), "markers": n},
+        "lb_only_mlups": round(lb_only, 1),
+        "lb_only_hbm_frac": round(lb_only * 1e6 * b_lup / (HBM_PEAK_GBS * 1e9), 4),
+        "mass": diag["mass"],
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

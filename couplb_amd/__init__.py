@@ -83,6 +83,7 @@ def lib() -> C.CDLL:
         "couplb_set_state_f": (ip, [vp, dp]),
         "couplb_get_state_f": (ip, [vp, dp]),
         "couplb_state_len": (C.c_long, [vp]),
+        "couplb_init_equilibrium": (ip, [vp, C.c_double, C.c_double, C.c_double, C.c_double]),
         "couplb_lbm_step": (ip, [vp, dp, ip]),
         "couplb_lbm_steps_async": (ip, [vp, dp, ip, ip]),
         "couplb_ibm_substep": (ip, [vp, ip, dp, dp, dp, dp, C.POINTER(IbmArgs)]),
@@ -99,6 +100,7 @@ def lib() -> C.CDLL:
         "couplb_fix_last_error": (C.c_char_p, [vp]),
         "couplb_fix_set_variable_callback": (ip, [vp, VAR_EVAL, vp]),
         "couplb_fix_init": (ip, [vp]),
+        "couplb_fix_set_domain": (ip, [vp, C.POINTER(Domain)]),
         "couplb_fix_setup": (ip, [vp, C.POINTER(Atoms)]),
         "couplb_fix_post_force": (ip, [vp, C.POINTER(Atoms)]),
         "couplb_fix_end_of_step": (ip, [vp, C.c_long]),
@@ -180,6 +182,9 @@ class Device:
         self._chk(lib().couplb_get_state_f(self.h, _p(out)), "get_state_f")
         return out.reshape(self.Q, gz, gy, gx)
 
+    def init_equilibrium(self, rho=1.0, u=(0.0, 0.0, 0.0)):
+        self._chk(lib().couplb_init_equilibrium(self.h, rho, *u), "init_equilibrium")
+
     def step(self, g=(0.0, 0.0, 0.0), flags=0):
         ga = np.asarray(g, dtype=np.float64)
         self._chk(lib().couplb_lbm_step(self.h, _p(ga), flags), "lbm_step")
@@ -259,18 +264,20 @@ class Fix:
         self._argv = (C.c_char_p * len(args))(*[a.encode() for a in args])
         d = Domain()
         d.dimension = dimension
-        if boxhi is None:
+        if boxhi is None and len(args) >= 6:
             N = [float(args[3]), float(args[4]), float(args[5])]
             boxhi = (N[0], N[1], N[2] if dimension == 3 else 0.5)
             if dimension == 2:
                 boxlo = (boxlo[0], boxlo[1], -0.5)
+        if boxhi is None:
+            boxhi = (1.0, 1.0, 1.0)
         for k in range(3):
             d.boxlo[k], d.boxhi[k], d.periodicity[k] = boxlo[k], boxhi[k], periodicity[k]
         d.dt, d.ntimestep = dt, ntimestep
         self.domain = d
         self.step = ntimestep
         self.dimension = dimension
-        self.Ny = int(args[4])
+        self.Ny = int(args[4]) if len(args) > 4 else 0
         h = C.c_void_p()
         rc = L.couplb_fix_create(len(args), self._argv, C.byref(d), C.byref(h))
         self.h = h

@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <string>
 #include <vector>
 
@@ -10,6 +11,7 @@
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include "../../include/couplb_hip.h"
+#include "gather.hpp"
 #include "ibm.hpp"
 #include "kernels.hpp"
 
@@ -41,16 +43,18 @@ struct couplb_ctx {
   float scale = 1.0f;
   int cycle = 0;         // state counter for the u stamps
   hipStream_t stream = nullptr;
-  hipEvent_t ev[4] = {};
-  double last_ms[2] = {0, 0};
+  hipEvent_t ev[5] = {};
+  double last_ms[3] = {0, 0, 0};
   std::string err;
   // macroscopic staging
   double* mac = nullptr;      // 4*N
   double* partial = nullptr;  // 9*ny*nz
   double* red = nullptr;      // 6 + 3*ny
   // IBM
-  double *ux = nullptr, *uy = nullptr, *uz = nullptr;
+  double* u3 = nullptr;   // IBM fluid velocity, node-interleaved
   int* stamp = nullptr;
+  double *Fx = nullptr, *Fy = nullptr, *Fz = nullptr;  // spread force
+  unsigned char *flag = nullptr, *cover = nullptr;
   int cap = 0;
   unsigned *keys = nullptr, *skeys = nullptr;
   int *vals = nullptr, *perm = nullptr;
@@ -63,6 +67,8 @@ struct couplb_ctx {
   int ugather_cycle = -1; // state counter of the last velocity gather
   std::vector<SlotHost> slots;
   int nslots = 0;
+  Slot* slot_table = nullptr;  // device copy of the slot descriptors
+  int slot_table_n = -1;       // descriptors currently uploaded
   double* staging = nullptr;  // ghost-inclusive FP64 state for import/export
   size_t staging_len = 0;
 };
@@ -91,6 +97,13 @@ dim3 row_block(int nx) {
 }
 dim3 row_grid(const Geo& g, int bx) {
   return dim3((unsigned)((g.nx + bx - 1) / bx), (unsigned)g.ny, (unsigned)(g.D == 3 ? g.nz : 1));
+}
+
+// IBM tile kernels: one block per tile of rows (gather.hpp, tile_shape)
+dim3 tile_grid(const Geo& g) {
+  const TileShape ts = tile_shape(g.D, g.nx);
+  return dim3(1, (unsigned)((g.ny + ts.TJ - 1) / ts.TJ),
+              (unsigned)(g.D == 3 ? (g.nz + ts.TK - 1) / ts.TK : 1));
 }
 
 PullCtx make_pull(couplb_ctx* c) {
@@ -146,6 +159,13 @@ template <int D, typename S> struct LaunchImport {
                        c->pitch, c->geo, c->p.rho0, c->scale);
   }
 };
+template <int D, typename S> struct LaunchInitEq {
+  static void run(couplb_ctx* c, void* dst, double r, const double* u) {
+    const dim3 b = row_block(c->geo.nx);
+    hipLaunchKernelGGL((k_init_eq<D, S>), row_grid(c->geo, b.x), b, 0, c->stream, dst, c->pitch,
+                       c->geo, r, u[0], u[1], u[2], c->base, c->scale);
+  }
+};
 template <int D, typename S> struct LaunchExport {
   static void run(couplb_ctx* c, const PullCtx& pc, double* fout) {
     const int gx = c->geo.nx + 2, gy = c->geo.ny + 2, gz = (D == 3) ? c->geo.nz + 2 : 1;
@@ -156,9 +176,15 @@ template <int D, typename S> struct LaunchExport {
 };
 template <int D, typename S> struct LaunchUgather {
   static void run(couplb_ctx* c, const PullCtx& pc, const BaseSlot& bs) {
+    if (D == 3)
+      hipLaunchKernelGGL((k_cover<3>), tile_grid(c->geo), dim3(TILE_THREADS), 0, c->stream, c->geo,
+                         c->walls, bs, c->cover);
+    else
+      hipLaunchKernelGGL((k_cover<2>), tile_grid(c->geo), dim3(TILE_THREADS), 0, c->stream, c->geo,
+                         c->walls, bs, c->cover);
     const dim3 b = row_block(c->geo.nx);
-    hipLaunchKernelGGL((k_ugather<D, S>), row_grid(c->geo, b.x), b, 0, c->stream, pc, bs, c->ux,
-                       c->uy, c->uz, c->stamp, c->cycle);
+    hipLaunchKernelGGL((k_upull<D, S>), row_grid(c->geo, b.x), b, 0, c->stream, pc, c->cover, c->u3,
+                       c->stamp, c->cycle);
   }
 };
 
@@ -196,13 +222,18 @@ unsigned key_bits(int nrows) {
 }
 
 int ensure_ibm(couplb_ctx* c, int n) {
-  if (!c->ux) {
-    HIPCHK(c, hipMalloc(&c->ux, sizeof(double) * c->N));
-    HIPCHK(c, hipMalloc(&c->uy, sizeof(double) * c->N));
-    HIPCHK(c, hipMalloc(&c->uz, sizeof(double) * c->N));
+  if (c->geo.nx > MAX_NX_IBM)
+    return fail(c, COUPLB_EINVAL, "IBM coupling supports rows of at most 1024 nodes in x");
+  if (!c->u3) {
+    HIPCHK(c, hipMalloc(&c->u3, sizeof(double) * 3 * c->N));
     HIPCHK(c, hipMalloc(&c->stamp, sizeof(int) * c->N));
     HIPCHK(c, hipMemsetAsync(c->stamp, 0xFF, sizeof(int) * c->N, c->stream));
     HIPCHK(c, hipMalloc(&c->rowstart_cur, sizeof(int) * (c->geo.nrows + 1)));
+    HIPCHK(c, hipMalloc(&c->Fx, sizeof(double) * c->N));
+    HIPCHK(c, hipMalloc(&c->Fy, sizeof(double) * c->N));
+    HIPCHK(c, hipMalloc(&c->Fz, sizeof(double) * c->N));
+    HIPCHK(c, hipMalloc(&c->flag, c->N));
+    HIPCHK(c, hipMalloc(&c->cover, c->N));
   }
   if (n <= c->cap) return COUPLB_OK;
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -230,6 +261,7 @@ int ensure_ibm(couplb_ctx* c, int n) {
   }
   c->cap = cap;
   c->order_n = -1;
+  c->slot_table_n = -1;
   return COUPLB_OK;
 }
 
@@ -286,12 +318,14 @@ int ibm_impl(couplb_ctx* c, int n, const double* x, const double* v, const doubl
     c->ugather_cycle = c->cycle;
   }
   SlotHost& sl = c->slots[s];
-  if (c->D == 3)
-    hipLaunchKernelGGL((k_marker<3>), dim3(nb), dim3(tb), 0, c->stream, n, c->perm, x, v, m, f_dev,
-                       fmode, ip, c->geo, c->walls, c->ux, c->uy, c->uz, sl.rec);
-  else
+  if (c->D == 3) {
+    const int nb4 = (4 * n + tb - 1) / tb;  // four lanes per marker
+    hipLaunchKernelGGL((k_marker<3>), dim3(nb4), dim3(tb), 0, c->stream, n, c->perm, x, v, m, f_dev,
+                       fmode, ip, c->geo, c->walls, c->u3, sl.rec);
+  } else {
     hipLaunchKernelGGL((k_marker<2>), dim3(nb), dim3(tb), 0, c->stream, n, c->perm, x, v, m, f_dev,
-                       fmode, ip, c->geo, c->walls, c->ux, c->uy, c->uz, sl.rec);
+                       fmode, ip, c->geo, c->walls, c->u3, sl.rec);
+  }
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipMemcpyAsync(sl.rowstart, c->rowstart_cur, sizeof(int) * (c->geo.nrows + 1),
                            hipMemcpyDeviceToDevice, c->stream));
@@ -316,10 +350,34 @@ int step_impl(couplb_ctx* c, const double g[3], int flags, bool timed) {
   a.write_rho = (grav && !(flags & COUPLB_STEP_RHO_REFRESHED_NEXT)) ? 1 : 0;
   a.has_moving = c->has_moving;
   a.rho_out = c->rho;
-  a.nslots = c->nslots;
-  for (int s = 0; s < c->nslots; s++) {
-    a.slots[s].rowstart = c->slots[s].rowstart;
-    a.slots[s].rec = c->slots[s].rec;
+  a.has_ibm = c->nslots > 0;
+  a.flag = c->flag;
+  a.Fx = c->Fx; a.Fy = c->Fy; a.Fz = c->Fz;
+  if (c->nslots > 0) {
+    // spread the pending marker records into the per-node force (k_spread)
+    if (!c->slot_table) HIPCHK(c, hipMalloc(&c->slot_table, sizeof(Slot) * MAX_SLOTS));
+    if (c->slot_table_n != (int)c->slots.size()) {
+      Slot h[MAX_SLOTS];
+      for (size_t s = 0; s < c->slots.size(); s++) {
+        h[s].rowstart = c->slots[s].rowstart;
+        h[s].rec = c->slots[s].rec;
+      }
+      HIPCHK(c, hipMemcpy(c->slot_table, h, sizeof(Slot) * c->slots.size(), hipMemcpyHostToDevice));
+      c->slot_table_n = (int)c->slots.size();
+    }
+    SpreadArgs sa;
+    sa.g = c->geo;
+    sa.w = c->walls;
+    sa.nslots = c->nslots;
+    sa.slots = c->slot_table;
+    sa.Fx = c->Fx; sa.Fy = c->Fy; sa.Fz = c->Fz;
+    sa.flag = c->flag;
+    if (timed) HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
+    if (c->D == 3)
+      hipLaunchKernelGGL((k_spread<3>), tile_grid(c->geo), dim3(TILE_THREADS), 0, c->stream, sa);
+    else
+      hipLaunchKernelGGL((k_spread<2>), tile_grid(c->geo), dim3(TILE_THREADS), 0, c->stream, sa);
+    HIPCHK(c, hipGetLastError());
   }
   if (timed) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
   dispatch<LaunchStep>(c, a);
@@ -372,7 +430,14 @@ int couplb_create(const couplb_params* p, couplb_ctx** out) {
       if (d < p->dim && p->wall[d][s] == 2) c->has_moving = 1;
     }
   c->N = (size_t)p->nx * p->ny * nz;
-  c->pitch = round_up(c->N, 512);
+  // q-plane pitch: whole 4 KiB pages plus a skew so the 19 plane streams
+  // do not start on the same HBM channel; COUPLB_PLANE_PAD (elements)
+  // overrides the skew for tuning.
+  {
+    size_t pad = 256;
+    if (const char* e = std::getenv("COUPLB_PLANE_PAD")) pad = (size_t)std::strtoull(e, nullptr, 10);
+    c->pitch = round_up(c->N, 512) + pad;
+  }
   c->elem = storage_bytes(p->storage);
   c->scale = (p->storage == COUPLB_FP16) ? 1024.0f : 1.0f;
   c->base = (p->storage == COUPLB_FP64) ? 0.0 : p->rho0;
@@ -427,12 +492,15 @@ int couplb_destroy(couplb_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   (void)hipFree(c->f[0]); (void)hipFree(c->f[1]); (void)hipFree(c->rho);
   (void)hipFree(c->mac); (void)hipFree(c->partial); (void)hipFree(c->red);
-  (void)hipFree(c->ux); (void)hipFree(c->uy); (void)hipFree(c->uz); (void)hipFree(c->stamp);
+  (void)hipFree(c->u3); (void)hipFree(c->stamp);
+  (void)hipFree(c->Fx); (void)hipFree(c->Fy); (void)hipFree(c->Fz);
+  (void)hipFree(c->flag); (void)hipFree(c->cover);
   (void)hipFree(c->keys); (void)hipFree(c->skeys); (void)hipFree(c->vals); (void)hipFree(c->perm);
   (void)hipFree(c->rowstart_cur); (void)hipFree(c->bases); (void)hipFree(c->sort_tmp);
   (void)hipFree(c->dx); (void)hipFree(c->dv); (void)hipFree(c->dm); (void)hipFree(c->df);
   (void)hipFree(c->staging);
   for (auto& s : c->slots) { (void)hipFree(s.rowstart); (void)hipFree(s.rec); }
+  (void)hipFree(c->slot_table);
   for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -477,6 +545,25 @@ int couplb_set_state_f(couplb_ctx* c, const double* fh) {
   return COUPLB_OK;
 }
 
+int couplb_init_equilibrium(couplb_ctx* c, double rho, double ux, double uy, double uz) {
+  if (!c) return COUPLB_EINVAL;
+  if (!(rho > 0) || !std::isfinite(ux) || !std::isfinite(uy) || !std::isfinite(uz))
+    return fail(c, COUPLB_EINVAL, "init_equilibrium: rho must be positive and u finite");
+  const double u[3] = {ux, uy, c->D == 3 ? uz : uz};
+  c->cur = 0;
+  dispatch<LaunchInitEq>(c, c->f[0], rho, u);
+  HIPCHK(c, hipGetLastError());
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)((c->N + 255) / 256)), dim3(256), 0, c->stream, c->rho,
+                     c->N, rho);
+  HIPCHK(c, hipGetLastError());
+  c->local = 1;
+  c->nslots = 0;
+  c->cycle++;
+  c->rho_fresh = 0;  // like Grid::init_equilibrium: the array holds rho0
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return COUPLB_OK;
+}
+
 int couplb_get_state_f(couplb_ctx* c, double* fh) {
   if (!c || !fh) return fail(c, COUPLB_EINVAL, "null argument");
   int rc = ensure_staging(c);
@@ -492,12 +579,18 @@ int couplb_get_state_f(couplb_ctx* c, double* fh) {
 
 int couplb_lbm_step(couplb_ctx* c, const double g[3], int flags) {
   if (!c) return COUPLB_EINVAL;
+  const bool had_ibm = c->nslots > 0;
   int rc = step_impl(c, g, flags, true);
   if (rc) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   float ms = 0;
   HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
   c->last_ms[0] = ms;
+  c->last_ms[2] = 0;
+  if (had_ibm) {
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev[4], c->ev[0]));
+    c->last_ms[2] = ms;
+  }
   return COUPLB_OK;
 }
 
@@ -600,7 +693,7 @@ int couplb_synchronize(couplb_ctx* c) {
 void* couplb_stream(couplb_ctx* c) { return c ? (void*)c->stream : nullptr; }
 
 double couplb_last_kernel_ms(const couplb_ctx* c, int which) {
-  return (c && which >= 0 && which < 2) ? c->last_ms[which] : -1.0;
+  return (c && which >= 0 && which < 3) ? c->last_ms[which] : -1.0;
 }
 
 }  // extern "C"

@@ -7,7 +7,7 @@
 // contraction off, so floor() picks the same base cell and the delta values
 // round exactly as the reference's couplb_ibm.h:11-55 compiled without FMA.
 #pragma once
-#include "kernels.hpp"
+#include "gather.hpp"
 
 namespace couplb {
 
@@ -118,12 +118,64 @@ struct BaseSlot {
   const Base* rec;
 };
 
+// Which interior nodes some marker stencil (or its ghost-plane image)
+// covers: cover[node] = 1/0 for every node of the tile.
+template <int D>
+__global__ void __launch_bounds__(TILE_THREADS) k_cover(Geo g, Walls w, BaseSlot bs,
+                                                        unsigned char* cover) {
+  __shared__ Base win[COVER_WIN];
+  __shared__ int cnt[MAX_BIN], start[MAX_BIN + 1], list[COVER_WIN], sh[TILE_THREADS / 64];
+  __shared__ TileLists tl;
+  const TileShape ts = tile_shape(D, g.nx);
+  const int J0 = 1 + blockIdx.y * ts.TJ, K0 = 1 + blockIdx.z * ts.TK;
+  const int nbin = g.nx + 5;
+  tile_lists<D>(g, w, J0, K0, ts, tl);
+  bool cov[NPT];
+#pragma unroll
+  for (int p = 0; p < NPT; p++) cov[p] = false;
+  BaseSlots slots{bs.rowstart};
+  stage_tile<D, BaseSlots, Base, COVER_WIN>(
+      slots, 1, g, tl, win, sh, [&](int, int idx, Base& c) { c = bs.rec[idx]; },
+      [&](int n) {
+        build_bins(n, nbin, cnt, start, list, sh,
+                   [&](int c) { return win[c].valid ? win[c].i0 + 2 : -1; });
+#pragma unroll
+        for (int p = 0; p < NPT; p++) {
+          int x, j, k;
+          if (cov[p] || !tile_node(g, ts, J0, K0, p, x, j, k)) continue;
+          const Img xa = images(x + 1, g.nx, w.type[0][0], w.type[0][1]);
+          const Img ya = images(j + 1, g.ny, w.type[1][0], w.type[1][1]);
+          const Img za = (D == 3) ? images(k + 1, g.nz, w.type[2][0], w.type[2][1]) : Img{1, 0, -9, -9};
+          for (int ia = 0; ia < xa.n && !cov[p]; ia++) {
+            const int aa = xa.at(ia);
+            for (int kb = aa; kb <= aa + 3 && kb < nbin && !cov[p]; kb++)
+              for (int e = start[kb]; e < start[kb + 1] && !cov[p]; e++) {
+                const Base& m = win[list[e]];
+                for (int ib = 0; ib < ya.n; ib++)
+                  for (int ic = 0; ic < za.n; ic++) {
+                    const int u = ya.at(ib) - m.j0 + 1;
+                    const int v = (D == 3) ? za.at(ic) - m.k0 + 1 : 0;
+                    if (u >= 0 && u <= 3 && v >= 0 && v <= 3) cov[p] = true;
+                  }
+              }
+          }
+        }
+      });
+#pragma unroll
+  for (int p = 0; p < NPT; p++) {
+    int x, j, k;
+    if (!tile_node(g, ts, J0, K0, p, x, j, k)) continue;
+    cover[(unsigned)x + (unsigned)g.nx * ((unsigned)j + (unsigned)g.ny * (unsigned)k)] = cov[p] ? 1 : 0;
+  }
+}
+
 // Fluid velocity u = j/rho (compute_macroscopic(false), couplb_lattice.h:
-// 155-202) at every interior node that an image of some marker stencil
-// covers, once per state (stamp = state counter).
+// 155-202) at covered nodes, once per state (stamp = state counter).  u is
+// stored node-interleaved (ux, uy, uz): a stencil row is one contiguous
+// segment for the marker gathers.
 template <int D, typename S>
-__global__ void __launch_bounds__(256) k_ugather(PullCtx pc, BaseSlot bs, double* ux, double* uy,
-                                                 double* uz, int* stamp, int cycle) {
+__global__ void __launch_bounds__(256) k_upull(PullCtx pc, const unsigned char* cover, double* u3,
+                                               int* stamp, int cycle) {
   using L = Lat<D>;
   using T = typename Store<S>::T;
   constexpr int Q = L::Q;
@@ -131,18 +183,8 @@ __global__ void __launch_bounds__(256) k_ugather(PullCtx pc, BaseSlot bs, double
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y, k = blockIdx.z;
   if (i >= g.nx) return;
-  int xa[3];
-  const int na = images(i + 1, g.nx, pc.w.type[0][0], pc.w.type[0][1], xa);
-  bool covered = false;
-  visit_row<D>(bs, g, pc.w, j + 1, k + 1, [&](const Base& m, int, int) {
-    for (int ia = 0; ia < na; ia++) {
-      const int t = xa[ia] - m.i0 + 1;
-      if (t >= 0 && t <= 3) covered = true;
-    }
-  });
-  if (!covered) return;
   const unsigned node = (unsigned)i + (unsigned)g.nx * ((unsigned)j + (unsigned)g.ny * (unsigned)k);
-  if (stamp[node] == cycle) return;
+  if (!cover[node] || stamp[node] == cycle) return;
   T f[Q];
   pull_node<D, S>(pc, i, j, k, f);
   double A = 0, jx = 0, jy = 0, jz = 0;
@@ -158,59 +200,66 @@ __global__ void __launch_bounds__(256) k_ugather(PullCtx pc, BaseSlot bs, double
   if (r <= RHO_MIN) r = RHO_MIN;
   if (!isfinite(r)) r = RHO_MIN;
   const double inv_r = 1.0 / r;
-  ux[node] = jx * inv_r;
-  uy[node] = jy * inv_r;
-  uz[node] = jz * inv_r;
+  u3[3 * (size_t)node + 0] = jx * inv_r;
+  u3[3 * (size_t)node + 1] = jy * inv_r;
+  u3[3 * (size_t)node + 2] = jz * inv_r;
   stamp[node] = cycle;
 }
 
-// One marker per lane (markers in base-row order).  IBM::interpolate
-// (couplb_ibm.h:57-90) + penalty force (fix_couplb.cpp:779-785) + the data
-// IBM::spread needs (couplb_ibm.h:92-133), stored as a MarkerRec.
+// Markers in base-row order, MARKER_LANES lanes per marker (one per z-plane
+// of the 4x4x4 stencil in 3D).  IBM::interpolate (couplb_ibm.h:57-90) +
+// penalty force (fix_couplb.cpp:779-785) + the data IBM::spread needs
+// (couplb_ibm.h:92-133), stored as a MarkerRec.  The per-plane partial sums
+// are combined in plane order, so results are deterministic; they differ
+// from the reference's single running sum by rounding only.
 // fmode 0: f_atom[idx] += F (device f); 1: f_atom[idx] = F (host adds).
 template <int D>
 __global__ void __launch_bounds__(256) k_marker(int n, const int* perm, const double* x,
                                                 const double* v, const double* m, double* f_atom,
                                                 int fmode, IbmParams ip, Geo g, Walls w,
-                                                const double* ux, const double* uy,
-                                                const double* uz, MarkerRec* rec) {
+                                                const double* u3, MarkerRec* rec) {
 #pragma clang fp contract(off)
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  const int idx = perm[p];
-  double lx, ly, lz;
-  to_local(ip, D, x + 3 * (size_t)idx, lx, ly, lz);
-  const bool fin = isfinite(lx) && isfinite(ly) && isfinite(lz) && fabs(lx) < 1e9 && fabs(ly) < 1e9 && fabs(lz) < 1e9;
-  MarkerRec r;
+  constexpr int LPM = (D == 3) ? 4 : 1;
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = tid / LPM, vv = tid % LPM;
+  const bool live = p < n;
+  const int idx = live ? perm[p] : 0;
+  double lx = 0, ly = 0, lz = 0;
+  if (live) to_local(ip, D, x + 3 * (size_t)idx, lx, ly, lz);
+  const bool fin = live && isfinite(lx) && isfinite(ly) && isfinite(lz) && fabs(lx) < 1e9 &&
+                   fabs(ly) < 1e9 && fabs(lz) < 1e9;
   const int i0 = fin ? (int)floor(lx) : -100;
   const int j0 = fin ? (int)floor(ly) : -100;
   const int k0 = (fin && D == 3) ? (int)floor(lz) : 0;
+  double wx[4], wy[4];
+#pragma unroll
   for (int t = 0; t < 4; t++) {
-    r.wx[t] = delta1(ip.kern, lx - (double)(i0 - 1 + t));
-    r.wy[t] = delta1(ip.kern, ly - (double)(j0 - 1 + t));
-    r.wz[t] = (D == 3) ? delta1(ip.kern, lz - (double)(k0 - 1 + t)) : 1.0;
+    wx[t] = delta1(ip.kern, lx - (double)(i0 - 1 + t));
+    wy[t] = delta1(ip.kern, ly - (double)(j0 - 1 + t));
   }
+  const double wzv = (D == 3) ? delta1(ip.kern, lz - (double)(k0 - 1 + vv)) : 1.0;
   double ui = 0, vi = 0, wi = 0, ws = 0, wsf = 0;
-  const int vlo = (D == 3) ? 0 : 1, vhi = (D == 3) ? 3 : 1;  // 2D: single plane c = 0
-  for (int vv = vlo; vv <= vhi; vv++) {
-    const int c = (D == 3) ? k0 - 1 + vv : 0;
-    if (D == 3 && (c < 0 || c > g.nz + 1)) continue;
+  const int c = (D == 3) ? k0 - 1 + vv : 0;
+  if (fin && !(D == 3 && (c < 0 || c > g.nz + 1))) {
+#pragma unroll
     for (int u = 0; u < 4; u++) {
       const int b = j0 - 1 + u;
       if (b < 0 || b > g.ny + 1) continue;
+#pragma unroll
       for (int t = 0; t < 4; t++) {
         const int a = i0 - 1 + t;
         if (a < 0 || a > g.nx + 1) continue;
         int wd, wsd;
         const int nt = node_type(w, g, a, b, c, wd, wsd);
-        const double d = (D == 3) ? (r.wx[t] * r.wy[u]) * r.wz[vv] : r.wx[t] * r.wy[u];
+        const double d = (D == 3) ? (wx[t] * wy[u]) * wzv : wx[t] * wy[u];
         if (nt == 0) {
           const unsigned node = (unsigned)wrap_ghost(a, g.nx) +
                                 (unsigned)g.nx * ((unsigned)wrap_ghost(b, g.ny) +
                                                   (unsigned)g.ny * (unsigned)((D == 3) ? wrap_ghost(c, g.nz) : 0));
-          ui += ux[node] * d;
-          vi += uy[node] * d;
-          wi += uz[node] * d;
+          const double* un = u3 + 3 * (size_t)node;
+          ui += un[0] * d;
+          vi += un[1] * d;
+          wi += un[2] * d;
           ws += d;
           wsf += d;
         } else if (nt == 1 || nt == 3) {
@@ -224,6 +273,24 @@ __global__ void __launch_bounds__(256) k_marker(int n, const int* perm, const do
       }
     }
   }
+  double wz[4] = {1.0, 1.0, 1.0, 1.0};
+  if (LPM > 1) {
+    // combine the planes in order 0, 1, 2, 3 (every lane of the group
+    // computes the same sums)
+    const int lane0 = (threadIdx.x & 63) & ~(LPM - 1);
+    double s[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+    for (int l = 0; l < LPM; l++) {
+      s[0] += __shfl(ui, lane0 + l);
+      s[1] += __shfl(vi, lane0 + l);
+      s[2] += __shfl(wi, lane0 + l);
+      s[3] += __shfl(ws, lane0 + l);
+      s[4] += __shfl(wsf, lane0 + l);
+      wz[l] = __shfl(wzv, lane0 + l);
+    }
+    ui = s[0]; vi = s[1]; wi = s[2]; ws = s[3]; wsf = s[4];
+  }
+  if (!live || vv != 0) return;
   double uf0 = 0, uf1 = 0, uf2 = 0;
   if (ws > DELTA_TOL) {
     const double iw = 1.0 / ws;
@@ -237,6 +304,9 @@ __global__ void __launch_bounds__(256) k_marker(int n, const int* perm, const do
   double* fa = f_atom + 3 * (size_t)idx;
   if (fmode == 0) { fa[0] += Fx; fa[1] += Fy; fa[2] += Fz; }
   else { fa[0] = Fx; fa[1] = Fy; fa[2] = Fz; }
+  MarkerRec r;
+#pragma unroll
+  for (int t = 0; t < 4; t++) { r.wx[t] = wx[t]; r.wy[t] = wy[t]; r.wz[t] = wz[t]; }
   r.fx = -Fx * ip.inv_N / ip.force_scale;
   r.fy = -Fy * ip.inv_N / ip.force_scale;
   r.fz = -Fz * ip.inv_N / ip.force_scale;

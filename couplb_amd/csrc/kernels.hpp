@@ -76,46 +76,16 @@ __device__ inline int wrap_ghost(int a, int n) {
 // inclusive frame: itself, plus the ghost plane that the self-periodic fold
 // (couplb_streaming.h:217-225: ghost_lo -> hi face, ghost_hi -> lo face)
 // maps onto it.  Faces carrying a wall never receive spread force.
-__device__ inline int images(int I, int n, int tlo, int thi, int out[3]) {
-  int c = 0;
-  out[c++] = I;
-  if (I == n && tlo == 0) out[c++] = 0;
-  if (I == 1 && thi == 0) out[c++] = n + 1;
-  return c;
-}
-
-// ---------------------------------------------------------------------------
-// Visit every marker record of one slot whose 4^D stencil contains an image
-// of interior node row (J, K) (ghost-inclusive).  fn(rec, u, v) receives the
-// record and the y/z stencil offsets of that image; the caller tests x.
-// Rows are visited in a fixed order, so any reduction done by fn is
-// deterministic.
-// ---------------------------------------------------------------------------
-template <int D, typename SlotT, typename Fn>
-__device__ inline void visit_row(const SlotT& s, const Geo& g, const Walls& w,
-                                 int J, int K, Fn&& fn) {
-  int by[3], bz[3];
-  const int nb = images(J, g.ny, w.type[1][0], w.type[1][1], by);
-  int nc = 1;
-  bz[0] = 0;
-  if (D == 3) nc = images(K, g.nz, w.type[2][0], w.type[2][1], bz);
-  for (int ib = 0; ib < nb; ib++) {
-    for (int ic = 0; ic < nc; ic++) {
-      const int b = by[ib], c = bz[ic];
-      const int klo = (D == 3) ? c - 2 : 0, khi = (D == 3) ? c + 1 : 0;
-      for (int k0 = klo; k0 <= khi; k0++) {
-        for (int j0 = b - 2; j0 <= b + 1; j0++) {
-          const int row = (j0 + 2) + g.nry * ((D == 3) ? (k0 + 2) : 0);
-          const int r0 = s.rowstart[row], r1 = s.rowstart[row + 1];
-          for (int r = r0; r < r1; r++) {
-            const auto& m = s.rec[r];
-            if (!m.valid) continue;
-            fn(m, b - j0 + 1, (D == 3) ? (c - k0 + 1) : 0);
-          }
-        }
-      }
-    }
-  }
+struct Img {
+  int n, a0, a1, a2;
+  __device__ int at(int k) const { return k == 0 ? a0 : (k == 1 ? a1 : a2); }
+};
+__device__ inline Img images(int I, int n, int tlo, int thi) {
+  Img r{1, I, -9, -9};
+  const bool lo = (I == n && tlo == 0), hi = (I == 1 && thi == 0);
+  if (lo) { r.a1 = 0; r.n = 2; if (hi) { r.a2 = n + 1; r.n = 3; } }
+  else if (hi) { r.a1 = n + 1; r.n = 2; }
+  return r;
 }
 
 // ---------------------------------------------------------------------------
@@ -209,11 +179,19 @@ __device__ inline void pull_node(const PullCtx& pc, int i, int j, int k,
       f[q] = Store<S>::load(src + qs * pc.pitch + node);
       if (wt == 2) {
         const double* uw = pc.w.vel[wd][wsd];
-        const T eu = (T)(ex * uw[0] + ey * uw[1] + ez * uw[2]);
-        // 2 w rho_w eu / cs^2 with cs^2 = 1/3; stored values carry `scale`
-        T mv = (T)(6.0 * L::w(L::opp(q))) * rw * eu;
-        if (Store<S>::deviation) mv *= (T)pc.scale;
-        f[q] += mv;
+        if (Store<S>::deviation) {
+          // 2 w rho_w eu / cs^2 with cs^2 = 1/3; stored values carry `scale`
+          const T eu = (T)(ex * uw[0] + ey * uw[1] + ez * uw[2]);
+          f[q] += (T)(6.0 * L::w(L::opp(q))) * rw * eu * (T)pc.scale;
+        } else {
+#pragma clang fp contract(off)
+          // f[opp] + 2.0*w[opp]*rw*eu/cs2 in the reference's order
+          double eu = 0.0;
+          if (ex) eu = ex * uw[0];
+          if (ey) eu = (ex ? eu + ey * uw[1] : ey * uw[1]);
+          if (ez) eu = ((ex || ey) ? eu + ez * uw[2] : ez * uw[2]);
+          f[q] = f[q] + div_cs2(2.0 * L::w(L::opp(q)) * (double)rw * eu);
+        }
       }
     }
     if (Store<S>::deviation) f[q] *= (T)pc.inv_scale;
@@ -228,6 +206,61 @@ __device__ inline void pull_node(const PullCtx& pc, int i, int j, int k,
 // BGK::collide (couplb_collision.h:18-62), exchange + stream
 // (couplb_streaming.h:82-139) and clear_forces (couplb_lattice.h:243-247).
 // ---------------------------------------------------------------------------
+// a + b*c; the FP64 path rounds the product separately, as the reference
+template <typename T>
+__device__ inline T add_prod(T a, T b, T c) { return a + b * c; }
+template <>
+__device__ inline double add_prod<double>(double a, double b, double c) {
+#pragma clang fp contract(off)
+  return a + b * c;
+}
+
+// Signed sum of the nonzero terms of c . v for a lattice direction: equal in
+// value to the reference's cx*v0 + cy*v1 + cz*v2 (the omitted terms are
+// products with 0 and do not change a sum).
+template <int D>
+__device__ inline double edot(int q, double v0, double v1, double v2) {
+#pragma clang fp contract(off)
+  using L = Lat<D>;
+  const int ex = L::ex(q), ey = L::ey(q), ez = L::ez(q);
+  double s = 0.0;
+  bool any = false;
+  if (ex) { s = ex > 0 ? v0 : -v0; any = true; }
+  if (ey) { s = any ? (ey > 0 ? s + v1 : s - v1) : (ey > 0 ? v1 : -v1); any = true; }
+  if (ez) { s = any ? (ez > 0 ? s + v2 : s - v2) : (ez > 0 ? v2 : -v2); any = true; }
+  return s;
+}
+
+// BGK + Guo in FP64 with the reference's expression order and rounding
+// (couplb_lattice.h:196-200 velocity, couplb_collision.h:36-58 collision):
+// the result equals the reference's to the last bit.
+template <int D>
+__device__ inline void collide_exact(const double (&f)[Lat<D>::Q], double r, double su, double sv,
+                                     double sw, double F0, double F1, double F2, double om,
+                                     double* dst, size_t pitch, unsigned node) {
+#pragma clang fp contract(off)
+  using L = Lat<D>;
+  constexpr int Q = L::Q;
+  const double inv_r = 1.0 / r;
+  const double u0 = (su + 0.5 * F0) * inv_r;
+  const double u1 = (sv + 0.5 * F1) * inv_r;
+  const double u2 = (sw + 0.5 * F2) * inv_r;
+  const double half_om = 1.0 - 0.5 * om;
+  const double uu = u0 * u0 + u1 * u1 + u2 * u2;
+  const double uf = u0 * F0 + u1 * F1 + u2 * F2;
+  const double uu_term = div_cs2(0.5 * uu);
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    const double w = L::w(q);
+    const double eu = edot<D>(q, u0, u1, u2);
+    const double ef = edot<D>(q, F0, F1, F2);
+    const double feq = w * r * (1.0 + div_cs2(eu) + div_cs4(0.5 * eu * eu) - uu_term);
+    const double Si = w * half_om * (div_cs2(ef - uf) + div_cs4(eu * ef));
+    const double fn = f[q] + (-om * (f[q] - feq) + Si);
+    Store<double>::store(dst + q * pitch + node, fn);
+  }
+}
+
 struct StepArgs {
   PullCtx pc;
   void* dst;
@@ -237,8 +270,11 @@ struct StepArgs {
   int write_rho;        // 1: store the collision density at every node
   int has_moving;       // store the density at nodes next to type-2 faces
   double* rho_out;      // may alias pc.rho_coll (same node read then written)
-  int nslots;
-  Slot slots[MAX_SLOTS];
+  int has_ibm;          // spread force of the last cycle present
+  const unsigned char* flag;
+  const double* Fx;
+  const double* Fy;
+  const double* Fz;
 };
 
 template <int D, typename S>
@@ -252,30 +288,13 @@ __global__ void __launch_bounds__(256) k_step(StepArgs a) {
   if (i >= g.nx) return;
   const unsigned node = (unsigned)i + (unsigned)g.nx * ((unsigned)j + (unsigned)g.ny * (unsigned)k);
 
-  // IBM force (spread by the marker records of the last cycle)
+  // IBM force spread by the last cycle's markers (k_spread), present only
+  // at flagged nodes
   T Fx = 0, Fy = 0, Fz = 0;
-  if (a.nslots > 0) {
-    double ax = 0, ay = 0, az = 0;
-    int xa[3];
-    const int na = images(i + 1, g.nx, a.pc.w.type[0][0], a.pc.w.type[0][1], xa);
-    for (int s = 0; s < a.nslots; s++) {
-      visit_row<D>(a.slots[s], g, a.pc.w, j + 1, k + 1,
-                   [&](const MarkerRec& m, int u, int v) {
-#pragma clang fp contract(off)
-                     const double wyz_u = m.wy[u];
-                     const double wz_v = (D == 3) ? m.wz[v] : 1.0;
-                     for (int ia = 0; ia < na; ia++) {
-                       const int t = xa[ia] - m.i0 + 1;
-                       if (t < 0 || t > 3) continue;
-                       const double d = (D == 3) ? (m.wx[t] * wyz_u) * wz_v : m.wx[t] * wyz_u;
-                       const double wgt = d * m.sc;
-                       ax += m.fx * wgt;
-                       ay += m.fy * wgt;
-                       az += m.fz * wgt;
-                     }
-                   });
-    }
-    Fx = (T)ax; Fy = (T)ay; Fz = (T)az;
+  if (a.has_ibm && a.flag[node]) {
+    Fx = (T)a.Fx[node];
+    Fy = (T)a.Fy[node];
+    Fz = (T)a.Fz[node];
   }
 
   T f[Q];
@@ -299,9 +318,9 @@ __global__ void __launch_bounds__(256) k_step(StepArgs a) {
   // gravity: F += g * rho_array (fix_couplb.cpp:706)
   if (a.grav_mode) {
     const T rg = (a.grav_mode == 1) ? rho : (T)a.pc.rho_coll[node];
-    Fx += (T)a.g[0] * rg;
-    Fy += (T)a.g[1] * rg;
-    Fz += (T)a.g[2] * rg;
+    Fx = add_prod(Fx, (T)a.g[0], rg);
+    Fy = add_prod(Fy, (T)a.g[1], rg);
+    Fz = add_prod(Fz, (T)a.g[2], rg);
   }
   if (a.write_rho) a.rho_out[node] = (double)rho;
   else if (a.has_moving) {
@@ -311,29 +330,34 @@ __global__ void __launch_bounds__(256) k_step(StepArgs a) {
         (D == 3 && ((k == 0 && a.pc.w.type[2][0] == 2) || (k == g.nz - 1 && a.pc.w.type[2][1] == 2)));
     if (wall_adj) a.rho_out[node] = (double)rho;
   }
-  const T inv_r = (T)1 / rho;
-  const T ux = (jx + (T)0.5 * Fx) * inv_r;
-  const T uy = (jy + (T)0.5 * Fy) * inv_r;
-  const T uz = (jz + (T)0.5 * Fz) * inv_r;
-  const T om = (T)a.omega;
-  const T hom = (T)1 - (T)0.5 * om;
-  const T uu = ux * ux + uy * uy + uz * uz;
-  const T uf = ux * Fx + uy * Fy + uz * Fz;
   S* dst = static_cast<S*>(a.dst);
-  const T scale = (T)a.pc.scale;
+  if constexpr (!Store<S>::deviation) {
+    collide_exact<D>(f, rho, jx, jy, jz, Fx, Fy, Fz, a.omega, dst, a.pc.pitch, node);
+    return;
+  } else {
+    const T inv_r = (T)1 / rho;
+    const T ux = (jx + (T)0.5 * Fx) * inv_r;
+    const T uy = (jy + (T)0.5 * Fy) * inv_r;
+    const T uz = (jz + (T)0.5 * Fz) * inv_r;
+    const T om = (T)a.omega;
+    const T hom = (T)1 - (T)0.5 * om;
+    const T uu = ux * ux + uy * uy + uz * uz;
+    const T uf = ux * Fx + uy * Fy + uz * Fz;
+    const T scale = (T)a.pc.scale;
 #pragma unroll
-  for (int q = 0; q < Q; q++) {
-    const T cx = (T)L::ex(q), cy = (T)L::ey(q), cz = (T)L::ez(q);
-    const T wq = (T)L::w(q);
-    const T eu = cx * ux + cy * uy + cz * uz;
-    const T ef = cx * Fx + cy * Fy + cz * Fz;
-    // feq - w*base = w*(A + rho*(3eu + 4.5eu^2 - 1.5uu))   (couplb_lattice.h:137-144)
-    const T feq = wq * (A + rho * ((T)3 * eu + (T)4.5 * eu * eu - (T)1.5 * uu));
-    // Guo source w(1-om/2)[(e.F - u.F)/cs^2 + (e.u)(e.F)/cs^4]  (couplb_collision.h:51-52)
-    const T Si = wq * hom * ((T)3 * (ef - uf) + (T)9 * eu * ef);
-    T fn = f[q] - om * (f[q] - feq) + Si;
-    if (Store<S>::deviation) fn *= scale;
-    Store<S>::store(dst + q * a.pc.pitch + node, fn);
+    for (int q = 0; q < Q; q++) {
+      const T cx = (T)L::ex(q), cy = (T)L::ey(q), cz = (T)L::ez(q);
+      const T wq = (T)L::w(q);
+      const T eu = cx * ux + cy * uy + cz * uz;
+      const T ef = cx * Fx + cy * Fy + cz * Fz;
+      // feq - w*base = w*(A + rho*(3eu + 4.5eu^2 - 1.5uu))   (couplb_lattice.h:137-144)
+      const T feq = wq * (A + rho * ((T)3 * eu + (T)4.5 * eu * eu - (T)1.5 * uu));
+      // Guo source w(1-om/2)[(e.F - u.F)/cs^2 + (e.u)(e.F)/cs^4]  (couplb_collision.h:51-52)
+      const T Si = wq * hom * ((T)3 * (ef - uf) + (T)9 * eu * ef);
+      T fn = f[q] - om * (f[q] - feq) + Si;
+      fn *= scale;
+      Store<S>::store(dst + q * a.pc.pitch + node, fn);
+    }
   }
 }
 
@@ -467,6 +491,33 @@ __global__ void k_import(const double* fin, void* dstv, size_t pitch, Geo g, dou
     if (Store<S>::deviation) v = (v - L::w(q) * base_w_rho0) * scale;
     Store<S>::store(dst + q * pitch + node, (typename Store<S>::T)v);
   }
+}
+
+// Grid::init_equilibrium(rho0, ux0, uy0, uz0), couplb_lattice.h:137-154
+template <int D, typename S>
+__global__ void k_init_eq(void* dstv, size_t pitch, Geo g, double r, double u0, double u1,
+                          double u2, double base, float scale) {
+  using L = Lat<D>;
+  constexpr int Q = L::Q;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y, k = blockIdx.z;
+  if (i >= g.nx) return;
+  const unsigned node = (unsigned)i + (unsigned)g.nx * ((unsigned)j + (unsigned)g.ny * (unsigned)k);
+  S* dst = static_cast<S*>(dstv);
+  const double cs2 = 1.0 / 3.0, cs4 = cs2 * cs2;
+  const double uu = u0 * u0 + u1 * u1 + u2 * u2;
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    const double eu = L::ex(q) * u0 + L::ey(q) * u1 + L::ez(q) * u2;
+    double v = L::w(q) * r * (1.0 + eu / cs2 + 0.5 * eu * eu / cs4 - 0.5 * uu / cs2);
+    if (Store<S>::deviation) v = (v - L::w(q) * base) * scale;
+    Store<S>::store(dst + q * pitch + node, (typename Store<S>::T)v);
+  }
+}
+
+__global__ void k_fill(double* p, size_t n, double v) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
 }
 
 template <int D, typename S>

@@ -70,6 +70,27 @@ template <> struct Lat<3> {
 };
 
 // ---------------------------------------------------------------------------
+// Correctly rounded division by the constants cs^2 = fl(1/3) and
+// cs^4 = fl(fl(1/3)^2) in three FP64 operations (Markstein: q = x*r with
+// r = RN(1/c), remainder by FMA, one correction FMA), so the FP64 path
+// rounds exactly like the reference's `x / cs2` and `x / cs4`
+// (couplb_collision.h:51-52, couplb_lattice.h:143) at a fraction of the cost
+// of a general division.
+// ---------------------------------------------------------------------------
+__device__ inline double div_cs2(double x) {
+  const double c = 1.0 / 3.0;
+  const double q = x * 3.0;
+  const double e = __builtin_fma(-q, c, x);
+  return __builtin_fma(e, 3.0, q);
+}
+__device__ inline double div_cs4(double x) {
+  const double c = (1.0 / 3.0) * (1.0 / 3.0);
+  const double q = x * 9.0;
+  const double e = __builtin_fma(-q, c, x);
+  return __builtin_fma(e, 9.0, q);
+}
+
+// ---------------------------------------------------------------------------
 // Storage formats.  FP64 stores f_i itself (the reference's representation).
 // The reduced formats store the deviation (f_i - w_i*rho0)*scale and compute
 // in float; scale is a power of two (exact) that lifts small deviations out

@@ -579,6 +579,12 @@ int couplb_fix_set_variable_callback(couplb_fix* fx, couplb_var_eval fn, void* u
   return COUPLB_OK;
 }
 
+int couplb_fix_set_domain(couplb_fix* fx, const couplb_domain* dom) {
+  if (!fx || !fx->core || !dom) return COUPLB_EINVAL;
+  fx->core->set_domain(*dom);
+  return COUPLB_OK;
+}
+
 int couplb_fix_init(couplb_fix* fx) {
   if (!fx || !fx->core) return COUPLB_EINVAL;
   return guard(fx, [&] { fx->core->init(); });

@@ -31,6 +31,7 @@ class FixCoupLBCore {
   int end_of_step(long step);
 
   void set_variable_callback(couplb_var_eval fn, void* user) { var_fn = fn; var_user = user; }
+  void set_domain(const couplb_domain& d) { dom = d; }
   double scalar(const std::string& name) const;
   const double* last_diag() const { return diag; }
   bool last_profile(double* r, double* u, double* v) const;

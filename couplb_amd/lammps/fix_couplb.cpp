@@ -1,0 +1,107 @@
+// fix_couplb.cpp -- LAMMPS adapter for the MI355X build of `fix couplb`.
+// Each callback forwards to the host core through include/couplb_fix.h; the
+// core reproduces FixCoupLB's behaviour (CoupLB/fix_couplb.cpp).
+#include "fix_couplb.h"
+
+#include "atom.h"
+#include "comm.h"
+#include "domain.h"
+#include "error.h"
+#include "input.h"
+#include "modify.h"
+#include "update.h"
+#include "variable.h"
+
+using namespace LAMMPS_NS;
+using namespace FixConst;
+
+FixCoupLB::FixCoupLB(LAMMPS *lmp, int narg, char **arg) : Fix(lmp, narg, arg), core(nullptr)
+{
+  // the whole grid lives on one GPU: one MPI rank drives it
+  if (comm->nprocs != 1)
+    error->all(FLERR, "fix couplb (MI355X build) runs on one MPI rank holding the whole grid");
+  couplb_domain dom = current_domain();
+  const int rc = couplb_fix_create(narg, arg, &dom, &core);
+  if (rc != COUPLB_OK) {
+    const std::string msg = couplb_fix_last_error(core);
+    couplb_fix_destroy(core);
+    core = nullptr;
+    error->all(FLERR, msg);
+  }
+  couplb_fix_set_variable_callback(core, &FixCoupLB::eval_variable, this);
+}
+
+FixCoupLB::~FixCoupLB() { couplb_fix_destroy(core); }
+
+int FixCoupLB::setmask() { return couplb_fix_setmask() == 3 ? (POST_FORCE | END_OF_STEP) : 0; }
+
+couplb_domain FixCoupLB::current_domain() const
+{
+  couplb_domain d;
+  d.dimension = domain->dimension;
+  for (int k = 0; k < 3; k++) {
+    d.boxlo[k] = domain->boxlo[k];
+    d.boxhi[k] = domain->boxhi[k];
+    d.periodicity[k] = domain->periodicity[k];
+  }
+  d.dt = update->dt;
+  d.ntimestep = (long) update->ntimestep;
+  return d;
+}
+
+couplb_atoms FixCoupLB::atom_view() const
+{
+  couplb_atoms a;
+  a.nlocal = atom->nlocal;
+  a.x = atom->nlocal ? &atom->x[0][0] : nullptr;
+  a.v = atom->nlocal ? &atom->v[0][0] : nullptr;
+  a.f = atom->nlocal ? &atom->f[0][0] : nullptr;
+  a.mask = atom->mask;
+  a.groupbit = groupbit;
+  a.rmass = atom->rmass;
+  a.mass = atom->mass;
+  a.type = atom->type;
+  return a;
+}
+
+void FixCoupLB::check(int rc)
+{
+  if (rc != COUPLB_OK) error->all(FLERR, couplb_fix_last_error(core));
+}
+
+// input->variable lookup for `gravity v_name` (fix_couplb.cpp:263-280, 671-678)
+int FixCoupLB::eval_variable(void *self, const char *name, double *value)
+{
+  auto *fx = static_cast<FixCoupLB *>(self);
+  const int ivar = fx->input->variable->find(name);
+  if (ivar < 0 || !fx->input->variable->equalstyle(ivar)) return 1;
+  fx->modify->clearstep_compute();
+  *value = fx->input->variable->compute_equal(ivar);
+  fx->modify->addstep_compute(fx->update->ntimestep + 1);
+  return 0;
+}
+
+void FixCoupLB::init()
+{
+  couplb_domain dom = current_domain();
+  check(couplb_fix_set_domain(core, &dom));
+  check(couplb_fix_init(core));
+  if (comm->me == 0 && screen) fputs(couplb_fix_log(core), screen);
+}
+
+void FixCoupLB::setup(int)
+{
+  couplb_atoms a = atom_view();
+  check(couplb_fix_setup(core, &a));
+}
+
+void FixCoupLB::post_force(int)
+{
+  couplb_atoms a = atom_view();
+  check(couplb_fix_post_force(core, &a));
+}
+
+void FixCoupLB::end_of_step()
+{
+  check(couplb_fix_end_of_step(core, (long) update->ntimestep));
+}

@@ -57,6 +57,9 @@ int couplb_fix_create(int narg, const char* const* arg, const couplb_domain* dom
 int couplb_fix_destroy(couplb_fix* fx);
 const char* couplb_fix_last_error(const couplb_fix* fx);
 int couplb_fix_set_variable_callback(couplb_fix* fx, couplb_var_eval fn, void* user);
+/* refresh box, timestep and periodicity before init (FixCoupLB::init reads
+ * domain->boxlo/boxhi and update->dt, fix_couplb.cpp:282, 305) */
+int couplb_fix_set_domain(couplb_fix* fx, const couplb_domain* dom);
 int couplb_fix_init(couplb_fix* fx);                     /* FixCoupLB::init */
 int couplb_fix_setup(couplb_fix* fx, const couplb_atoms* a);      /* ::setup */
 int couplb_fix_post_force(couplb_fix* fx, const couplb_atoms* a); /* ::post_force */

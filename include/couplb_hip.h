@@ -79,6 +79,12 @@ int couplb_set_state_f(couplb_ctx* ctx, const double* f_host);
 int couplb_get_state_f(couplb_ctx* ctx, double* f_host);
 long couplb_state_len(const couplb_ctx* ctx); /* Q*ntotal */
 
+/* Uniform equilibrium state f_q = f^eq_q(rho, u) at every node and the
+ * density array set to rho: replaces Grid::init_equilibrium
+ * (couplb_lattice.h:145-154), which the fix calls with (1, 0, 0, 0) at
+ * fix_couplb.cpp:504/513.  couplb_create already applies (rho0, 0, 0, 0). */
+int couplb_init_equilibrium(couplb_ctx* ctx, double rho, double ux, double uy, double uz);
+
 /* One LB step: replaces do_lbm_step (fix_couplb.cpp:659-669), i.e.
  * apply_external_force -> compute_macroscopic(true) -> BGK::collide ->
  * Streaming::exchange -> Streaming::stream -> clear_forces, fused into one
@@ -145,7 +151,8 @@ int couplb_synchronize(couplb_ctx* ctx);
 void* couplb_stream(couplb_ctx* ctx);
 
 /* Timing of the last call's kernels (milliseconds, HIP events on the
- * context stream): which = 0 LB kernel, 1 IBM sub-step total. */
+ * context stream): which = 0 fused LB kernel, 1 IBM sub-step total,
+ * 2 force spreading that preceded the last LB kernel. */
 double couplb_last_kernel_ms(const couplb_ctx* ctx, int which);
 
 #ifdef __cplusplus

@@ -5,10 +5,13 @@ Each test runs one `fix couplb` deck through the HIP library (couplb_amd.Fix
 restatement of the reference, then compares the post-stream populations in
 the reference's layout, the per-atom IBM forces and the diagnostics.
 """
+import os
+
 import numpy as np
 import pytest
 
 import oracle
+from tests.golden.make_golden import CASES, ibm_atoms, moving
 from tests.scenarios import (atoms_from, copy_atoms, interior, rel_l2, run_device,
                              run_oracle)
 
@@ -27,8 +30,8 @@ def test_c1_poiseuille_2d_matches_oracle_and_parabola(tmp_path):
     fd = interior(dev.device_state().reshape(9, 1, 34, 10), 2)
     fo = interior(ref.f(), 2)
     err = rel_l2(fd, fo)
-    print("C1 rel-L2(f) =", err)
-    assert err <= TOL
+    print("C1 rel-L2(f) =", err, "bit-exact", np.array_equal(fd, fo))
+    assert np.array_equal(fd, fo)    # FP64 path without markers: identical bits
     dd, od = dev.last_diag(), ref.last_diag()
     assert abs(dd["mass"] - od["mass"]) <= 1e-12 * od["mass"]
     assert abs(dd["Ma"] - od["Ma"]) <= 1e-10 * od["Ma"]
@@ -41,6 +44,38 @@ def test_c1_poiseuille_2d_matches_oracle_and_parabola(tmp_path):
     assert rel < 0.01
     ro, uo, vo = ref.last_profile()
     assert np.max(np.abs(ux - uo)) <= 1e-12 * np.max(np.abs(uo))
+
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_device_matches_reference_fixture(name):
+    """Replay each fixture written by the reference's own headers. Without
+    markers the FP64 path is bit-identical; with markers only the spreading
+    summation order differs (<= 1e-12 relative)."""
+    c = CASES[name]
+    gold = np.load(os.path.join(GOLD, f"{name}.npz"))
+    atoms_fn = None
+    if c["atoms"] is not None:
+        n, L, seed = c["atoms"]
+        base = ibm_atoms(n, L, seed)
+        if c["dim"] == 2:
+            base["x"][:, 2] = 0.0
+            base["v"][:, 2] = 0.0
+        atoms_fn = moving(base)
+    print("case", name, flush=True)
+    dev, forces = run_device(c["cmd"], c["steps"], dimension=c["dim"], dt=c["dt"], atoms_fn=atoms_fn)
+    fd = interior(dev.device_state().reshape(gold["f"].shape), c["dim"])
+    fo = interior(gold["f"], c["dim"])
+    err = rel_l2(fd, fo)
+    print(f"{name}: rel-L2(f) = {err:.3e} bit-exact={np.array_equal(fd, fo)}")
+    if c["atoms"] is None:
+        assert np.array_equal(fd, fo)
+    else:
+        assert err <= TOL
+        ferr = max(rel_l2(a, b) for a, b in zip(forces, gold["atom_forces"]))
+        assert ferr <= 1e-11
 
 
 WALLS = [

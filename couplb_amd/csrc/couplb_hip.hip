@@ -176,12 +176,18 @@ template <int D, typename S> struct LaunchExport {
 };
 template <int D, typename S> struct LaunchUgather {
   static void run(couplb_ctx* c, const PullCtx& pc, const BaseSlot& bs) {
-    if (D == 3)
-      hipLaunchKernelGGL((k_cover<3>), tile_grid(c->geo), dim3(TILE_THREADS), 0, c->stream, c->geo,
-                         c->walls, bs, c->cover);
-    else
-      hipLaunchKernelGGL((k_cover<2>), tile_grid(c->geo), dim3(TILE_THREADS), 0, c->stream, c->geo,
-                         c->walls, bs, c->cover);
+    const TileShape ts = tile_shape(D, c->geo.nx);
+#define COVER_LAUNCH(TJ, TK, XM)                                                                  \
+  hipLaunchKernelGGL((k_cover<D, TJ, TK, XM>), tile_grid(c->geo), dim3(TILE_THREADS), 0, c->stream, \
+                     c->geo, c->walls, bs, c->cover)
+    if (ts.XM == 1) {
+      if (D == 3) COVER_LAUNCH(2, 2, 1); else COVER_LAUNCH(4, 1, 1);
+    } else if (ts.XM == 2) {
+      COVER_LAUNCH(2, 1, 2);
+    } else {
+      COVER_LAUNCH(1, 1, 4);
+    }
+#undef COVER_LAUNCH
     const dim3 b = row_block(c->geo.nx);
     hipLaunchKernelGGL((k_upull<D, S>), row_grid(c->geo, b.x), b, 0, c->stream, pc, c->cover, c->u3,
                        c->stamp, c->cycle);
@@ -373,10 +379,19 @@ int step_impl(couplb_ctx* c, const double g[3], int flags, bool timed) {
     sa.Fx = c->Fx; sa.Fy = c->Fy; sa.Fz = c->Fz;
     sa.flag = c->flag;
     if (timed) HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
-    if (c->D == 3)
-      hipLaunchKernelGGL((k_spread<3>), tile_grid(c->geo), dim3(TILE_THREADS), 0, c->stream, sa);
-    else
-      hipLaunchKernelGGL((k_spread<2>), tile_grid(c->geo), dim3(TILE_THREADS), 0, c->stream, sa);
+    const TileShape ts = tile_shape(c->D, c->geo.nx);
+#define SPREAD_LAUNCH(DD, TJ, TK, XM) \
+  hipLaunchKernelGGL((k_spread<DD, TJ, TK, XM>), tile_grid(c->geo), dim3(TILE_THREADS), 0, c->stream, sa)
+    if (c->D == 3) {
+      if (ts.XM == 1) SPREAD_LAUNCH(3, 2, 2, 1);
+      else if (ts.XM == 2) SPREAD_LAUNCH(3, 2, 1, 2);
+      else SPREAD_LAUNCH(3, 1, 1, 4);
+    } else {
+      if (ts.XM == 1) SPREAD_LAUNCH(2, 4, 1, 1);
+      else if (ts.XM == 2) SPREAD_LAUNCH(2, 2, 1, 2);
+      else SPREAD_LAUNCH(2, 1, 1, 4);
+    }
+#undef SPREAD_LAUNCH
     HIPCHK(c, hipGetLastError());
   }
   if (timed) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));

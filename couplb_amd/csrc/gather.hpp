@@ -25,19 +25,15 @@ constexpr int MAX_BIN = MAX_NX_IBM + 5;          // base x-cells -2 .. nx+2
 constexpr int MAX_AXIS_ROWS = 32;                // listed base rows per axis
 
 struct TileShape {
-  int TJ, TK;
+  int TJ, TK, XM;  // rows in y and z per tile, x positions per thread
 };
 
-// Rows per tile so a tile holds at most TILE_NODES nodes.
+// Tile variants: a thread owns XM x-positions of each of the TJ*TK rows
+// (at most NPT nodes), so a tile holds at most TILE_NODES nodes.
 __host__ __device__ inline TileShape tile_shape(int D, int nx) {
-  if (D == 2) {
-    int tj = TILE_NODES / (nx > 0 ? nx : 1);
-    tj = tj > 16 ? 16 : (tj < 1 ? 1 : tj);
-    return {tj, 1};
-  }
-  if (nx <= TILE_NODES / 4) return {2, 2};
-  if (nx <= TILE_NODES / 2) return {2, 1};
-  return {1, 1};
+  if (nx <= TILE_THREADS) return D == 3 ? TileShape{2, 2, 1} : TileShape{4, 1, 1};
+  if (nx <= 2 * TILE_THREADS) return TileShape{2, 1, 2};
+  return TileShape{1, 1, 4};
 }
 
 // Base-row values of one axis a tile covering ghost-inclusive rows lo..hi
@@ -176,20 +172,7 @@ __device__ inline void build_bins(int n, int nbin, int* cnt, int* start, int* li
   __syncthreads();
 }
 
-// node slot p of this thread -> interior coordinates (x, j, k); false if the
-// slot falls outside the (clipped) tile
-__device__ inline bool tile_node(const Geo& g, TileShape ts, int J0, int K0, int p, int& x, int& j,
-                                 int& k) {
-  const int idx = threadIdx.x + p * blockDim.x;
-  if (idx >= ts.TJ * ts.TK * g.nx) return false;
-  x = idx % g.nx;
-  const int r = idx / g.nx;
-  j = J0 - 1 + r % ts.TJ;
-  k = K0 - 1 + r / ts.TJ;
-  return j < g.ny && k < (g.D == 3 ? g.nz : 1);
-}
-
-constexpr int SPREAD_WIN = 96;   // MarkerRec per LDS window (13.5 KiB)
+constexpr int SPREAD_WIN = 160;  // MarkerRec per LDS window (22.5 KiB)
 constexpr int COVER_WIN = 512;   // base cells per LDS window (8 KiB)
 
 struct RecSlots {
@@ -208,24 +191,63 @@ struct SpreadArgs {
   unsigned char* flag;  // per node: 1 where some record contributes
 };
 
+// Per-thread view of its tile nodes: XM x-positions x TJ*TK rows, with the
+// ghost-plane images of every coordinate computed once.
+template <int D, int TJ, int TK, int XM>
+struct TileNodes {
+  int x[XM], j[TJ], k[TK];
+  Img xa[XM], ya[TJ], za[TK];
+  bool xin[XM], yin[TJ], zin[TK];
+  __device__ TileNodes(const Geo& g, const Walls& w, int J0, int K0) {
+#pragma unroll
+    for (int m = 0; m < XM; m++) {
+      x[m] = threadIdx.x + m * TILE_THREADS;
+      xin[m] = x[m] < g.nx;
+      xa[m] = images(x[m] + 1, g.nx, w.type[0][0], w.type[0][1]);
+    }
+#pragma unroll
+    for (int r = 0; r < TJ; r++) {
+      j[r] = J0 - 1 + r;
+      yin[r] = j[r] < g.ny;
+      ya[r] = images(j[r] + 1, g.ny, w.type[1][0], w.type[1][1]);
+    }
+#pragma unroll
+    for (int r = 0; r < TK; r++) {
+      k[r] = K0 - 1 + r;
+      zin[r] = (D == 3) ? k[r] < g.nz : r == 0;
+      za[r] = (D == 3) ? images(k[r] + 1, g.nz, w.type[2][0], w.type[2][1]) : Img{1, 0, -9, -9};
+    }
+  }
+};
+
 // IBM::spread for every record of the pending slots, as a per-node gather
 // (couplb_ibm.h:92-133; ghost-plane images carry the self-periodic fold of
-// couplb_streaming.h:217-225).  One block per tile of rows.
-template <int D>
+// couplb_streaming.h:217-225).  One block per tile of rows.  Per node the
+// contributions are summed in the fixed order (window, x-image, base cell,
+// record, y-image, z-image).
+template <int D, int TJ, int TK, int XM>
 __global__ void __launch_bounds__(TILE_THREADS) k_spread(SpreadArgs a) {
   __shared__ MarkerRec win[SPREAD_WIN];
   __shared__ int cnt[MAX_BIN], start[MAX_BIN + 1], list[SPREAD_WIN], sh[TILE_THREADS / 64];
   __shared__ TileLists tl;
   const Geo g = a.g;
   const Walls w = a.w;
-  const TileShape ts = tile_shape(D, g.nx);
-  const int J0 = 1 + blockIdx.y * ts.TJ, K0 = 1 + blockIdx.z * ts.TK;
+  const TileShape ts{TJ, TK, XM};
+  const int J0 = 1 + blockIdx.y * TJ, K0 = 1 + blockIdx.z * TK;
   const int nbin = g.nx + 5;
   tile_lists<D>(g, w, J0, K0, ts, tl);
-  double ax[NPT], ay[NPT], az[NPT];
-  bool hit[NPT];
+  const TileNodes<D, TJ, TK, XM> tn(g, w, J0, K0);
+  double acc[XM][TJ][TK][3];
+  bool hit[XM][TJ][TK];
 #pragma unroll
-  for (int p = 0; p < NPT; p++) { ax[p] = ay[p] = az[p] = 0.0; hit[p] = false; }
+  for (int m = 0; m < XM; m++)
+#pragma unroll
+    for (int r = 0; r < TJ; r++)
+#pragma unroll
+      for (int q = 0; q < TK; q++) {
+        acc[m][r][q][0] = acc[m][r][q][1] = acc[m][r][q][2] = 0.0;
+        hit[m][r][q] = false;
+      }
   RecSlots slots{a.slots};
   stage_tile<D, RecSlots, MarkerRec, SPREAD_WIN>(
       slots, a.nslots, g, tl, win, sh,
@@ -234,33 +256,36 @@ __global__ void __launch_bounds__(TILE_THREADS) k_spread(SpreadArgs a) {
         build_bins(n, nbin, cnt, start, list, sh,
                    [&](int c) { return win[c].valid ? win[c].i0 + 2 : -1; });
 #pragma unroll
-        for (int p = 0; p < NPT; p++) {
+        for (int m = 0; m < XM; m++) {
 #pragma clang fp contract(off)
-          int x, j, k;
-          if (!tile_node(g, ts, J0, K0, p, x, j, k)) continue;
-          const Img xa = images(x + 1, g.nx, w.type[0][0], w.type[0][1]);
-          const Img ya = images(j + 1, g.ny, w.type[1][0], w.type[1][1]);
-          const Img za = (D == 3) ? images(k + 1, g.nz, w.type[2][0], w.type[2][1]) : Img{1, 0, -9, -9};
-          for (int ia = 0; ia < xa.n; ia++) {
-            const int aa = xa.at(ia);
-            for (int ib = 0; ib < ya.n; ib++) {
-              const int bb = ya.at(ib);
-              for (int ic = 0; ic < za.n; ic++) {
-                const int cc = za.at(ic);
-                for (int kb = aa; kb <= aa + 3; kb++) {  // base cells aa-2 .. aa+1
-                  if (kb >= nbin) break;
-                  const int t = aa - kb + 3;
-                  for (int e = start[kb]; e < start[kb + 1]; e++) {
-                    const MarkerRec& m = win[list[e]];
-                    const int u = bb - m.j0 + 1;
-                    const int v = (D == 3) ? cc - m.k0 + 1 : 0;
-                    if (u < 0 || u > 3 || v < 0 || v > 3) continue;
-                    const double d = (D == 3) ? (m.wx[t] * m.wy[u]) * m.wz[v] : m.wx[t] * m.wy[u];
-                    const double wgt = d * m.sc;
-                    ax[p] += m.fx * wgt;
-                    ay[p] += m.fy * wgt;
-                    az[p] += m.fz * wgt;
-                    hit[p] = true;
+          if (!tn.xin[m]) continue;
+          for (int ia = 0; ia < tn.xa[m].n; ia++) {
+            const int aa = tn.xa[m].at(ia);
+            for (int kb = aa; kb <= aa + 3 && kb < nbin; kb++) {  // base cells aa-2 .. aa+1
+              const int t = aa - kb + 3;
+              for (int e = start[kb]; e < start[kb + 1]; e++) {
+                const MarkerRec& c = win[list[e]];
+                const double wxt = c.wx[t];
+#pragma unroll
+                for (int r = 0; r < TJ; r++) {
+                  if (!tn.yin[r]) continue;
+#pragma unroll
+                  for (int q = 0; q < TK; q++) {
+                    if (!tn.zin[q]) continue;
+                    for (int ib = 0; ib < tn.ya[r].n; ib++) {
+                      const int u = tn.ya[r].at(ib) - c.j0 + 1;
+                      if (u < 0 || u > 3) continue;
+                      for (int ic = 0; ic < tn.za[q].n; ic++) {
+                        const int v = (D == 3) ? tn.za[q].at(ic) - c.k0 + 1 : 0;
+                        if (v < 0 || v > 3) continue;
+                        const double d = (D == 3) ? (wxt * c.wy[u]) * c.wz[v] : wxt * c.wy[u];
+                        const double wgt = d * c.sc;
+                        acc[m][r][q][0] += c.fx * wgt;
+                        acc[m][r][q][1] += c.fy * wgt;
+                        acc[m][r][q][2] += c.fz * wgt;
+                        hit[m][r][q] = true;
+                      }
+                    }
                   }
                 }
               }
@@ -269,17 +294,20 @@ __global__ void __launch_bounds__(TILE_THREADS) k_spread(SpreadArgs a) {
         }
       });
 #pragma unroll
-  for (int p = 0; p < NPT; p++) {
-    int x, j, k;
-    if (!tile_node(g, ts, J0, K0, p, x, j, k)) continue;
-    const unsigned node = (unsigned)x + (unsigned)g.nx * ((unsigned)j + (unsigned)g.ny * (unsigned)k);
-    a.flag[node] = hit[p] ? 1 : 0;
-    if (hit[p]) {
-      a.Fx[node] = ax[p];
-      a.Fy[node] = ay[p];
-      a.Fz[node] = az[p];
-    }
-  }
+  for (int m = 0; m < XM; m++)
+#pragma unroll
+    for (int r = 0; r < TJ; r++)
+#pragma unroll
+      for (int q = 0; q < TK; q++) {
+        if (!tn.xin[m] || !tn.yin[r] || !tn.zin[q]) continue;
+        const unsigned node = (unsigned)tn.x[m] + (unsigned)g.nx * ((unsigned)tn.j[r] + (unsigned)g.ny * (unsigned)tn.k[q]);
+        a.flag[node] = hit[m][r][q] ? 1 : 0;
+        if (hit[m][r][q]) {
+          a.Fx[node] = acc[m][r][q][0];
+          a.Fy[node] = acc[m][r][q][1];
+          a.Fz[node] = acc[m][r][q][2];
+        }
+      }
 }
 
 struct BaseSlots {

@@ -120,19 +120,24 @@ struct BaseSlot {
 
 // Which interior nodes some marker stencil (or its ghost-plane image)
 // covers: cover[node] = 1/0 for every node of the tile.
-template <int D>
+template <int D, int TJ, int TK, int XM>
 __global__ void __launch_bounds__(TILE_THREADS) k_cover(Geo g, Walls w, BaseSlot bs,
                                                         unsigned char* cover) {
   __shared__ Base win[COVER_WIN];
   __shared__ int cnt[MAX_BIN], start[MAX_BIN + 1], list[COVER_WIN], sh[TILE_THREADS / 64];
   __shared__ TileLists tl;
-  const TileShape ts = tile_shape(D, g.nx);
-  const int J0 = 1 + blockIdx.y * ts.TJ, K0 = 1 + blockIdx.z * ts.TK;
+  const TileShape ts{TJ, TK, XM};
+  const int J0 = 1 + blockIdx.y * TJ, K0 = 1 + blockIdx.z * TK;
   const int nbin = g.nx + 5;
   tile_lists<D>(g, w, J0, K0, ts, tl);
-  bool cov[NPT];
+  const TileNodes<D, TJ, TK, XM> tn(g, w, J0, K0);
+  bool cov[XM][TJ][TK];
 #pragma unroll
-  for (int p = 0; p < NPT; p++) cov[p] = false;
+  for (int m = 0; m < XM; m++)
+#pragma unroll
+    for (int r = 0; r < TJ; r++)
+#pragma unroll
+      for (int q = 0; q < TK; q++) cov[m][r][q] = false;
   BaseSlots slots{bs.rowstart};
   stage_tile<D, BaseSlots, Base, COVER_WIN>(
       slots, 1, g, tl, win, sh, [&](int, int idx, Base& c) { c = bs.rec[idx]; },
@@ -140,33 +145,37 @@ __global__ void __launch_bounds__(TILE_THREADS) k_cover(Geo g, Walls w, BaseSlot
         build_bins(n, nbin, cnt, start, list, sh,
                    [&](int c) { return win[c].valid ? win[c].i0 + 2 : -1; });
 #pragma unroll
-        for (int p = 0; p < NPT; p++) {
-          int x, j, k;
-          if (cov[p] || !tile_node(g, ts, J0, K0, p, x, j, k)) continue;
-          const Img xa = images(x + 1, g.nx, w.type[0][0], w.type[0][1]);
-          const Img ya = images(j + 1, g.ny, w.type[1][0], w.type[1][1]);
-          const Img za = (D == 3) ? images(k + 1, g.nz, w.type[2][0], w.type[2][1]) : Img{1, 0, -9, -9};
-          for (int ia = 0; ia < xa.n && !cov[p]; ia++) {
-            const int aa = xa.at(ia);
-            for (int kb = aa; kb <= aa + 3 && kb < nbin && !cov[p]; kb++)
-              for (int e = start[kb]; e < start[kb + 1] && !cov[p]; e++) {
-                const Base& m = win[list[e]];
-                for (int ib = 0; ib < ya.n; ib++)
-                  for (int ic = 0; ic < za.n; ic++) {
-                    const int u = ya.at(ib) - m.j0 + 1;
-                    const int v = (D == 3) ? za.at(ic) - m.k0 + 1 : 0;
-                    if (u >= 0 && u <= 3 && v >= 0 && v <= 3) cov[p] = true;
-                  }
+        for (int m = 0; m < XM; m++) {
+          if (!tn.xin[m]) continue;
+          for (int ia = 0; ia < tn.xa[m].n; ia++) {
+            const int aa = tn.xa[m].at(ia);
+            for (int kb = aa; kb <= aa + 3 && kb < nbin; kb++)
+              for (int e = start[kb]; e < start[kb + 1]; e++) {
+                const Base& c = win[list[e]];
+#pragma unroll
+                for (int r = 0; r < TJ; r++)
+#pragma unroll
+                  for (int q = 0; q < TK; q++)
+                    for (int ib = 0; ib < tn.ya[r].n; ib++)
+                      for (int ic = 0; ic < tn.za[q].n; ic++) {
+                        const int u = tn.ya[r].at(ib) - c.j0 + 1;
+                        const int v = (D == 3) ? tn.za[q].at(ic) - c.k0 + 1 : 0;
+                        if (u >= 0 && u <= 3 && v >= 0 && v <= 3) cov[m][r][q] = true;
+                      }
               }
           }
         }
       });
 #pragma unroll
-  for (int p = 0; p < NPT; p++) {
-    int x, j, k;
-    if (!tile_node(g, ts, J0, K0, p, x, j, k)) continue;
-    cover[(unsigned)x + (unsigned)g.nx * ((unsigned)j + (unsigned)g.ny * (unsigned)k)] = cov[p] ? 1 : 0;
-  }
+  for (int m = 0; m < XM; m++)
+#pragma unroll
+    for (int r = 0; r < TJ; r++)
+#pragma unroll
+      for (int q = 0; q < TK; q++) {
+        if (!tn.xin[m] || !tn.yin[r] || !tn.zin[q]) continue;
+        cover[(unsigned)tn.x[m] + (unsigned)g.nx * ((unsigned)tn.j[r] + (unsigned)g.ny * (unsigned)tn.k[q])] =
+            cov[m][r][q] ? 1 : 0;
+      }
 }
 
 // Fluid velocity u = j/rho (compute_macroscopic(false), couplb_lattice.h:

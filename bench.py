@@ -110,6 +110,22 @@ def copy_calibration(mb, reps):
                       "copy_gbs": 2 * n * 8 / (ms * 1e-3) / 1e9}), flush=True)
 
 
+def pmc_traffic(storage, n):
+    """HBM bytes per k_step launch measured by rocprofv3 --pmc FETCH_SIZE and
+    WRITE_SIZE (scaled by a plain copy of known size), read from the
+    committed summary of that measurement; None when there is none."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        f"pmc_k_step_d3q19_{n}cube_{storage}.md")
+    try:
+        with open(path) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -118,8 +134,8 @@ def main():
     ap.add_argument("--n", type=int, default=256, help="grid edge (interior nodes)")
     ap.add_argument("--particles", type=int, default=100000)
     ap.add_argument("--storage", default="fp64", choices=["fp64", "fp32", "fp16", "bf16"])
-    ap.add_argument("--cpu-steps", type=int, default=3)
-    ap.add_argument("--cpu-nz", type=int, default=32)
+    ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--cpu-nz", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="time a plain device copy of MB megabytes (PMC byte-counter scale)")
@@ -153,19 +169,22 @@ def main():
                   inv_N=1.0 / md, force_scale=1.0)
     g = (1e-6, 0.0, 0.0)
 
-    def ibm(first=False):
+    # The workload's markers are prescribed and fixed, so every sub-step but
+    # the very first passes positions_unchanged.  moving=True re-declares the
+    # positions as new at every sub-step (the path of moving particles).
+    def ibm(first=False, moving=False):
         dev.ibm_device(n, tx.data_ptr(), tv.data_ptr(), tm.data_ptr(), tf.data_ptr(),
-                       positions_unchanged=not first, **ibm_kw)
+                       positions_unchanged=not (first or moving), **ibm_kw)
 
     ibm(first=True)  # FixCoupLB::setup primes the IBM force (fix_couplb.cpp:558)
     lb_ms, ibm_ms, spread_ms = [], [], []
 
-    def one_step():
+    def one_step(moving=False):
         dev.step(g, couplb_amd.RHO_REFRESHED_NEXT)
         lb_ms.append(dev.last_ms(0))
         spread_ms.append(dev.last_ms(2))
         for _ in range(md):
-            ibm()
+            ibm(moving=moving)
             ibm_ms.append(dev.last_ms(1))
 
     for _ in range(args.warmup):
@@ -204,6 +223,18 @@ def main():
     dev.sync()
     lb_only = nodes * dev2_steps / (time.perf_counter() - t1) / 1e6
 
+    # same workload with the positions declared new at every sub-step: the
+    # markers are re-sorted, the coverage rebuilt and the velocity
+    # re-interpolated each time, as for particles that move
+    mv_steps = max(4, args.steps // 4)
+    one_step(moving=True)
+    dev.sync()
+    t2 = time.perf_counter()
+    for _ in range(mv_steps):
+        one_step(moving=True)
+    dev.sync()
+    mlups_moving = nodes * mv_steps / (time.perf_counter() - t2) / 1e6
+
     cpu = None
     if not args.no_cpu_baseline:
         log("bench: cpu baseline")
@@ -229,11 +260,13 @@ def main():
                                f"md_per_lb={md}, u0=0.02, g_x=1e-6, nu=0.1",
                    "grid": [N, N, N], "markers": n, "md_per_lb": md, "storage": args.storage},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": pmc_traffic(args.storage, N),
                      "kernel": f"k_step<3,{args.storage}> (fused pull + IBM force gather)",
                      "bytes_per_lup": b_lup, "avg_kernel_ms": round(lb_avg, 4)},
         "ibm": {"markers_per_s": round(markers_per_s, 1), "avg_substep_ms": round(ibm_avg, 4),
                 "avg_spread_ms_per_lb_step": round(spread_avg, 4), "markers": n},
+        "mlups_markers_moving": round(mlups_moving, 1),
         "lb_only_mlups": round(lb_only, 1),
         "lb_only_hbm_frac": round(lb_only * 1e6 * b_lup / (HBM_PEAK_GBS * 1e9), 4),
         "mass": diag["mass"],

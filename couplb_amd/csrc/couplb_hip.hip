@@ -14,6 +14,7 @@
 #include "gather.hpp"
 #include "ibm.hpp"
 #include "kernels.hpp"
+#include "spread_pairs.hpp"
 
 using namespace couplb;
 
@@ -23,6 +24,7 @@ struct SlotHost {
   int* rowstart = nullptr;
   MarkerRec* rec = nullptr;
   int n = 0;
+  int epoch = -1;  // marker ordering the records were built from
 };
 
 }  // namespace
@@ -64,6 +66,20 @@ struct couplb_ctx {
   size_t sort_tmp_bytes = 0;
   double *dx = nullptr, *dv = nullptr, *dm = nullptr, *df = nullptr;
   int order_n = -1;       // marker count of the cached ordering
+  double order_key[5] = {0, 0, 0, 0, -1};  // lo, inv_dx, kernel of the cached ordering
+  int pos_epoch = 0;      // incremented whenever the ordering is rebuilt
+  int cover_epoch = -1;   // ordering the coverage flags were built from
+  double* ufc = nullptr;  // interpolated fluid velocity per sorted marker (3*cap)
+  int uf_cycle = -1, uf_epoch = -1;  // state and ordering of ufc
+  // fixed-marker spreading (spread_pairs.hpp)
+  unsigned *pkey = nullptr, *pkey_s = nullptr;
+  int *pval = nullptr, *pval_s = nullptr, *pp = nullptr;
+  double* pw = nullptr;
+  void* ptmp = nullptr;
+  size_t ptmp_bytes = 0;
+  long pcap = 0, pne = 0;
+  int pair_epoch = -1;       // ordering the pairs (and the node flags) were built for
+  int prev_step_epoch = -1;  // ordering of the single record slot at the previous LB step
   int ugather_cycle = -1; // state counter of the last velocity gather
   std::vector<SlotHost> slots;
   int nslots = 0;
@@ -175,12 +191,13 @@ template <int D, typename S> struct LaunchExport {
   }
 };
 template <int D, typename S> struct LaunchUgather {
-  static void run(couplb_ctx* c, const PullCtx& pc, const BaseSlot& bs) {
+  static void run(couplb_ctx* c, const PullCtx& pc, const BaseSlot& bs, bool cover) {
     const TileShape ts = tile_shape(D, c->geo.nx);
 #define COVER_LAUNCH(TJ, TK, XM)                                                                  \
   hipLaunchKernelGGL((k_cover<D, TJ, TK, XM>), tile_grid(c->geo), dim3(TILE_THREADS), 0, c->stream, \
                      c->geo, c->walls, bs, c->cover)
-    if (ts.XM == 1) {
+    if (!cover) {
+    } else if (ts.XM == 1) {
       if (D == 3) COVER_LAUNCH(2, 2, 1); else COVER_LAUNCH(4, 1, 1);
     } else if (ts.XM == 2) {
       COVER_LAUNCH(2, 1, 2);
@@ -221,9 +238,9 @@ int run_macro(couplb_ctx* c, bool want_fields) {
   return COUPLB_OK;
 }
 
-unsigned key_bits(int nrows) {
+unsigned key_bits(size_t nrows) {
   unsigned b = 1;
-  while ((1u << b) <= (unsigned)nrows) b++;
+  while (b < 32 && ((size_t)1 << b) <= nrows) b++;
   return b;
 }
 
@@ -246,6 +263,7 @@ int ensure_ibm(couplb_ctx* c, int n) {
   const int cap = std::max(n, 2 * c->cap);
   (void)hipFree(c->keys); (void)hipFree(c->skeys); (void)hipFree(c->vals); (void)hipFree(c->perm); (void)hipFree(c->bases);
   (void)hipFree(c->sort_tmp); (void)hipFree(c->dx); (void)hipFree(c->dv); (void)hipFree(c->dm); (void)hipFree(c->df);
+  (void)hipFree(c->ufc);
   HIPCHK(c, hipMalloc(&c->keys, sizeof(unsigned) * cap));
   HIPCHK(c, hipMalloc(&c->skeys, sizeof(unsigned) * cap));
   HIPCHK(c, hipMalloc(&c->vals, sizeof(int) * cap));
@@ -255,6 +273,7 @@ int ensure_ibm(couplb_ctx* c, int n) {
   HIPCHK(c, hipMalloc(&c->dv, sizeof(double) * 3 * cap));
   HIPCHK(c, hipMalloc(&c->dm, sizeof(double) * cap));
   HIPCHK(c, hipMalloc(&c->df, sizeof(double) * 3 * cap));
+  HIPCHK(c, hipMalloc(&c->ufc, sizeof(double) * 3 * cap));
   size_t bytes = 0;
   HIPCHK(c, rocprim::radix_sort_pairs(nullptr, bytes, c->keys, c->skeys, c->vals, c->perm,
                                       (size_t)cap, 0u, key_bits(c->geo.nrows), c->stream));
@@ -267,6 +286,7 @@ int ensure_ibm(couplb_ctx* c, int n) {
   }
   c->cap = cap;
   c->order_n = -1;
+  c->uf_epoch = -1;
   c->slot_table_n = -1;
   return COUPLB_OK;
 }
@@ -289,10 +309,6 @@ int ibm_impl(couplb_ctx* c, int n, const double* x, const double* v, const doubl
   if (n == 0) return COUPLB_OK;
   int rc = ensure_ibm(c, n);
   if (rc) return rc;
-  const int s = c->nslots;
-  rc = ensure_slot(c, s);
-  if (rc) return rc;
-  if (c->slots[s].rec == nullptr || c->cap < n) return fail(c, COUPLB_ENOMEM, "slot capacity");
   IbmParams ip;
   for (int d = 0; d < 3; d++) ip.lo[d] = a->lo[d];
   ip.inv_dx = a->inv_dx;
@@ -303,7 +319,9 @@ int ibm_impl(couplb_ctx* c, int n, const double* x, const double* v, const doubl
   ip.kern = c->p.kernel;
   const int tb = 256, nb = (n + tb - 1) / tb;
   HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
-  const bool reorder = !(a->positions_unchanged && c->order_n == n);
+  const double key[5] = {a->lo[0], a->lo[1], a->lo[2], a->inv_dx, (double)c->p.kernel};
+  const bool reorder = !(a->positions_unchanged && c->order_n == n &&
+                         std::memcmp(key, c->order_key, sizeof(key)) == 0);
   if (reorder) {
     hipLaunchKernelGGL(k_keys, dim3(nb), dim3(tb), 0, c->stream, n, x, ip, c->geo, c->keys, c->vals);
     HIPCHK(c, hipGetLastError());
@@ -315,30 +333,107 @@ int ibm_impl(couplb_ctx* c, int n, const double* x, const double* v, const doubl
     hipLaunchKernelGGL(k_bases, dim3(nb), dim3(tb), 0, c->stream, n, c->perm, x, ip, c->geo, c->bases);
     HIPCHK(c, hipGetLastError());
     c->order_n = n;
+    std::memcpy(c->order_key, key, sizeof(key));
+    c->pos_epoch++;
   }
   if (reorder || c->ugather_cycle != c->cycle) {
+    // coverage depends on the marker cells only, the velocities on the state
     BaseSlot bs{c->rowstart_cur, c->bases};
     PullCtx pc = make_pull(c);
-    dispatch<LaunchUgather>(c, pc, bs);
+    dispatch<LaunchUgather>(c, pc, bs, c->cover_epoch != c->pos_epoch);
     HIPCHK(c, hipGetLastError());
+    c->cover_epoch = c->pos_epoch;
     c->ugather_cycle = c->cycle;
   }
+  // Same positions and same LB state as the previous sub-step of this
+  // interval: u_f is known and only the force is new (k_marker_force).
+  const bool merge = !reorder && c->nslots > 0 && c->slots[c->nslots - 1].epoch == c->pos_epoch &&
+                     c->slots[c->nslots - 1].n == n && c->uf_cycle == c->cycle &&
+                     c->uf_epoch == c->pos_epoch;
+  if (merge) {
+    SlotHost& sl = c->slots[c->nslots - 1];
+    hipLaunchKernelGGL(k_marker_force, dim3(nb), dim3(tb), 0, c->stream, n, c->perm, v, m, f_dev, fmode,
+                       ip, (const double*)c->ufc, sl.rec);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+    c->rho_fresh = 1;  // compute_macroscopic(false) at fix_couplb.cpp:758
+    return COUPLB_OK;
+  }
+  const int s = c->nslots;
+  rc = ensure_slot(c, s);
+  if (rc) return rc;
+  if (c->slots[s].rec == nullptr || c->cap < n) return fail(c, COUPLB_ENOMEM, "slot capacity");
   SlotHost& sl = c->slots[s];
   if (c->D == 3) {
     const int nb4 = (4 * n + tb - 1) / tb;  // four lanes per marker
     hipLaunchKernelGGL((k_marker<3>), dim3(nb4), dim3(tb), 0, c->stream, n, c->perm, x, v, m, f_dev,
-                       fmode, ip, c->geo, c->walls, c->u3, sl.rec);
+                       fmode, ip, c->geo, c->walls, c->u3, sl.rec, c->ufc);
   } else {
     hipLaunchKernelGGL((k_marker<2>), dim3(nb), dim3(tb), 0, c->stream, n, c->perm, x, v, m, f_dev,
-                       fmode, ip, c->geo, c->walls, c->u3, sl.rec);
+                       fmode, ip, c->geo, c->walls, c->u3, sl.rec, c->ufc);
   }
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipMemcpyAsync(sl.rowstart, c->rowstart_cur, sizeof(int) * (c->geo.nrows + 1),
                            hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
   sl.n = n;
+  sl.epoch = c->pos_epoch;
+  c->uf_cycle = c->cycle;
+  c->uf_epoch = c->pos_epoch;
   c->nslots = s + 1;
   c->rho_fresh = 1;  // compute_macroscopic(false) at fix_couplb.cpp:758
+  return COUPLB_OK;
+}
+
+// Sparse spreading map of the single record slot (spread_pairs.hpp); also
+// rewrites every node flag.
+int build_pairs(couplb_ctx* c, const SlotHost& sl) {
+  const int S = (c->D == 3) ? stencil_size<3>() : stencil_size<2>();
+  const long ne = (long)sl.n * S;
+  const unsigned sentinel = (unsigned)c->N;
+  const unsigned bits = key_bits(c->N);
+  if (ne > c->pcap) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    (void)hipFree(c->pkey); (void)hipFree(c->pkey_s); (void)hipFree(c->pval); (void)hipFree(c->pval_s);
+    (void)hipFree(c->pp); (void)hipFree(c->pw); (void)hipFree(c->ptmp);
+    c->pkey = c->pkey_s = nullptr; c->pval = c->pval_s = c->pp = nullptr; c->pw = nullptr; c->ptmp = nullptr;
+    c->pcap = 0;
+    HIPCHK(c, hipMalloc(&c->pkey, sizeof(unsigned) * ne));
+    HIPCHK(c, hipMalloc(&c->pkey_s, sizeof(unsigned) * ne));
+    HIPCHK(c, hipMalloc(&c->pval, sizeof(int) * ne));
+    HIPCHK(c, hipMalloc(&c->pval_s, sizeof(int) * ne));
+    HIPCHK(c, hipMalloc(&c->pp, sizeof(int) * ne));
+    HIPCHK(c, hipMalloc(&c->pw, sizeof(double) * ne));
+    size_t bytes = 0;
+    HIPCHK(c, rocprim::radix_sort_pairs(nullptr, bytes, c->pkey, c->pkey_s, c->pval, c->pval_s,
+                                        (size_t)ne, 0u, bits, c->stream));
+    HIPCHK(c, hipMalloc(&c->ptmp, bytes));
+    c->ptmp_bytes = bytes;
+    c->pcap = ne;
+  }
+  const int tb = 256;
+  const unsigned nb = (unsigned)((ne + tb - 1) / tb);
+  if (c->D == 3)
+    hipLaunchKernelGGL(k_pairs<3>, dim3(nb), dim3(tb), 0, c->stream, sl.n, c->geo, c->walls, sl.rec,
+                       sentinel, c->pkey, c->pval);
+  else
+    hipLaunchKernelGGL(k_pairs<2>, dim3(nb), dim3(tb), 0, c->stream, sl.n, c->geo, c->walls, sl.rec,
+                       sentinel, c->pkey, c->pval);
+  HIPCHK(c, hipGetLastError());
+  size_t bytes = c->ptmp_bytes;
+  HIPCHK(c, rocprim::radix_sort_pairs(c->ptmp, bytes, c->pkey, c->pkey_s, c->pval, c->pval_s,
+                                      (size_t)ne, 0u, bits, c->stream));
+  hipLaunchKernelGGL(k_fill_u8, dim3((unsigned)((c->N + tb - 1) / tb)), dim3(tb), 0, c->stream, c->flag,
+                     c->N, (unsigned char)0);
+  if (c->D == 3)
+    hipLaunchKernelGGL(k_pair_weights<3>, dim3(nb), dim3(tb), 0, c->stream, ne, c->pkey_s, c->pval_s,
+                       sentinel, sl.rec, c->pp, c->pw, c->flag);
+  else
+    hipLaunchKernelGGL(k_pair_weights<2>, dim3(nb), dim3(tb), 0, c->stream, ne, c->pkey_s, c->pval_s,
+                       sentinel, sl.rec, c->pp, c->pw, c->flag);
+  HIPCHK(c, hipGetLastError());
+  c->pne = ne;
+  c->pair_epoch = sl.epoch;
   return COUPLB_OK;
 }
 
@@ -359,8 +454,23 @@ int step_impl(couplb_ctx* c, const double g[3], int flags, bool timed) {
   a.has_ibm = c->nslots > 0;
   a.flag = c->flag;
   a.Fx = c->Fx; a.Fy = c->Fy; a.Fz = c->Fz;
-  if (c->nslots > 0) {
+  const int S = (c->D == 3) ? stencil_size<3>() : stencil_size<2>();
+  // markers at the positions of the previous LB interval: constant map
+  const bool fixed = c->nslots == 1 && c->slots[0].epoch == c->prev_step_epoch &&
+                     (long)c->slots[0].n * S < (long)INT32_MAX;
+  if (fixed) {
+    if (timed) HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
+    if (c->pair_epoch != c->slots[0].epoch) {
+      const int rc = build_pairs(c, c->slots[0]);
+      if (rc) return rc;
+    }
+    hipLaunchKernelGGL(k_pair_spread, dim3((unsigned)((c->pne + 255) / 256)), dim3(256), 0, c->stream,
+                       c->pne, (const unsigned*)c->pkey_s, (const int*)c->pp, (const double*)c->pw,
+                       (unsigned)c->N, (const MarkerRec*)c->slots[0].rec, c->Fx, c->Fy, c->Fz);
+    HIPCHK(c, hipGetLastError());
+  } else if (c->nslots > 0) {
     // spread the pending marker records into the per-node force (k_spread)
+    c->pair_epoch = -1;  // k_spread rewrites the node flags
     if (!c->slot_table) HIPCHK(c, hipMalloc(&c->slot_table, sizeof(Slot) * MAX_SLOTS));
     if (c->slot_table_n != (int)c->slots.size()) {
       Slot h[MAX_SLOTS];
@@ -398,6 +508,7 @@ int step_impl(couplb_ctx* c, const double g[3], int flags, bool timed) {
   dispatch<LaunchStep>(c, a);
   HIPCHK(c, hipGetLastError());
   if (timed) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+  if (c->nslots > 0) c->prev_step_epoch = (c->nslots == 1) ? c->slots[0].epoch : -1;
   c->cur = dst;
   c->local = 0;
   c->nslots = 0;
@@ -513,7 +624,10 @@ int couplb_destroy(couplb_ctx* c) {
   (void)hipFree(c->keys); (void)hipFree(c->skeys); (void)hipFree(c->vals); (void)hipFree(c->perm);
   (void)hipFree(c->rowstart_cur); (void)hipFree(c->bases); (void)hipFree(c->sort_tmp);
   (void)hipFree(c->dx); (void)hipFree(c->dv); (void)hipFree(c->dm); (void)hipFree(c->df);
+  (void)hipFree(c->ufc);
   (void)hipFree(c->staging);
+  (void)hipFree(c->pkey); (void)hipFree(c->pkey_s); (void)hipFree(c->pval); (void)hipFree(c->pval_s);
+  (void)hipFree(c->pp); (void)hipFree(c->pw); (void)hipFree(c->ptmp);
   for (auto& s : c->slots) { (void)hipFree(s.rowstart); (void)hipFree(s.rec); }
   (void)hipFree(c->slot_table);
   for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);

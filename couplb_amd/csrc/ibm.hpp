@@ -226,7 +226,7 @@ template <int D>
 __global__ void __launch_bounds__(256) k_marker(int n, const int* perm, const double* x,
                                                 const double* v, const double* m, double* f_atom,
                                                 int fmode, IbmParams ip, Geo g, Walls w,
-                                                const double* u3, MarkerRec* rec) {
+                                                const double* u3, MarkerRec* rec, double* ufc) {
 #pragma clang fp contract(off)
   constexpr int LPM = (D == 3) ? 4 : 1;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -305,6 +305,9 @@ __global__ void __launch_bounds__(256) k_marker(int n, const int* perm, const do
     const double iw = 1.0 / ws;
     uf0 = ui * iw; uf1 = vi * iw; uf2 = wi * iw;
   }
+  ufc[3 * (size_t)p + 0] = uf0;
+  ufc[3 * (size_t)p + 1] = uf1;
+  ufc[3 * (size_t)p + 2] = uf2;
   const double* vp = v + 3 * (size_t)idx;
   const double mm = m[idx];
   const double Fx = mm * ip.xi_over_dtLB * (uf0 * ip.vel_scale - vp[0]);
@@ -323,6 +326,36 @@ __global__ void __launch_bounds__(256) k_marker(int n, const int* perm, const do
   r.valid = (fin && !(wsf < DELTA_TOL)) ? 1 : 0;
   r.sc = r.valid ? 1.0 / wsf : 0.0;
   rec[p] = r;
+}
+
+// A later sub-step with unchanged positions in the same LB state: the
+// stencil, the fluid velocity field and therefore u_f (ufc, written by
+// k_marker) are the same as at the first sub-step, so only the penalty force
+// (fix_couplb.cpp:779-785) is recomputed from the current v and m.  Its
+// reaction is added into the first sub-step's record; spreading the summed
+// reaction once equals spreading each sub-step's reaction (couplb_ibm.h:
+// 125-130 is linear in the marker force) up to summation order.
+__global__ void __launch_bounds__(256) k_marker_force(int n, const int* perm, const double* v,
+                                                      const double* m, double* f_atom, int fmode,
+                                                      IbmParams ip, const double* ufc,
+                                                      MarkerRec* rec) {
+#pragma clang fp contract(off)
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int idx = perm[p];
+  const double uf0 = ufc[3 * (size_t)p + 0], uf1 = ufc[3 * (size_t)p + 1], uf2 = ufc[3 * (size_t)p + 2];
+  const double* vp = v + 3 * (size_t)idx;
+  const double mm = m[idx];
+  const double Fx = mm * ip.xi_over_dtLB * (uf0 * ip.vel_scale - vp[0]);
+  const double Fy = mm * ip.xi_over_dtLB * (uf1 * ip.vel_scale - vp[1]);
+  const double Fz = mm * ip.xi_over_dtLB * (uf2 * ip.vel_scale - vp[2]);
+  double* fa = f_atom + 3 * (size_t)idx;
+  if (fmode == 0) { fa[0] += Fx; fa[1] += Fy; fa[2] += Fz; }
+  else { fa[0] = Fx; fa[1] = Fy; fa[2] = Fz; }
+  MarkerRec& r = rec[p];
+  r.fx = r.fx + -Fx * ip.inv_N / ip.force_scale;
+  r.fy = r.fy + -Fy * ip.inv_N / ip.force_scale;
+  r.fz = r.fz + -Fz * ip.inv_N / ip.force_scale;
 }
 
 }  // namespace couplb

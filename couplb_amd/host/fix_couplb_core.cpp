@@ -383,7 +383,12 @@ void FixCoupLBCore::ibm_sub_coupling(const couplb_atoms& a, bool first) {
   ia.vel_scale = vel_scale;
   ia.inv_N = 1.0 / static_cast<double>(md_per_lb);
   ia.force_scale = force_scale;
-  ia.positions_unchanged = 0;
+  // same atoms at bit-identical positions as the previous sub-step: the
+  // device keeps its marker ordering, stencils and fluid velocity
+  ia.positions_unchanged =
+      (px_prev.size() == px.size() && std::memcmp(px_prev.data(), px.data(), sizeof(double) * px.size()) == 0)
+          ? 1 : 0;
+  px_prev = px;
   check(couplb_ibm_substep(ctx, ng, px.data(), pv.data(), pm.data(), pf.data(), &ia), "IBM sub-step");
   for (int k = 0; k < ng; k++)
     for (int d = 0; d < 3; d++) a.f[3 * pidx[k] + d] = pf[3 * k + d];

@@ -78,6 +78,7 @@ class FixCoupLBCore {
   std::string screen;
 
   std::vector<double> px, pv, pm, pf;
+  std::vector<double> px_prev;  // positions of the previous sub-step (reuse test)
   std::vector<int> pidx;
 
   void say(const char* fmt, ...);

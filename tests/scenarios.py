@@ -120,3 +120,48 @@ def interior(f, dimension):
 def rel_l2(a, b):
     den = np.linalg.norm(b)
     return np.linalg.norm(a - b) / (den if den > 0 else 1.0)
+
+
+class DeviceArray:
+    """FP64 device buffer for the device-pointer entry points, allocated with
+    the HIP runtime the library itself links (no second runtime in the
+    process)."""
+    _hip = None
+
+    @classmethod
+    def hip(cls):
+        if cls._hip is None:
+            import ctypes as C
+            import couplb_amd
+            couplb_amd.lib()            # loads libamdhip64 as a dependency
+            h = C.CDLL("libamdhip64.so.7")
+            h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+            h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+            h.hipFree.argtypes = [C.c_void_p]
+            cls._hip = h
+        return cls._hip
+
+    def __init__(self, a):
+        import ctypes as C
+        self.host = np.ascontiguousarray(a, np.float64)
+        self.ptr = C.c_void_p()
+        assert self.hip().hipMalloc(C.byref(self.ptr), max(8, self.host.nbytes)) == 0
+        self.upload(self.host)
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, np.float64)
+        assert a.shape == self.host.shape
+        assert self.hip().hipMemcpy(self.ptr, a.ctypes.data, a.nbytes, 1) == 0
+
+    def download(self):
+        out = np.empty_like(self.host)
+        assert self.hip().hipMemcpy(out.ctypes.data, self.ptr, out.nbytes, 2) == 0
+        return out
+
+    @property
+    def addr(self):
+        return self.ptr.value
+
+    def __del__(self):
+        if self._hip is not None and self.ptr:
+            self._hip.hipFree(self.ptr)

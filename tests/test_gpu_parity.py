@@ -131,3 +131,68 @@ def test_state_roundtrip_is_exact():
         g = d.get_state()
         fi, gi = interior(f, dim), interior(g, dim)
         assert np.array_equal(fi, gi)
+
+
+@pytest.mark.parametrize("walls", [WALLS[0], WALLS[2], WALLS[4], WALLS[5]])
+def test_fixed_markers_reuse_matches_oracle(walls):
+    """Markers at fixed positions whose velocities change at every MD step.
+    The fix passes positions_unchanged, so the device keeps the ordering, the
+    coverage and the interpolated fluid velocity over the sub-steps of one LB
+    interval and spreads the summed reaction once; the oracle recomputes
+    everything at every sub-step as the reference does."""
+    w = " ".join(f"wall_{ax} {walls[d][0]} {walls[d][1]}" for d, ax in enumerate("xyz"))
+    cmd = (f"flow all couplb 12 10 14 0.1 1.0 {w} wall_vel 0.01 0.0 0.005 "
+           f"gravity 1e-5 2e-6 -1e-6 kernel peskin4 md_per_lb 4 check_every 5")
+    print("case fixed markers", cmd, flush=True)
+    base = _ibm_atoms(60, 10.0, 11)
+
+    def atoms_fn(step):
+        a = copy_atoms(base)
+        a["v"] = a["v"] * (1.0 + 0.5 * np.sin(0.7 * step))
+        return a
+
+    dev, fdv = run_device(cmd, 32, dimension=3, dt=0.25, atoms_fn=atoms_fn)
+    ref, fro = run_oracle(cmd, 32, dimension=3, dt=0.25, atoms_fn=atoms_fn)
+    fd = interior(dev.device_state().reshape(19, 16, 12, 14), 3)
+    fo = interior(ref.f(), 3)
+    err = rel_l2(fd, fo)
+    ferr = max(rel_l2(a, b) for a, b in zip(fdv, fro))
+    print(f"rel-L2(f)={err:.3e} rel-L2(F_atom)={ferr:.3e}")
+    assert err <= TOL
+    assert ferr <= 1e-11
+    dd, od = dev.last_diag(), ref.last_diag()
+    assert abs(dd["mass"] - od["mass"]) <= 1e-12 * od["mass"]
+
+
+def test_device_reuse_matches_full_recompute():
+    """Device API: the same four sub-steps with positions declared unchanged
+    (merged records) and declared new (one record per sub-step) give the
+    same populations and atom forces up to summation order."""
+    import couplb_amd
+    from tests.scenarios import DeviceArray
+    rng = np.random.default_rng(5)
+    n = 500
+    x = rng.uniform(0.0, 24.0, (n, 3))
+    res = []
+    for reuse in (True, False):
+        d = couplb_amd.Device(3, 24, 24, 24, tau=0.8, kernel=couplb_amd.PESKIN4, md_per_lb=4)
+        d.init_equilibrium(1.0, (0.02, 0.01, 0.0))
+        dx, dm = DeviceArray(x), DeviceArray(np.ones(n))
+        dv, df = DeviceArray(np.zeros((n, 3))), DeviceArray(np.zeros((n, 3)))
+        kw = dict(lo=(0.0, 0.0, 0.0), inv_dx=1.0, xi_over_dtLB=1.0, vel_scale=1.0,
+                  inv_N=0.25, force_scale=1.0)
+        for step in range(3):
+            d.step((1e-6, 0.0, 0.0), couplb_amd.RHO_REFRESHED_NEXT)
+            for sub in range(4):
+                d.sync()
+                dv.upload(np.full((n, 3), 1e-3 * (sub + 1)))
+                d.ibm_device(n, dx.addr, dv.addr, dm.addr, df.addr,
+                             positions_unchanged=reuse and not (step == 0 and sub == 0), **kw)
+        d.step((1e-6, 0.0, 0.0), couplb_amd.RHO_REFRESHED_NEXT)
+        d.sync()
+        res.append((d.get_state(), df.download()))
+    (f1, a1), (f2, a2) = res
+    print("reuse vs recompute: rel-L2(f) =", rel_l2(f1, f2), "rel-L2(F_atom) =", rel_l2(a1, a2))
+    assert np.all(np.isfinite(f1)) and np.abs(a1).max() > 0
+    assert rel_l2(f1, f2) <= 1e-14
+    assert rel_l2(a1, a2) <= 1e-14

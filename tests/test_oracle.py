@@ -355,3 +355,18 @@ def test_checkpoint_roundtrip_and_header_validation(tmp_path):
     open(fn, "wb").write(bytes(raw[:-8]))
     with pytest.raises(IOError, match="short read"):
         oracle.read_checkpoint(fn, 3, Q, dims, dims, nf)
+
+
+def test_opposite_moving_walls_couette_closed_form():
+    # per-face wall velocities (wall_vel_face): u = U_lo + (U_hi - U_lo)(y + 1/2)/H,
+    # the linear Couette profile that halfway bounce-back reproduces exactly
+    from tests.scenarios import run_oracle
+    cmd = ("flow all couplb 4 16 1 0.1 1.0 wall_y 2 2 "
+           "wall_vel_face y lo -0.04 0 0 wall_vel_face y hi 0.04 0 0")
+    fx, _ = run_oracle(cmd, 20000, dimension=2)
+    ux = fx.field(oracle.F_UX).reshape(18, 6)[1:-1, 1:-1]
+    y = np.arange(16.0)
+    exact = -0.04 + 0.08 * (y + 0.5) / 16
+    assert np.max(np.abs(ux - exact[:, None])) < 1e-10
+    uy = fx.field(oracle.F_UY).reshape(18, 6)[1:-1, 1:-1]
+    assert np.max(np.abs(uy)) < 1e-12

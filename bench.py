@@ -177,38 +177,37 @@ def main():
                        positions_unchanged=not (first or moving), **ibm_kw)
 
     ibm(first=True)  # FixCoupLB::setup primes the IBM force (fix_couplb.cpp:558)
-    lb_ms, ibm_ms, spread_ms = [], [], []
 
+    # every call is asynchronous on the context stream; kernel times come
+    # from HIP events on that stream, read back after the timed region
     def one_step(moving=False):
         dev.step(g, couplb_amd.RHO_REFRESHED_NEXT)
-        lb_ms.append(dev.last_ms(0))
-        spread_ms.append(dev.last_ms(2))
         for _ in range(md):
             ibm(moving=moving)
-            ibm_ms.append(dev.last_ms(1))
 
     for _ in range(args.warmup):
         one_step()
-    lb_ms.clear()
-    ibm_ms.clear()
-    spread_ms.clear()
     dev.sync()
     torch.cuda.synchronize()
+    dev.kernel_ms_reset()
     t0 = time.perf_counter()
     for s in range(args.steps):
         one_step()
         if (s + 1) % 10 == 0:
-            log(f"bench: step {s + 1}/{args.steps}")
+            log(f"bench: step {s + 1}/{args.steps} issued")
     dev.sync()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    lb_tot, lb_n = dev.kernel_ms_total(0)
+    ibm_tot, ibm_n = dev.kernel_ms_total(1)
+    sp_tot, sp_n = dev.kernel_ms_total(2)
     nodes = N ** 3
     mlups = nodes * args.steps / wall / 1e6
     s_bytes = {"fp64": 8, "fp32": 4, "fp16": 2, "bf16": 2}[args.storage]
     b_lup = 2 * Q * s_bytes
-    lb_avg = float(np.mean(lb_ms))
-    ibm_avg = float(np.mean(ibm_ms)) if ibm_ms else 0.0
-    spread_avg = float(np.mean(spread_ms)) if spread_ms else 0.0
+    lb_avg = lb_tot / max(lb_n, 1)
+    ibm_avg = ibm_tot / max(ibm_n, 1)
+    spread_avg = sp_tot / max(args.steps, 1)
     achieved = b_lup * nodes / (lb_avg * 1e-3) / 1e9
     # IBM time per marker sub-step: the sub-step kernels plus this sub-step's
     # share of the force spreading that precedes the next LB kernel

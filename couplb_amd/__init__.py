@@ -46,6 +46,19 @@ class IbmArgs(C.Structure):
                 ("positions_unchanged", C.c_int)]
 
 
+MDF_RICHARDSON, MDF_CG = 0, 1
+
+
+class MdfArgs(C.Structure):
+    _fields_ = [("ibm", IbmArgs), ("tol", C.c_double), ("relax", C.c_double),
+                ("max_iter", C.c_int), ("method", C.c_int)]
+
+
+class MdfStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("slip_first", C.c_double),
+                ("slip_last", C.c_double)]
+
+
 class Domain(C.Structure):
     _fields_ = [("dimension", C.c_int), ("boxlo", C.c_double * 3), ("boxhi", C.c_double * 3),
                 ("periodicity", C.c_int * 3), ("dt", C.c_double), ("ntimestep", C.c_long)]
@@ -95,6 +108,10 @@ def lib() -> C.CDLL:
         "couplb_synchronize": (ip, [vp]),
         "couplb_stream": (vp, [vp]),
         "couplb_last_kernel_ms": (C.c_double, [vp, ip]),
+        "couplb_kernel_ms_total": (ip, [vp, ip, dp, C.POINTER(C.c_long)]),
+        "couplb_kernel_ms_reset": (ip, [vp]),
+        "couplb_ibm_mdf_device": (ip, [vp, ip, vp, vp, vp, vp, C.POINTER(MdfArgs),
+                                       C.POINTER(MdfStats)]),
         "couplb_fix_create": (ip, [ip, C.POINTER(C.c_char_p), C.POINTER(Domain), C.POINTER(vp)]),
         "couplb_fix_destroy": (ip, [vp]),
         "couplb_fix_last_error": (C.c_char_p, [vp]),
@@ -222,6 +239,25 @@ class Device:
         self._chk(lib().couplb_ibm_substep_device(self.h, n, x_ptr, v_ptr, m_ptr, f_ptr, C.byref(a)),
                   "ibm_substep_device")
 
+    def mdf_device(self, n, x_ptr, V_ptr, F0_ptr, fbody_ptr, lo, inv_dx, vel_scale, force_scale,
+                   tol=1e-6, relax=1.0, max_iter=200, positions_unchanged=False, method="cg"):
+        """Multi-direct-forcing solve for markers with prescribed velocity V
+        (device pointers; F0_ptr may be None); method "cg" or "richardson".
+        Returns the solver statistics."""
+        a = MdfArgs()
+        for d in range(3):
+            a.ibm.lo[d] = lo[d]
+        a.ibm.inv_dx, a.ibm.vel_scale, a.ibm.force_scale = inv_dx, vel_scale, force_scale
+        a.ibm.xi_over_dtLB, a.ibm.inv_N = 0.0, 1.0
+        a.ibm.positions_unchanged = int(positions_unchanged)
+        a.tol, a.relax, a.max_iter = tol, relax, max_iter
+        a.method = MDF_CG if method == "cg" else MDF_RICHARDSON
+        st = MdfStats()
+        self._chk(lib().couplb_ibm_mdf_device(self.h, n, x_ptr, V_ptr, F0_ptr, fbody_ptr,
+                                              C.byref(a), C.byref(st)), "ibm_mdf_device")
+        return dict(iterations=st.iterations, converged=bool(st.converged),
+                    slip_first=st.slip_first, slip_last=st.slip_last)
+
     def refresh(self):
         self._chk(lib().couplb_refresh_macro(self.h), "refresh_macro")
 
@@ -247,6 +283,17 @@ class Device:
 
     def last_ms(self, which=0):
         return lib().couplb_last_kernel_ms(self.h, which)
+
+    def kernel_ms_total(self, which=0):
+        """(total milliseconds, count) of the timed regions of one kind since
+        the last reset: 0 LB kernel, 1 IBM sub-step, 2 spreading, 3 MDF."""
+        t, n = C.c_double(), C.c_long()
+        self._chk(lib().couplb_kernel_ms_total(self.h, which, C.byref(t), C.byref(n)),
+                  "kernel_ms_total")
+        return t.value, n.value
+
+    def kernel_ms_reset(self):
+        self._chk(lib().couplb_kernel_ms_reset(self.h), "kernel_ms_reset")
 
 
 class Fix:

@@ -14,11 +14,26 @@
 #include "gather.hpp"
 #include "ibm.hpp"
 #include "kernels.hpp"
+#include "mdf.hpp"
 #include "spread_pairs.hpp"
 
 using namespace couplb;
 
 namespace {
+
+// Kernel timing without host synchronisation: each timed region records a
+// pair of events into a ring; a pair is read back (hipEventSynchronize) only
+// when its ring entry is reused or when the caller asks for the times.
+struct TimerRing {
+  static constexpr int R = 256;
+  static constexpr int W = 4;  // 0 LB kernel, 1 IBM sub-step, 2 spreading, 3 MDF solve
+  hipEvent_t a[R] = {}, b[R] = {};
+  int which[R] = {};
+  bool pending[R] = {};
+  int head = 0;
+  double total[W] = {0, 0, 0, 0}, last[W] = {0, 0, 0, 0};
+  long count[W] = {0, 0, 0, 0};
+};
 
 struct SlotHost {
   int* rowstart = nullptr;
@@ -45,15 +60,14 @@ struct couplb_ctx {
   float scale = 1.0f;
   int cycle = 0;         // state counter for the u stamps
   hipStream_t stream = nullptr;
-  hipEvent_t ev[5] = {};
-  double last_ms[3] = {0, 0, 0};
+  TimerRing tr;
   std::string err;
   // macroscopic staging
   double* mac = nullptr;      // 4*N
   double* partial = nullptr;  // 9*ny*nz
   double* red = nullptr;      // 6 + 3*ny
   // IBM
-  double* u3 = nullptr;   // IBM fluid velocity, node-interleaved
+  double* u4 = nullptr;   // IBM fluid velocity and density (ux, uy, uz, rho) per node
   int* stamp = nullptr;
   double *Fx = nullptr, *Fy = nullptr, *Fz = nullptr;  // spread force
   unsigned char *flag = nullptr, *cover = nullptr;
@@ -80,6 +94,14 @@ struct couplb_ctx {
   long pcap = 0, pne = 0;
   int pair_epoch = -1;       // ordering the pairs (and the node flags) were built for
   int prev_step_epoch = -1;  // ordering of the single record slot at the previous LB step
+  // multi-direct forcing (mdf.hpp)
+  double* ut = nullptr;            // operator node values at pair nodes, 3 per node
+  double* mdf_v = nullptr;         // target velocity per sorted marker (3*cap)
+  double *mdf_F = nullptr, *mdf_R = nullptr, *mdf_Ap = nullptr;  // force, residual, A p
+  double* mdf_part = nullptr;      // block partials
+  int mdf_cap = 0;
+  MdfScalars* mdf_S = nullptr;
+  MdfScalars* mdf_host = nullptr;  // pinned
   int ugather_cycle = -1; // state counter of the last velocity gather
   std::vector<SlotHost> slots;
   int nslots = 0;
@@ -102,6 +124,44 @@ int fail(couplb_ctx* c, int code, const std::string& msg) {
     if (e_ != hipSuccess)                                                            \
       return fail(ctx, COUPLB_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
+
+int timer_harvest(couplb_ctx* c, int e) {
+  TimerRing& t = c->tr;
+  if (!t.pending[e]) return COUPLB_OK;
+  HIPCHK(c, hipEventSynchronize(t.b[e]));
+  float ms = 0;
+  HIPCHK(c, hipEventElapsedTime(&ms, t.a[e], t.b[e]));
+  t.total[t.which[e]] += ms;
+  t.count[t.which[e]] += 1;
+  t.last[t.which[e]] = ms;
+  t.pending[e] = false;
+  return COUPLB_OK;
+}
+int timer_harvest_all(couplb_ctx* c) {
+  // oldest first, so `last` ends with the newest region of each kind
+  for (int k = 0; k < TimerRing::R; k++) {
+    const int rc = timer_harvest(c, (c->tr.head + k) % TimerRing::R);
+    if (rc) return rc;
+  }
+  return COUPLB_OK;
+}
+// opens a timed region of kind `which`; returns its ring entry in *slot
+int timer_begin(couplb_ctx* c, int which, int* slot) {
+  TimerRing& t = c->tr;
+  const int e = t.head;
+  int rc = timer_harvest(c, e);
+  if (rc) return rc;
+  t.head = (t.head + 1) % TimerRing::R;
+  t.which[e] = which;
+  HIPCHK(c, hipEventRecord(t.a[e], c->stream));
+  *slot = e;
+  return COUPLB_OK;
+}
+int timer_end(couplb_ctx* c, int slot) {
+  HIPCHK(c, hipEventRecord(c->tr.b[slot], c->stream));
+  c->tr.pending[slot] = true;
+  return COUPLB_OK;
+}
 
 size_t round_up(size_t a, size_t b) { return (a + b - 1) / b * b; }
 
@@ -206,7 +266,7 @@ template <int D, typename S> struct LaunchUgather {
     }
 #undef COVER_LAUNCH
     const dim3 b = row_block(c->geo.nx);
-    hipLaunchKernelGGL((k_upull<D, S>), row_grid(c->geo, b.x), b, 0, c->stream, pc, c->cover, c->u3,
+    hipLaunchKernelGGL((k_upull<D, S>), row_grid(c->geo, b.x), b, 0, c->stream, pc, c->cover, c->u4,
                        c->stamp, c->cycle);
   }
 };
@@ -247,8 +307,8 @@ unsigned key_bits(size_t nrows) {
 int ensure_ibm(couplb_ctx* c, int n) {
   if (c->geo.nx > MAX_NX_IBM)
     return fail(c, COUPLB_EINVAL, "IBM coupling supports rows of at most 1024 nodes in x");
-  if (!c->u3) {
-    HIPCHK(c, hipMalloc(&c->u3, sizeof(double) * 3 * c->N));
+  if (!c->u4) {
+    HIPCHK(c, hipMalloc(&c->u4, sizeof(double) * 4 * c->N));
     HIPCHK(c, hipMalloc(&c->stamp, sizeof(int) * c->N));
     HIPCHK(c, hipMemsetAsync(c->stamp, 0xFF, sizeof(int) * c->N, c->stream));
     HIPCHK(c, hipMalloc(&c->rowstart_cur, sizeof(int) * (c->geo.nrows + 1)));
@@ -274,6 +334,9 @@ int ensure_ibm(couplb_ctx* c, int n) {
   HIPCHK(c, hipMalloc(&c->dm, sizeof(double) * cap));
   HIPCHK(c, hipMalloc(&c->df, sizeof(double) * 3 * cap));
   HIPCHK(c, hipMalloc(&c->ufc, sizeof(double) * 3 * cap));
+  (void)hipFree(c->mdf_v);
+  c->mdf_v = nullptr;
+  HIPCHK(c, hipMalloc(&c->mdf_v, sizeof(double) * 3 * cap));
   size_t bytes = 0;
   HIPCHK(c, rocprim::radix_sort_pairs(nullptr, bytes, c->keys, c->skeys, c->vals, c->perm,
                                       (size_t)cap, 0u, key_bits(c->geo.nrows), c->stream));
@@ -303,12 +366,7 @@ int ensure_slot(couplb_ctx* c, int s) {
   return COUPLB_OK;
 }
 
-int ibm_impl(couplb_ctx* c, int n, const double* x, const double* v, const double* m,
-             double* f_dev, int fmode, const couplb_ibm_args* a) {
-  if (n < 0 || !a) return fail(c, COUPLB_EINVAL, "bad IBM arguments");
-  if (n == 0) return COUPLB_OK;
-  int rc = ensure_ibm(c, n);
-  if (rc) return rc;
+IbmParams make_ibm_params(const couplb_ctx* c, const couplb_ibm_args* a) {
   IbmParams ip;
   for (int d = 0; d < 3; d++) ip.lo[d] = a->lo[d];
   ip.inv_dx = a->inv_dx;
@@ -317,10 +375,17 @@ int ibm_impl(couplb_ctx* c, int n, const double* x, const double* v, const doubl
   ip.inv_N = a->inv_N;
   ip.force_scale = a->force_scale;
   ip.kern = c->p.kernel;
+  return ip;
+}
+
+// Marker ordering by base row (kept while the caller declares the positions
+// unchanged) and the fluid velocity at covered nodes of the current state
+// (once per state).  Sets *reorder when the ordering was rebuilt.
+int prepare_markers(couplb_ctx* c, int n, const double* x, const IbmParams& ip,
+                    int positions_unchanged, bool* reorder_out) {
   const int tb = 256, nb = (n + tb - 1) / tb;
-  HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
-  const double key[5] = {a->lo[0], a->lo[1], a->lo[2], a->inv_dx, (double)c->p.kernel};
-  const bool reorder = !(a->positions_unchanged && c->order_n == n &&
+  const double key[5] = {ip.lo[0], ip.lo[1], ip.lo[2], ip.inv_dx, (double)c->p.kernel};
+  const bool reorder = !(positions_unchanged && c->order_n == n &&
                          std::memcmp(key, c->order_key, sizeof(key)) == 0);
   if (reorder) {
     hipLaunchKernelGGL(k_keys, dim3(nb), dim3(tb), 0, c->stream, n, x, ip, c->geo, c->keys, c->vals);
@@ -345,6 +410,24 @@ int ibm_impl(couplb_ctx* c, int n, const double* x, const double* v, const doubl
     c->cover_epoch = c->pos_epoch;
     c->ugather_cycle = c->cycle;
   }
+  *reorder_out = reorder;
+  return COUPLB_OK;
+}
+
+int ibm_impl(couplb_ctx* c, int n, const double* x, const double* v, const double* m,
+             double* f_dev, int fmode, const couplb_ibm_args* a) {
+  if (n < 0 || !a) return fail(c, COUPLB_EINVAL, "bad IBM arguments");
+  if (n == 0) return COUPLB_OK;
+  int rc = ensure_ibm(c, n);
+  if (rc) return rc;
+  const IbmParams ip = make_ibm_params(c, a);
+  const int tb = 256, nb = (n + tb - 1) / tb;
+  int tslot = 0;
+  rc = timer_begin(c, 1, &tslot);
+  if (rc) return rc;
+  bool reorder = false;
+  rc = prepare_markers(c, n, x, ip, a->positions_unchanged, &reorder);
+  if (rc) return rc;
   // Same positions and same LB state as the previous sub-step of this
   // interval: u_f is known and only the force is new (k_marker_force).
   const bool merge = !reorder && c->nslots > 0 && c->slots[c->nslots - 1].epoch == c->pos_epoch &&
@@ -355,9 +438,8 @@ int ibm_impl(couplb_ctx* c, int n, const double* x, const double* v, const doubl
     hipLaunchKernelGGL(k_marker_force, dim3(nb), dim3(tb), 0, c->stream, n, c->perm, v, m, f_dev, fmode,
                        ip, (const double*)c->ufc, sl.rec);
     HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
     c->rho_fresh = 1;  // compute_macroscopic(false) at fix_couplb.cpp:758
-    return COUPLB_OK;
+    return timer_end(c, tslot);
   }
   const int s = c->nslots;
   rc = ensure_slot(c, s);
@@ -367,15 +449,16 @@ int ibm_impl(couplb_ctx* c, int n, const double* x, const double* v, const doubl
   if (c->D == 3) {
     const int nb4 = (4 * n + tb - 1) / tb;  // four lanes per marker
     hipLaunchKernelGGL((k_marker<3>), dim3(nb4), dim3(tb), 0, c->stream, n, c->perm, x, v, m, f_dev,
-                       fmode, ip, c->geo, c->walls, c->u3, sl.rec, c->ufc);
+                       fmode, ip, c->geo, c->walls, c->u4, sl.rec, c->ufc);
   } else {
     hipLaunchKernelGGL((k_marker<2>), dim3(nb), dim3(tb), 0, c->stream, n, c->perm, x, v, m, f_dev,
-                       fmode, ip, c->geo, c->walls, c->u3, sl.rec, c->ufc);
+                       fmode, ip, c->geo, c->walls, c->u4, sl.rec, c->ufc);
   }
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipMemcpyAsync(sl.rowstart, c->rowstart_cur, sizeof(int) * (c->geo.nrows + 1),
                            hipMemcpyDeviceToDevice, c->stream));
-  HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+  rc = timer_end(c, tslot);
+  if (rc) return rc;
   sl.n = n;
   sl.epoch = c->pos_epoch;
   c->uf_cycle = c->cycle;
@@ -437,6 +520,159 @@ int build_pairs(couplb_ctx* c, const SlotHost& sl) {
   return COUPLB_OK;
 }
 
+int mdf_impl(couplb_ctx* c, int n, const double* x, const double* V, const double* F0,
+             double* f_body, const couplb_mdf_args* a, couplb_mdf_stats* st) {
+  if (n < 0 || !a || !x || !V || !f_body) return fail(c, COUPLB_EINVAL, "bad MDF arguments");
+  if (!(a->tol > 0) || a->max_iter < 1 || !(a->relax > 0) ||
+      (a->method != COUPLB_MDF_RICHARDSON && a->method != COUPLB_MDF_CG))
+    return fail(c, COUPLB_EINVAL, "MDF needs tol > 0, max_iter >= 1, relax > 0 and a known method");
+  if (c->nslots != 0)
+    return fail(c, COUPLB_ESTATE, "MDF must be the only IBM call between two LB steps");
+  if (st) *st = couplb_mdf_stats{0, 0, 0.0, 0.0};
+  if (n == 0) return COUPLB_OK;
+  const int S = (c->D == 3) ? stencil_size<3>() : stencil_size<2>();
+  if ((long)n * S >= (long)INT32_MAX) return fail(c, COUPLB_EINVAL, "too many markers for MDF");
+  int rc = ensure_ibm(c, n);
+  if (rc) return rc;
+  rc = ensure_slot(c, 0);
+  if (rc) return rc;
+  const int tb = 256, nb = (n + tb - 1) / tb;
+  const int nbi = (c->D == 3) ? (4 * n + tb - 1) / tb : nb;
+  if (!c->ut) HIPCHK(c, hipMalloc(&c->ut, sizeof(double) * 3 * c->N));
+  if (!c->mdf_S) HIPCHK(c, hipMalloc(&c->mdf_S, sizeof(MdfScalars)));
+  if (!c->mdf_host) HIPCHK(c, hipHostMalloc((void**)&c->mdf_host, sizeof(MdfScalars)));
+  if (c->mdf_cap < n) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    (void)hipFree(c->mdf_F); (void)hipFree(c->mdf_R); (void)hipFree(c->mdf_Ap); (void)hipFree(c->mdf_part);
+    c->mdf_F = c->mdf_R = c->mdf_Ap = c->mdf_part = nullptr;
+    c->mdf_cap = 0;
+    const int cap = std::max(n, c->cap);
+    const int nbmax = (4 * cap + tb - 1) / tb;
+    HIPCHK(c, hipMalloc(&c->mdf_F, sizeof(double) * 3 * cap));
+    HIPCHK(c, hipMalloc(&c->mdf_R, sizeof(double) * 3 * cap));
+    HIPCHK(c, hipMalloc(&c->mdf_Ap, sizeof(double) * 3 * cap));
+    HIPCHK(c, hipMalloc(&c->mdf_part, sizeof(double) * 2 * nbmax));
+    c->mdf_cap = cap;
+  }
+  double* part_a = c->mdf_part;
+  double* part_b = c->mdf_part + (4 * c->mdf_cap + tb - 1) / tb;
+  const IbmParams ip = make_ibm_params(c, &a->ibm);
+  int tslot = 0;
+  rc = timer_begin(c, 3, &tslot);
+  if (rc) return rc;
+  bool reorder = false;
+  rc = prepare_markers(c, n, x, ip, a->ibm.positions_unchanged, &reorder);
+  if (rc) return rc;
+  SlotHost& sl = c->slots[0];
+  MdfScalars* Sd = c->mdf_S;
+  hipLaunchKernelGGL(k_mdf_reset, dim3(1), dim3(1), 0, c->stream, Sd, 1);
+  if (c->D == 3)
+    hipLaunchKernelGGL(k_mdf_init<3>, dim3(nb), dim3(tb), 0, c->stream, n, c->perm, x, V, F0, ip, c->geo,
+                       c->walls, sl.rec, c->mdf_v, c->mdf_F);
+  else
+    hipLaunchKernelGGL(k_mdf_init<2>, dim3(nb), dim3(tb), 0, c->stream, n, c->perm, x, V, F0, ip, c->geo,
+                       c->walls, sl.rec, c->mdf_v, c->mdf_F);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(sl.rowstart, c->rowstart_cur, sizeof(int) * (c->geo.nrows + 1),
+                           hipMemcpyDeviceToDevice, c->stream));
+  sl.n = n;
+  sl.epoch = c->pos_epoch;
+  if (c->pair_epoch != sl.epoch) {
+    rc = build_pairs(c, sl);
+    if (rc) return rc;
+  }
+  const unsigned nbp = (unsigned)std::max<long>(1, (c->pne + 255) / 256);
+  // node values ut = [u*] + S(rec.f)/(2 rho), then the interpolation pass
+  auto apply = [&](int affine) {
+    hipLaunchKernelGGL(k_mdf_ut, dim3(nbp), dim3(256), 0, c->stream, c->pne, (const unsigned*)c->pkey_s,
+                       (const int*)c->pp, (const double*)c->pw, (unsigned)c->N, (const MarkerRec*)sl.rec,
+                       (const double*)c->u4, affine, c->ut, (const MdfScalars*)Sd);
+    double* out = affine ? c->mdf_R : c->mdf_Ap;
+    if (c->D == 3)
+      hipLaunchKernelGGL(k_mdf_interp<3>, dim3(nbi), dim3(MDF_THREADS), 0, c->stream, n, c->geo, c->walls,
+                         (const double*)c->ut, affine, (const MarkerRec*)sl.rec, (const double*)c->mdf_v,
+                         out, part_a, part_b, (const MdfScalars*)Sd);
+    else
+      hipLaunchKernelGGL(k_mdf_interp<2>, dim3(nbi), dim3(MDF_THREADS), 0, c->stream, n, c->geo, c->walls,
+                         (const double*)c->ut, affine, (const MarkerRec*)sl.rec, (const double*)c->mdf_v,
+                         out, part_a, part_b, (const MdfScalars*)Sd);
+  };
+  auto scalar = [&](int phase, int nblk, int first, int respect) {
+    hipLaunchKernelGGL(k_mdf_scalar, dim3(1), dim3(MDF_THREADS), 0, c->stream, (const double*)part_a,
+                       (const double*)part_b, nblk, phase, a->tol, first, a->max_iter, respect, Sd);
+  };
+  auto read = [&]() -> int {
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(c->mdf_host, Sd, sizeof(MdfScalars), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return COUPLB_OK;
+  };
+  // true residual of F (rec.f = F): sets done = 1 when converged
+  auto true_residual = [&](int first) -> int {
+    hipLaunchKernelGGL(k_mdf_setf, dim3(nb), dim3(tb), 0, c->stream, n, (const double*)c->mdf_F, sl.rec);
+    hipLaunchKernelGGL(k_mdf_reset, dim3(1), dim3(1), 0, c->stream, Sd, 0);
+    apply(1);
+    scalar(MDF_RESID, nbi, first, 0);
+    return read();
+  };
+  constexpr int CHUNK = 4;  // iterations queued between two host checks
+  rc = true_residual(1);
+  if (rc) return rc;
+  int method = a->method;
+  while (c->mdf_host->done == 0) {
+    if (method == COUPLB_MDF_CG) {
+      hipLaunchKernelGGL(k_cg_direction, dim3(nb), dim3(tb), 0, c->stream, n, (const double*)c->mdf_R,
+                         sl.rec, (const MdfScalars*)Sd, 1);
+      do {
+        for (int k = 0; k < CHUNK; k++) {
+          apply(0);
+          scalar(MDF_ALPHA, nbi, 0, 1);
+          hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(MDF_THREADS), 0, c->stream, n,
+                             (const MarkerRec*)sl.rec, (const double*)c->mdf_Ap, c->mdf_F, c->mdf_R, part_a,
+                             part_b, (const MdfScalars*)Sd);
+          scalar(MDF_BETA, nb, 0, 1);
+          hipLaunchKernelGGL(k_cg_direction, dim3(nb), dim3(tb), 0, c->stream, n, (const double*)c->mdf_R,
+                             sl.rec, (const MdfScalars*)Sd, 0);
+        }
+        rc = read();
+        if (rc) return rc;
+      } while (c->mdf_host->done == 0);
+      const int why = c->mdf_host->done;
+      if (c->mdf_host->iters >= a->max_iter) {
+        rc = true_residual(0);
+        if (rc) return rc;
+        break;
+      }
+      rc = true_residual(0);  // recursive residual converged or CG broke down
+      if (rc) return rc;
+      if (c->mdf_host->done == 0 && why == 2) method = COUPLB_MDF_RICHARDSON;
+    } else {
+      do {
+        for (int k = 0; k < CHUNK; k++) {
+          hipLaunchKernelGGL(k_mdf_relax, dim3(nb), dim3(tb), 0, c->stream, n, (const double*)c->mdf_R,
+                             a->relax, c->mdf_F, sl.rec, (const MdfScalars*)Sd);
+          apply(1);
+          scalar(MDF_RESID, nbi, 0, 1);
+        }
+        rc = read();
+        if (rc) return rc;
+      } while (c->mdf_host->done == 0);
+    }
+  }
+  const MdfScalars& hs = *c->mdf_host;
+  couplb_mdf_stats out{hs.iters, hs.done == 1 ? 1 : 0, hs.smax_first, hs.smax};
+  hipLaunchKernelGGL(k_mdf_setf, dim3(nb), dim3(tb), 0, c->stream, n, (const double*)c->mdf_F, sl.rec);
+  hipLaunchKernelGGL(k_mdf_out, dim3(nb), dim3(tb), 0, c->stream, n, c->perm, (const double*)c->mdf_F,
+                     (const MarkerRec*)sl.rec, a->ibm.force_scale, f_body);
+  HIPCHK(c, hipGetLastError());
+  rc = timer_end(c, tslot);
+  if (rc) return rc;
+  c->nslots = 1;
+  c->rho_fresh = 1;  // the velocity gather is compute_macroscopic(false) of this state
+  if (st) *st = out;
+  return COUPLB_OK;
+}
+
 int step_impl(couplb_ctx* c, const double g[3], int flags, bool timed) {
   StepArgs a;
   a.pc = make_pull(c);
@@ -458,8 +694,12 @@ int step_impl(couplb_ctx* c, const double g[3], int flags, bool timed) {
   // markers at the positions of the previous LB interval: constant map
   const bool fixed = c->nslots == 1 && c->slots[0].epoch == c->prev_step_epoch &&
                      (long)c->slots[0].n * S < (long)INT32_MAX;
+  int tspread = -1;
   if (fixed) {
-    if (timed) HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
+    if (timed) {
+      const int rc = timer_begin(c, 2, &tspread);
+      if (rc) return rc;
+    }
     if (c->pair_epoch != c->slots[0].epoch) {
       const int rc = build_pairs(c, c->slots[0]);
       if (rc) return rc;
@@ -488,7 +728,10 @@ int step_impl(couplb_ctx* c, const double g[3], int flags, bool timed) {
     sa.slots = c->slot_table;
     sa.Fx = c->Fx; sa.Fy = c->Fy; sa.Fz = c->Fz;
     sa.flag = c->flag;
-    if (timed) HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
+    if (timed) {
+      const int rc = timer_begin(c, 2, &tspread);
+      if (rc) return rc;
+    }
     const TileShape ts = tile_shape(c->D, c->geo.nx);
 #define SPREAD_LAUNCH(DD, TJ, TK, XM) \
   hipLaunchKernelGGL((k_spread<DD, TJ, TK, XM>), tile_grid(c->geo), dim3(TILE_THREADS), 0, c->stream, sa)
@@ -504,10 +747,21 @@ int step_impl(couplb_ctx* c, const double g[3], int flags, bool timed) {
 #undef SPREAD_LAUNCH
     HIPCHK(c, hipGetLastError());
   }
-  if (timed) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+  if (tspread >= 0) {
+    const int rc = timer_end(c, tspread);
+    if (rc) return rc;
+  }
+  int tstep = -1;
+  if (timed) {
+    const int rc = timer_begin(c, 0, &tstep);
+    if (rc) return rc;
+  }
   dispatch<LaunchStep>(c, a);
   HIPCHK(c, hipGetLastError());
-  if (timed) HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+  if (tstep >= 0) {
+    const int rc = timer_end(c, tstep);
+    if (rc) return rc;
+  }
   if (c->nslots > 0) c->prev_step_epoch = (c->nslots == 1) ? c->slots[0].epoch : -1;
   c->cur = dst;
   c->local = 0;
@@ -571,8 +825,9 @@ int couplb_create(const couplb_params* p, couplb_ctx** out) {
   if (hipSetDevice(dev) != hipSuccess) return bad("hipSetDevice failed");
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
     return bad("hipStreamCreate failed");
-  for (auto& e : c->ev)
-    if (hipEventCreate(&e) != hipSuccess) return bad("hipEventCreate failed");
+  for (int e = 0; e < TimerRing::R; e++)
+    if (hipEventCreate(&c->tr.a[e]) != hipSuccess || hipEventCreate(&c->tr.b[e]) != hipSuccess)
+      return bad("hipEventCreate failed");
   const size_t fbytes = c->elem * c->pitch * c->Q;
   for (int b = 0; b < 2; b++) {
     if (hipMalloc(&c->f[b], fbytes) != hipSuccess) {
@@ -618,7 +873,7 @@ int couplb_destroy(couplb_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   (void)hipFree(c->f[0]); (void)hipFree(c->f[1]); (void)hipFree(c->rho);
   (void)hipFree(c->mac); (void)hipFree(c->partial); (void)hipFree(c->red);
-  (void)hipFree(c->u3); (void)hipFree(c->stamp);
+  (void)hipFree(c->u4); (void)hipFree(c->stamp);
   (void)hipFree(c->Fx); (void)hipFree(c->Fy); (void)hipFree(c->Fz);
   (void)hipFree(c->flag); (void)hipFree(c->cover);
   (void)hipFree(c->keys); (void)hipFree(c->skeys); (void)hipFree(c->vals); (void)hipFree(c->perm);
@@ -628,9 +883,15 @@ int couplb_destroy(couplb_ctx* c) {
   (void)hipFree(c->staging);
   (void)hipFree(c->pkey); (void)hipFree(c->pkey_s); (void)hipFree(c->pval); (void)hipFree(c->pval_s);
   (void)hipFree(c->pp); (void)hipFree(c->pw); (void)hipFree(c->ptmp);
+  (void)hipFree(c->ut); (void)hipFree(c->mdf_v); (void)hipFree(c->mdf_S);
+  (void)hipFree(c->mdf_F); (void)hipFree(c->mdf_R); (void)hipFree(c->mdf_Ap); (void)hipFree(c->mdf_part);
+  if (c->mdf_host) (void)hipHostFree(c->mdf_host);
   for (auto& s : c->slots) { (void)hipFree(s.rowstart); (void)hipFree(s.rec); }
   (void)hipFree(c->slot_table);
-  for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
+  for (int e = 0; e < TimerRing::R; e++) {
+    if (c->tr.a[e]) (void)hipEventDestroy(c->tr.a[e]);
+    if (c->tr.b[e]) (void)hipEventDestroy(c->tr.b[e]);
+  }
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return COUPLB_OK;
@@ -708,19 +969,7 @@ int couplb_get_state_f(couplb_ctx* c, double* fh) {
 
 int couplb_lbm_step(couplb_ctx* c, const double g[3], int flags) {
   if (!c) return COUPLB_EINVAL;
-  const bool had_ibm = c->nslots > 0;
-  int rc = step_impl(c, g, flags, true);
-  if (rc) return rc;
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  float ms = 0;
-  HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
-  c->last_ms[0] = ms;
-  c->last_ms[2] = 0;
-  if (had_ibm) {
-    HIPCHK(c, hipEventElapsedTime(&ms, c->ev[4], c->ev[0]));
-    c->last_ms[2] = ms;
-  }
-  return COUPLB_OK;
+  return step_impl(c, g, flags, true);
 }
 
 int couplb_lbm_steps_async(couplb_ctx* c, const double g[3], int flags, int nsteps) {
@@ -735,13 +984,13 @@ int couplb_lbm_steps_async(couplb_ctx* c, const double g[3], int flags, int nste
 int couplb_ibm_substep_device(couplb_ctx* c, int n, const double* x, const double* v,
                               const double* m, double* f, const couplb_ibm_args* a) {
   if (!c) return COUPLB_EINVAL;
-  int rc = ibm_impl(c, n, x, v, m, f, 0, a);
-  if (rc) return rc;
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  float ms = 0;
-  HIPCHK(c, hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
-  c->last_ms[1] = ms;
-  return COUPLB_OK;
+  return ibm_impl(c, n, x, v, m, f, 0, a);
+}
+
+int couplb_ibm_mdf_device(couplb_ctx* c, int n, const double* x, const double* V, const double* F0,
+                          double* f_body, const couplb_mdf_args* a, couplb_mdf_stats* st) {
+  if (!c) return COUPLB_EINVAL;
+  return mdf_impl(c, n, x, V, F0, f_body, a, st);
 }
 
 int couplb_ibm_substep(couplb_ctx* c, int n, const double* x, const double* v, const double* m,
@@ -759,9 +1008,6 @@ int couplb_ibm_substep(couplb_ctx* c, int n, const double* x, const double* v, c
   HIPCHK(c, hipMemcpyAsync(F.data(), c->df, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   for (size_t i = 0; i < (size_t)3 * n; i++) f[i] += F[i];  // f[i] += F (fix_couplb.cpp:783-785)
-  float ms = 0;
-  HIPCHK(c, hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
-  c->last_ms[1] = ms;
   return COUPLB_OK;
 }
 
@@ -821,8 +1067,25 @@ int couplb_synchronize(couplb_ctx* c) {
 
 void* couplb_stream(couplb_ctx* c) { return c ? (void*)c->stream : nullptr; }
 
-double couplb_last_kernel_ms(const couplb_ctx* c, int which) {
-  return (c && which >= 0 && which < 3) ? c->last_ms[which] : -1.0;
+double couplb_last_kernel_ms(couplb_ctx* c, int which) {
+  if (!c || which < 0 || which >= TimerRing::W) return -1.0;
+  if (timer_harvest_all(c)) return -1.0;
+  return c->tr.last[which];
+}
+int couplb_kernel_ms_total(couplb_ctx* c, int which, double* total_ms, long* count) {
+  if (!c || which < 0 || which >= TimerRing::W || !total_ms || !count) return COUPLB_EINVAL;
+  const int rc = timer_harvest_all(c);
+  if (rc) return rc;
+  *total_ms = c->tr.total[which];
+  *count = c->tr.count[which];
+  return COUPLB_OK;
+}
+int couplb_kernel_ms_reset(couplb_ctx* c) {
+  if (!c) return COUPLB_EINVAL;
+  const int rc = timer_harvest_all(c);
+  if (rc) return rc;
+  for (int w = 0; w < TimerRing::W; w++) c->tr.total[w] = 0, c->tr.count[w] = 0;
+  return COUPLB_OK;
 }
 
 }  // extern "C"

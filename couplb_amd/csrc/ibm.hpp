@@ -183,7 +183,7 @@ __global__ void __launch_bounds__(TILE_THREADS) k_cover(Geo g, Walls w, BaseSlot
 // stored node-interleaved (ux, uy, uz): a stencil row is one contiguous
 // segment for the marker gathers.
 template <int D, typename S>
-__global__ void __launch_bounds__(256) k_upull(PullCtx pc, const unsigned char* cover, double* u3,
+__global__ void __launch_bounds__(256) k_upull(PullCtx pc, const unsigned char* cover, double* u4,
                                                int* stamp, int cycle) {
   using L = Lat<D>;
   using T = typename Store<S>::T;
@@ -209,14 +209,84 @@ __global__ void __launch_bounds__(256) k_upull(PullCtx pc, const unsigned char* 
   if (r <= RHO_MIN) r = RHO_MIN;
   if (!isfinite(r)) r = RHO_MIN;
   const double inv_r = 1.0 / r;
-  u3[3 * (size_t)node + 0] = jx * inv_r;
-  u3[3 * (size_t)node + 1] = jy * inv_r;
-  u3[3 * (size_t)node + 2] = jz * inv_r;
+  double4 out;
+  out.x = jx * inv_r;
+  out.y = jy * inv_r;
+  out.z = jz * inv_r;
+  out.w = r;
+  *reinterpret_cast<double4*>(u4 + 4 * (size_t)node) = out;
   stamp[node] = cycle;
 }
 
-// Markers in base-row order, MARKER_LANES lanes per marker (one per z-plane
-// of the 4x4x4 stencil in 3D).  IBM::interpolate (couplb_ibm.h:57-90) +
+// Interpolation sums of one marker over its stencil plane c = k0-1+vv (3D;
+// the whole 4x4 stencil in 2D), IBM::interpolate's rules per node type
+// (couplb_ibm.h:66-86): fluid nodes add vel(node)*d, bounce-back and
+// free-slip nodes add d to the normalisation only, moving-wall nodes add
+// the wall velocity; nodes beyond the ghost layer are skipped.  s = {sum u,
+// sum v, sum w, sum d over counted nodes, sum d over fluid nodes}.
+// wall_vel = false drops the moving-wall velocity term (the weight still
+// counts), which makes the map from node values to the marker value linear.
+template <int D, typename Vel>
+__device__ inline void interp_plane(const Geo& g, const Walls& w, bool fin, int i0, int j0, int k0,
+                                    int vv, const double* wx, const double* wy, double wzv, Vel vel,
+                                    double s[5], bool wall_vel = true) {
+#pragma clang fp contract(off)
+  s[0] = s[1] = s[2] = s[3] = s[4] = 0.0;
+  const int c = (D == 3) ? k0 - 1 + vv : 0;
+  if (!fin || (D == 3 && (c < 0 || c > g.nz + 1))) return;
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    const int b = j0 - 1 + u;
+    if (b < 0 || b > g.ny + 1) continue;
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+      const int a = i0 - 1 + t;
+      if (a < 0 || a > g.nx + 1) continue;
+      int wd, wsd;
+      const int nt = node_type(w, g, a, b, c, wd, wsd);
+      const double d = (D == 3) ? (wx[t] * wy[u]) * wzv : wx[t] * wy[u];
+      if (nt == 0) {
+        const unsigned node = (unsigned)wrap_ghost(a, g.nx) +
+                              (unsigned)g.nx * ((unsigned)wrap_ghost(b, g.ny) +
+                                                (unsigned)g.ny * (unsigned)((D == 3) ? wrap_ghost(c, g.nz) : 0));
+        double un[3];
+        vel(node, un);
+        s[0] += un[0] * d;
+        s[1] += un[1] * d;
+        s[2] += un[2] * d;
+        s[3] += d;
+        s[4] += d;
+      } else if (nt == 1 || nt == 3) {
+        s[3] += d;
+      } else if (nt == 2) {
+        if (wall_vel) {
+          s[0] += w.vel[wd][wsd][0] * d;
+          s[1] += w.vel[wd][wsd][1] * d;
+          s[2] += ((D == 3) ? w.vel[wd][wsd][2] : 0.0) * d;
+        }
+        s[3] += d;
+      }
+    }
+  }
+}
+
+// combine the LPM lanes' plane sums in plane order 0, 1, 2, 3 (every lane
+// of the group ends with the same totals)
+template <int LPM>
+__device__ inline void combine_planes(double s[5]) {
+  if (LPM == 1) return;
+  const int lane0 = (threadIdx.x & 63) & ~(LPM - 1);
+  double t[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+  for (int l = 0; l < LPM; l++)
+#pragma unroll
+    for (int r = 0; r < 5; r++) t[r] += __shfl(s[r], lane0 + l);
+#pragma unroll
+  for (int r = 0; r < 5; r++) s[r] = t[r];
+}
+
+// Markers in base-row order, four lanes per marker in 3D (one per z-plane
+// of the 4x4x4 stencil).  IBM::interpolate (couplb_ibm.h:57-90) +
 // penalty force (fix_couplb.cpp:779-785) + the data IBM::spread needs
 // (couplb_ibm.h:92-133), stored as a MarkerRec.  The per-plane partial sums
 // are combined in plane order, so results are deterministic; they differ
@@ -226,7 +296,7 @@ template <int D>
 __global__ void __launch_bounds__(256) k_marker(int n, const int* perm, const double* x,
                                                 const double* v, const double* m, double* f_atom,
                                                 int fmode, IbmParams ip, Geo g, Walls w,
-                                                const double* u3, MarkerRec* rec, double* ufc) {
+                                                const double* u4, MarkerRec* rec, double* ufc) {
 #pragma clang fp contract(off)
   constexpr int LPM = (D == 3) ? 4 : 1;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -247,57 +317,19 @@ __global__ void __launch_bounds__(256) k_marker(int n, const int* perm, const do
     wy[t] = delta1(ip.kern, ly - (double)(j0 - 1 + t));
   }
   const double wzv = (D == 3) ? delta1(ip.kern, lz - (double)(k0 - 1 + vv)) : 1.0;
-  double ui = 0, vi = 0, wi = 0, ws = 0, wsf = 0;
-  const int c = (D == 3) ? k0 - 1 + vv : 0;
-  if (fin && !(D == 3 && (c < 0 || c > g.nz + 1))) {
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-      const int b = j0 - 1 + u;
-      if (b < 0 || b > g.ny + 1) continue;
-#pragma unroll
-      for (int t = 0; t < 4; t++) {
-        const int a = i0 - 1 + t;
-        if (a < 0 || a > g.nx + 1) continue;
-        int wd, wsd;
-        const int nt = node_type(w, g, a, b, c, wd, wsd);
-        const double d = (D == 3) ? (wx[t] * wy[u]) * wzv : wx[t] * wy[u];
-        if (nt == 0) {
-          const unsigned node = (unsigned)wrap_ghost(a, g.nx) +
-                                (unsigned)g.nx * ((unsigned)wrap_ghost(b, g.ny) +
-                                                  (unsigned)g.ny * (unsigned)((D == 3) ? wrap_ghost(c, g.nz) : 0));
-          const double* un = u3 + 3 * (size_t)node;
-          ui += un[0] * d;
-          vi += un[1] * d;
-          wi += un[2] * d;
-          ws += d;
-          wsf += d;
-        } else if (nt == 1 || nt == 3) {
-          ws += d;
-        } else if (nt == 2) {
-          ui += w.vel[wd][wsd][0] * d;
-          vi += w.vel[wd][wsd][1] * d;
-          wi += ((D == 3) ? w.vel[wd][wsd][2] : 0.0) * d;
-          ws += d;
-        }
-      }
-    }
-  }
+  double s[5];
+  interp_plane<D>(g, w, fin, i0, j0, k0, vv, wx, wy, wzv,
+                  [&](unsigned node, double* un) {
+                    const double4 q = *reinterpret_cast<const double4*>(u4 + 4 * (size_t)node);
+                    un[0] = q.x; un[1] = q.y; un[2] = q.z;
+                  }, s);
+  combine_planes<LPM>(s);
+  double ui = s[0], vi = s[1], wi = s[2], ws = s[3], wsf = s[4];
   double wz[4] = {1.0, 1.0, 1.0, 1.0};
   if (LPM > 1) {
-    // combine the planes in order 0, 1, 2, 3 (every lane of the group
-    // computes the same sums)
     const int lane0 = (threadIdx.x & 63) & ~(LPM - 1);
-    double s[5] = {0, 0, 0, 0, 0};
 #pragma unroll
-    for (int l = 0; l < LPM; l++) {
-      s[0] += __shfl(ui, lane0 + l);
-      s[1] += __shfl(vi, lane0 + l);
-      s[2] += __shfl(wi, lane0 + l);
-      s[3] += __shfl(ws, lane0 + l);
-      s[4] += __shfl(wsf, lane0 + l);
-      wz[l] = __shfl(wzv, lane0 + l);
-    }
-    ui = s[0]; vi = s[1]; wi = s[2]; ws = s[3]; wsf = s[4];
+    for (int l = 0; l < LPM; l++) wz[l] = __shfl(wzv, lane0 + l);
   }
   if (!live || vv != 0) return;
   double uf0 = 0, uf1 = 0, uf2 = 0;

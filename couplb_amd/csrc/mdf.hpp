@@ -1,0 +1,344 @@
+// mdf.hpp -- multi-direct-forcing IBM for markers with prescribed velocity.
+//
+// The reference couples once per sub-step with a penalty force
+// (fix_couplb.cpp:779-785).  Multi-direct forcing instead finds the marker
+// forces F_k (lattice units, force on the fluid) for which the velocity the
+// next collision sees, u = (j + F/2)/rho (Guo, couplb_collision.h:28-33),
+// interpolates to the prescribed marker velocity V_k:
+//
+//   r(F) = V - I(u* + S(F)/(2 rho)) = b - A F,   b = V - I(u*),
+//   A F  = I0(S(F)/(2 rho))
+//
+// with u* = j/rho of the state, S = IBM::spread (couplb_ibm.h:92-133) over
+// the sorted pairs of spread_pairs.hpp and I = IBM::interpolate
+// (couplb_ibm.h:57-90) over four lanes per marker as in k_marker (I0: the
+// same without the moving-wall velocity term, so A is linear).  A is
+// symmetric positive semi-definite wherever the stencils see fluid only
+// (interpolation and spreading then share the normalisation).  Two solvers:
+//   Richardson  F += 2 omega r                 (classic multi-direct forcing;
+//                                               one pass from F = 0 is the
+//                                               reference's penalty force with
+//                                               m xi/dt_LB = 2 omega)
+//   CG          conjugate gradients on A F = b, restarted from the true
+//               residual until max_k |r_k| < tol.
+// The loops run on the device; a flag in device memory ends them, so the
+// host queues several iterations between two convergence checks.
+// Reductions are per block in a fixed order, then over blocks in a fixed
+// order: results do not depend on scheduling.
+#pragma once
+#include "ibm.hpp"
+
+namespace couplb {
+
+constexpr int MDF_THREADS = 256;
+
+struct MdfScalars {
+  double rr, pAp, alpha, beta, smax, smax_first;
+  int done;    // 0 running, 1 converged, 2 breakdown (p.Ap <= 0), 3 iteration limit
+  int iters;   // operator applications counted by the loop kernels
+};
+
+// block reduction of one value per thread: sum (op 0) or max (op 1), fixed order
+__device__ inline double block_reduce(double v, int op, double* sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int h = MDF_THREADS / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) {
+      const double o = sh[threadIdx.x + h];
+      sh[threadIdx.x] = op ? (sh[threadIdx.x] > o ? sh[threadIdx.x] : o) : sh[threadIdx.x] + o;
+    }
+    __syncthreads();
+  }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+// stencil geometry and weights of each marker (as k_marker, no velocity
+// gather), the initial force F0 (caller order, lattice units; null = zero)
+// into rec.f and Fm, and the target velocity in sorted order, lattice units
+template <int D>
+__global__ void __launch_bounds__(256) k_mdf_init(int n, const int* perm, const double* x,
+                                                  const double* V, const double* F0, IbmParams ip,
+                                                  Geo g, Walls w, MarkerRec* rec, double* Vs,
+                                                  double* Fm) {
+#pragma clang fp contract(off)
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int idx = perm[p];
+  double lx, ly, lz;
+  to_local(ip, D, x + 3 * (size_t)idx, lx, ly, lz);
+  const bool fin = isfinite(lx) && isfinite(ly) && isfinite(lz) && fabs(lx) < 1e9 &&
+                   fabs(ly) < 1e9 && fabs(lz) < 1e9;
+  MarkerRec r;
+  r.i0 = fin ? (int)floor(lx) : -100;
+  r.j0 = fin ? (int)floor(ly) : -100;
+  r.k0 = (fin && D == 3) ? (int)floor(lz) : 0;
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+    r.wx[t] = delta1(ip.kern, lx - (double)(r.i0 - 1 + t));
+    r.wy[t] = delta1(ip.kern, ly - (double)(r.j0 - 1 + t));
+    r.wz[t] = (D == 3) ? delta1(ip.kern, lz - (double)(r.k0 - 1 + t)) : 1.0;
+  }
+  // fluid weight sum per stencil plane, planes combined in order: the same
+  // rounding as the four-lane sums of k_marker
+  double wsf = 0.0;
+  if (fin) {
+#pragma unroll
+    for (int v = 0; v < ((D == 3) ? 4 : 1); v++) {
+      const int c = (D == 3) ? r.k0 - 1 + v : 0;
+      double pl = 0.0;
+      if (!(D == 3 && (c < 0 || c > g.nz + 1))) {
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const int b = r.j0 - 1 + u;
+          if (b < 0 || b > g.ny + 1) continue;
+#pragma unroll
+          for (int t = 0; t < 4; t++) {
+            const int a = r.i0 - 1 + t;
+            if (a < 0 || a > g.nx + 1) continue;
+            int wd, wsd;
+            if (node_type(w, g, a, b, c, wd, wsd) != 0) continue;
+            pl += (D == 3) ? (r.wx[t] * r.wy[u]) * r.wz[v] : r.wx[t] * r.wy[u];
+          }
+        }
+      }
+      wsf += pl;
+    }
+  }
+  r.valid = (fin && !(wsf < DELTA_TOL)) ? 1 : 0;
+  r.sc = r.valid ? 1.0 / wsf : 0.0;
+  r.fx = (r.valid && F0) ? F0[3 * (size_t)idx + 0] : 0.0;
+  r.fy = (r.valid && F0) ? F0[3 * (size_t)idx + 1] : 0.0;
+  r.fz = (r.valid && F0) ? F0[3 * (size_t)idx + 2] : 0.0;
+  rec[p] = r;
+  Fm[3 * (size_t)p + 0] = r.fx;
+  Fm[3 * (size_t)p + 1] = r.fy;
+  Fm[3 * (size_t)p + 2] = r.fz;
+#pragma unroll
+  for (int c = 0; c < 3; c++) Vs[3 * (size_t)p + c] = V[3 * (size_t)idx + c] / ip.vel_scale;
+}
+
+// node values of the operator: ut = [u* +] S(rec.f)/(2 rho) at every pair node
+__global__ void __launch_bounds__(256) k_mdf_ut(long ne, const unsigned* skey, const int* pp,
+                                                const double* pw, unsigned sentinel,
+                                                const MarkerRec* rec, const double* u4, int affine,
+                                                double* ut, const MdfScalars* S) {
+  if (S->done) return;
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  const unsigned k = skey[e];
+  if (k == sentinel || (e > 0 && skey[e - 1] == k)) return;
+  double ax = 0.0, ay = 0.0, az = 0.0;
+  for (long f = e; f < ne && skey[f] == k; f++) {
+    const MarkerRec& r = rec[pp[f]];
+    const double wgt = pw[f];
+    ax += r.fx * wgt;
+    ay += r.fy * wgt;
+    az += r.fz * wgt;
+  }
+  const double4 q = *reinterpret_cast<const double4*>(u4 + 4 * (size_t)k);
+  const double h = 0.5 / q.w;
+  ut[3 * (size_t)k + 0] = (affine ? q.x : 0.0) + ax * h;
+  ut[3 * (size_t)k + 1] = (affine ? q.y : 0.0) + ay * h;
+  ut[3 * (size_t)k + 2] = (affine ? q.z : 0.0) + az * h;
+}
+
+// Interpolate ut at every marker.  affine: out = r = Vs - I(ut), block
+// partials (sum r.r, max |r|).  linear: out = A p = I0(ut), block partial
+// sum p.Ap (p = rec.f).
+template <int D>
+__global__ void __launch_bounds__(MDF_THREADS) k_mdf_interp(int n, Geo g, Walls w, const double* ut,
+                                                            int affine, const MarkerRec* rec,
+                                                            const double* Vs, double* out,
+                                                            double* part_a, double* part_b,
+                                                            const MdfScalars* S) {
+  __shared__ double sh[MDF_THREADS];
+  if (S->done) return;
+  constexpr int LPM = (D == 3) ? 4 : 1;
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = tid / LPM, vv = tid % LPM;
+  const bool live = p < n;
+  const bool valid = live && rec[p].valid;
+  double wx[4], wy[4], wzv = 0.0;
+  int i0 = 0, j0 = 0, k0 = 0;
+#pragma unroll
+  for (int t = 0; t < 4; t++) { wx[t] = 0.0; wy[t] = 0.0; }
+  if (valid) {
+    const MarkerRec& r = rec[p];
+#pragma unroll
+    for (int t = 0; t < 4; t++) { wx[t] = r.wx[t]; wy[t] = r.wy[t]; }
+    wzv = (D == 3) ? r.wz[vv] : 1.0;
+    i0 = r.i0; j0 = r.j0; k0 = r.k0;
+  }
+  double s[5];
+  interp_plane<D>(g, w, valid, i0, j0, k0, vv, wx, wy, wzv,
+                  [&](unsigned node, double* un) {
+                    un[0] = ut[3 * (size_t)node + 0];
+                    un[1] = ut[3 * (size_t)node + 1];
+                    un[2] = ut[3 * (size_t)node + 2];
+                  }, s, affine != 0);
+  combine_planes<LPM>(s);
+  double a = 0.0, b = 0.0;
+  if (live && vv == 0) {
+    double o0 = 0.0, o1 = 0.0, o2 = 0.0;
+    if (valid) {
+      const double iw = 1.0 / s[3];
+      const double u0 = s[0] * iw, u1 = s[1] * iw, u2 = s[2] * iw;
+      if (affine) {
+        o0 = Vs[3 * (size_t)p + 0] - u0;
+        o1 = Vs[3 * (size_t)p + 1] - u1;
+        o2 = Vs[3 * (size_t)p + 2] - u2;
+        a = o0 * o0 + o1 * o1 + o2 * o2;
+        b = sqrt(a);
+      } else {
+        o0 = u0; o1 = u1; o2 = u2;
+        const MarkerRec& r = rec[p];
+        a = r.fx * o0 + r.fy * o1 + r.fz * o2;
+      }
+    }
+    out[3 * (size_t)p + 0] = o0;
+    out[3 * (size_t)p + 1] = o1;
+    out[3 * (size_t)p + 2] = o2;
+  }
+  a = block_reduce(a, 0, sh);
+  b = block_reduce(b, 1, sh);
+  if (threadIdx.x == 0) {
+    part_a[blockIdx.x] = a;
+    part_b[blockIdx.x] = b;
+  }
+}
+
+// One block: fold the block partials (fixed order) into the scalars.
+//   phase 0  true residual:   rr, smax (first time also smax_first);
+//                              done = smax < tol; counts one application
+//   phase 1  alpha:            pAp; alpha = rr/pAp, done = 2 if pAp <= 0
+//   phase 2  beta:             rr' ; beta = rr'/rr, rr = rr', smax;
+//                              done = smax < tol; counts one application
+enum { MDF_RESID = 0, MDF_ALPHA = 1, MDF_BETA = 2 };
+//   Phases 0 and 2 set done = 3 once maxit applications are reached.
+//   respect_done: phase 0 also returns at once when done is set (loops).
+__global__ void __launch_bounds__(MDF_THREADS) k_mdf_scalar(const double* part_a,
+                                                            const double* part_b, int nblk,
+                                                            int phase, double tol, int first,
+                                                            int maxit, int respect_done,
+                                                            MdfScalars* S) {
+  __shared__ double sh[MDF_THREADS];
+  if ((phase != MDF_RESID || respect_done) && S->done) return;
+  double a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += MDF_THREADS) {
+    a += part_a[i];
+    if (phase != MDF_ALPHA) b = b > part_b[i] ? b : part_b[i];
+  }
+  a = block_reduce(a, 0, sh);
+  b = block_reduce(b, 1, sh);
+  if (threadIdx.x != 0) return;
+  if (phase == MDF_RESID) {
+    S->rr = a;
+    S->smax = b;
+    if (first) S->smax_first = b;
+    S->done = (b < tol) ? 1 : 0;
+    S->iters += 1;
+    if (!S->done && S->iters >= maxit) S->done = 3;
+  } else if (phase == MDF_ALPHA) {
+    S->pAp = a;
+    if (!(a > 0.0)) S->done = 2;
+    else S->alpha = S->rr / a;
+  } else {
+    S->beta = a / S->rr;
+    S->rr = a;
+    S->smax = b;
+    S->iters += 1;
+    if (b < tol) S->done = 1;
+    else if (S->iters >= maxit) S->done = 3;
+  }
+}
+
+// F += alpha p, r -= alpha Ap; block partials of r.r and max |r|
+__global__ void __launch_bounds__(MDF_THREADS) k_cg_update(int n, const MarkerRec* rec,
+                                                           const double* Ap, double* Fm, double* R,
+                                                           double* part_a, double* part_b,
+                                                           const MdfScalars* S) {
+  __shared__ double sh[MDF_THREADS];
+  if (S->done) return;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  double a = 0.0, b = 0.0;
+  if (p < n && rec[p].valid) {
+    const double al = S->alpha;
+    const MarkerRec& r = rec[p];
+    const double pv[3] = {r.fx, r.fy, r.fz};
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      Fm[3 * (size_t)p + c] += al * pv[c];
+      const double rc = R[3 * (size_t)p + c] - al * Ap[3 * (size_t)p + c];
+      R[3 * (size_t)p + c] = rc;
+      a += rc * rc;
+    }
+    b = sqrt(a);
+  }
+  a = block_reduce(a, 0, sh);
+  b = block_reduce(b, 1, sh);
+  if (threadIdx.x == 0) {
+    part_a[blockIdx.x] = a;
+    part_b[blockIdx.x] = b;
+  }
+}
+
+// p = r + beta p (beta = 0: restart with p = r), p lives in rec.f
+__global__ void __launch_bounds__(256) k_cg_direction(int n, const double* R, MarkerRec* rec,
+                                                      const MdfScalars* S, int restart) {
+  if (!restart && S->done) return;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n || !rec[p].valid) return;
+  const double be = restart ? 0.0 : S->beta;
+  MarkerRec& r = rec[p];
+  r.fx = R[3 * (size_t)p + 0] + be * r.fx;
+  r.fy = R[3 * (size_t)p + 1] + be * r.fy;
+  r.fz = R[3 * (size_t)p + 2] + be * r.fz;
+}
+
+// Richardson step F += 2 omega r (after a true-residual evaluation that did
+// not converge); rec.f carries F
+__global__ void __launch_bounds__(256) k_mdf_relax(int n, const double* R, double omega, double* Fm,
+                                                   MarkerRec* rec, const MdfScalars* S) {
+  if (S->done) return;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n || !rec[p].valid) return;
+  const double k2 = 2.0 * omega;
+  MarkerRec& r = rec[p];
+  const double f0 = Fm[3 * (size_t)p + 0] + k2 * R[3 * (size_t)p + 0];
+  const double f1 = Fm[3 * (size_t)p + 1] + k2 * R[3 * (size_t)p + 1];
+  const double f2 = Fm[3 * (size_t)p + 2] + k2 * R[3 * (size_t)p + 2];
+  Fm[3 * (size_t)p + 0] = f0; Fm[3 * (size_t)p + 1] = f1; Fm[3 * (size_t)p + 2] = f2;
+  r.fx = f0; r.fy = f1; r.fz = f2;
+}
+
+// rec.f = F (before a true residual and for the LB step's spreading)
+__global__ void __launch_bounds__(256) k_mdf_setf(int n, const double* Fm, MarkerRec* rec) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  MarkerRec& r = rec[p];
+  r.fx = r.valid ? Fm[3 * (size_t)p + 0] : 0.0;
+  r.fy = r.valid ? Fm[3 * (size_t)p + 1] : 0.0;
+  r.fz = r.valid ? Fm[3 * (size_t)p + 2] : 0.0;
+}
+
+__global__ void k_mdf_reset(MdfScalars* S, int clear_iters) {
+  S->done = 0;
+  if (clear_iters) S->iters = 0;
+}
+
+// force on the body per marker, caller order and units: -F * force_scale
+__global__ void __launch_bounds__(256) k_mdf_out(int n, const int* perm, const double* Fm,
+                                                 const MarkerRec* rec, double force_scale,
+                                                 double* f_body) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int idx = perm[p];
+  const bool v = rec[p].valid;
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+    f_body[3 * (size_t)idx + c] = v ? -Fm[3 * (size_t)p + c] * force_scale : 0.0;
+}
+
+}  // namespace couplb

@@ -96,6 +96,9 @@ int couplb_init_equilibrium(couplb_ctx* ctx, double rho, double ux, double uy, d
  * couplb_refresh_macro, couplb_diagnostics) before the next step, so the
  * density array need not be kept for the next step's gravity. */
 #define COUPLB_STEP_RHO_REFRESHED_NEXT 1
+/* Asynchronous on the context stream; calls that return data to the host
+ * (get_state_f, get_macro, diagnostics, profile_y, the host-array IBM
+ * sub-step) wait for it. */
 int couplb_lbm_step(couplb_ctx* ctx, const double g_lb[3], int flags);
 int couplb_lbm_steps_async(couplb_ctx* ctx, const double g_lb[3], int flags, int nsteps);
 
@@ -123,9 +126,47 @@ typedef struct couplb_ibm_args {
 /* host arrays (copied over PCIe each call) */
 int couplb_ibm_substep(couplb_ctx* ctx, int n, const double* x, const double* v,
                        const double* m, double* f_inout, const couplb_ibm_args* a);
-/* device arrays (already resident in HBM); f_inout is device memory */
+/* device arrays (already resident in HBM); f_inout is device memory.
+ * Asynchronous on the context stream: the arrays must stay valid until the
+ * stream has passed this call (couplb_synchronize). */
 int couplb_ibm_substep_device(couplb_ctx* ctx, int n, const double* x, const double* v,
                               const double* m, double* f_inout, const couplb_ibm_args* a);
+
+/* Multi-direct-forcing IBM for markers with prescribed velocity.  No
+ * counterpart in the reference, which couples with one penalty pass per
+ * sub-step (fix_couplb.cpp:779-785).  Iterates the marker forces F (force on
+ * the fluid, lattice units) until the velocity of the next collision,
+ * (j + F/2)/rho, interpolates to V at every marker:
+ *   slip(F) = V/vel_scale - I(u* + S(F)/(2 rho)) = 0   to max|slip| < tol,
+ * with u* = j/rho of the current state, I = IBM::interpolate and
+ * S = IBM::spread (couplb_ibm.h:57-133), by Richardson steps
+ * F += 2*relax*slip (classic multi-direct forcing) or by conjugate gradients
+ * (restarted from the true slip; falls back to Richardson if it breaks
+ * down).  The converged F is spread into the
+ * next LB step.  It must be the only IBM call between two LB steps.
+ * x, V (physical units), F0 (initial F in lattice units, caller order; NULL
+ * starts from zero) and f_body are device arrays [n][3]; f_body receives the
+ * force on the body per marker, -F*force_scale.  Iterations run on the
+ * device and stop on a device-side flag; the host checks it every few
+ * iterations.  stats->iterations counts interpolation passes. */
+#define COUPLB_MDF_RICHARDSON 0  /* F += 2*relax*slip, the classic iteration */
+#define COUPLB_MDF_CG 1          /* conjugate gradients on the same fixed point */
+typedef struct couplb_mdf_args {
+  couplb_ibm_args ibm;  /* lo, inv_dx, vel_scale, force_scale, positions_unchanged */
+  double tol;           /* slip tolerance, lattice velocity units */
+  double relax;         /* omega of the Richardson iteration */
+  int max_iter;         /* operator applications (interpolation passes) */
+  int method;           /* COUPLB_MDF_RICHARDSON or COUPLB_MDF_CG */
+} couplb_mdf_args;
+typedef struct couplb_mdf_stats {
+  int iterations;       /* interpolations performed */
+  int converged;        /* 1 when max|slip| < tol was reached */
+  double slip_first;    /* max|slip| at the initial force */
+  double slip_last;     /* max|slip| before the last update */
+} couplb_mdf_stats;
+int couplb_ibm_mdf_device(couplb_ctx* ctx, int n, const double* x, const double* V,
+                          const double* F0, double* f_body, const couplb_mdf_args* a,
+                          couplb_mdf_stats* stats);
 
 /* compute_macroscopic(false) semantics without copying anything out: marks
  * the density as refreshed for the next step's gravity (fix_couplb.cpp:706,
@@ -150,10 +191,15 @@ int couplb_synchronize(couplb_ctx* ctx);
 /* the context's hipStream_t, for callers that time with HIP events */
 void* couplb_stream(couplb_ctx* ctx);
 
-/* Timing of the last call's kernels (milliseconds, HIP events on the
- * context stream): which = 0 fused LB kernel, 1 IBM sub-step total,
- * 2 force spreading that preceded the last LB kernel. */
-double couplb_last_kernel_ms(const couplb_ctx* ctx, int which);
+/* Kernel timing from HIP events on the context stream, read back without
+ * stalling the stream: which = 0 fused LB kernel, 1 IBM sub-step (all its
+ * kernels), 2 force spreading before an LB kernel, 3 MDF solve.
+ * couplb_last_kernel_ms waits for the newest region of that kind and returns
+ * its milliseconds; couplb_kernel_ms_total returns the sum and count of all
+ * regions since the last reset. */
+double couplb_last_kernel_ms(couplb_ctx* ctx, int which);
+int couplb_kernel_ms_total(couplb_ctx* ctx, int which, double* total_ms, long* count);
+int couplb_kernel_ms_reset(couplb_ctx* ctx);
 
 #ifdef __cplusplus
 }

@@ -348,3 +348,130 @@ def read_checkpoint(fname, D, Q, dims, glob, nf):
     if not ok:
         raise IOError(err.value.decode())
     return step.value, f
+
+
+class Grid:
+    """One numerics-table grid (couplb_lattice.h Grid<L>) driven directly
+    through the ops table: create with walls and per-face velocities, set
+    the populations, and call the reference's building blocks."""
+
+    def __init__(self, D, n, tau=0.8, walls=((0, 0), (0, 0), (0, 0)), face_vel=None,
+                 numerics="oracle"):
+        self.ops = ops(numerics).contents
+        p = GridParams()
+        p.D, p.nx, p.ny, p.nz, p.tau, p.rho_lb = D, n[0], n[1], n[2] if D == 3 else 1, tau, 1.0
+        for d in range(3):
+            for s in range(2):
+                p.wall[d][s] = walls[d][s]
+                for c in range(3):
+                    p.face_vel[d][s][c] = face_vel[d][s][c] if face_vel is not None else 0.0
+        create = C.CFUNCTYPE(C.c_void_p, C.POINTER(GridParams))(self.ops.create)
+        self.g = create(C.byref(p))
+        self._destroy = C.CFUNCTYPE(None, C.c_void_p)(self.ops.destroy)
+        gd = (C.c_int * 3)()
+        nt = C.c_long()
+        q = C.c_int()
+        self.ops.dims(self.g, gd, C.byref(nt), C.byref(q))
+        self.gdims, self.nt, self.Q, self.D = (gd[0], gd[1], gd[2]), nt.value, q.value, D
+
+    def field(self, which):
+        n = self.Q * self.nt if which in (F_F, F_FBUF) else self.nt
+        return np.ctypeslib.as_array(self.ops.dfield(self.g, which), shape=(n,))
+
+    def close(self):
+        if getattr(self, "g", None):
+            self._destroy(self.g)
+            self.g = None
+
+    def __del__(self):
+        self.close()
+
+
+def mdf_solve(grid, x, V, lo, inv_dx, kern, tol, relax=1.0, max_iter=200, vel_scale=1.0,
+              F0=None, method="richardson"):
+    """Multi-direct forcing restated over the reference's own interpolate /
+    spread / exchange building blocks (couplb_ibm.h:57-133,
+    couplb_streaming.h:140-158).  No reference counterpart: the reference
+    couples with one penalty pass.  With u* = j/rho of the grid's state the
+    slip of a marker force set F (force on the fluid, lattice units) is
+
+      r(F) = V/vel_scale - I(u* + S(F)/(2 rho))
+
+    richardson: F += 2 relax r until max_k |r_k| < tol (no update after the
+    converged evaluation); cg: conjugate gradients on A F = r(0) with
+    A p = I(u* + S(p)/(2 rho)) - I(u*), restarted from the true slip when
+    the recursive one converges.  Iterations count slip / operator
+    evaluations.  Returns (F, iterations, slip_first, slip_last); the grid
+    is left with F spread into its force fields (folded) and u* in its
+    velocity fields."""
+    o, g = grid.ops, grid.g
+    n = len(x)
+    x = np.ascontiguousarray(x, np.float64)
+    Vl = np.ascontiguousarray(V, np.float64) / vel_scale
+    F = np.zeros((n, 3)) if F0 is None else np.array(F0, np.float64)
+    lo_a = (C.c_double * 3)(*lo)
+    out = (C.c_double * 3)()
+    o.compute_macroscopic(g, 0)
+    o.exchange_velocity(g)
+    rho = grid.field(F_RHO).copy()
+    ustar = [grid.field(k).copy() for k in (F_UX, F_UY, F_UZ)]
+    fields_u = [grid.field(k) for k in (F_UX, F_UY, F_UZ)]
+    fields_f = [grid.field(k) for k in (F_FX, F_FY, F_FZ)]
+    dx = 1.0 / inv_dx
+    dv = dx ** grid.D
+
+    def spread(Fs):
+        o.clear_forces(g)
+        for k in range(n):
+            o.spread(g, x[k, 0], x[k, 1], x[k, 2], Fs[k, 0], Fs[k, 1], Fs[k, 2], dv, lo_a, inv_dx,
+                     dx, kern)
+        o.exchange_forces(g)
+
+    def interp_at(Fs):
+        """I(u* + S(Fs)/(2 rho)) at every marker"""
+        spread(Fs)
+        for c in range(3):
+            fields_u[c][:] = ustar[c] + fields_f[c] * (0.5 / rho)
+        o.exchange_velocity(g)
+        U = np.empty((n, 3))
+        for k in range(n):
+            o.interpolate(g, x[k, 0], x[k, 1], x[k, 2], lo_a, inv_dx, kern, out)
+            U[k] = out[:]
+        return U
+
+    smax = lambda r: float(np.sqrt((r ** 2).sum(1)).max())
+    U0 = interp_at(np.zeros((n, 3)))
+    r = Vl - interp_at(F)
+    it = 1
+    slip_first = slip_last = smax(r)
+    if method == "richardson":
+        while slip_last >= tol and it < max_iter:
+            F = F + 2.0 * relax * r
+            r = Vl - interp_at(F)
+            it += 1
+            slip_last = smax(r)
+    else:
+        while slip_last >= tol and it < max_iter:
+            p = r.copy()
+            rr = float((r * r).sum())
+            while it < max_iter:
+                Ap = interp_at(p) - U0
+                pAp = float((p * Ap).sum())
+                if not pAp > 0:
+                    break
+                alpha = rr / pAp
+                F = F + alpha * p
+                r = r - alpha * Ap
+                it += 1
+                rr_new = float((r * r).sum())
+                if smax(r) < tol:
+                    break
+                p = r + (rr_new / rr) * p
+                rr = rr_new
+            r = Vl - interp_at(F)     # true slip
+            it += 1
+            slip_last = smax(r)
+    spread(F)
+    for c in range(3):
+        fields_u[c][:] = ustar[c]
+    return F, it, slip_first, slip_last

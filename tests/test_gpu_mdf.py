@@ -1,0 +1,149 @@
+"""Multi-direct-forcing IBM on the device (couplb_ibm_mdf_device) against
+the oracle's restatement over the reference's interpolate / spread building
+blocks (oracle.mdf_solve), its reduction to the penalty force at one
+iteration, and the momentum balance of the converged forces."""
+import numpy as np
+import pytest
+
+import oracle
+from tests.scenarios import DeviceArray, interior, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+N3 = (20, 18, 16)
+WALLS = ((0, 0), (2, 2), (0, 0))
+FACE = [[[0, 0, 0], [0, 0, 0]], [[-0.02, 0, 0], [0.02, 0, 0]], [[0, 0, 0], [0, 0, 0]]]
+
+
+def sphere(center, radius, n):
+    """Fibonacci lattice on a sphere."""
+    k = np.arange(n) + 0.5
+    phi = np.arccos(1 - 2 * k / n)
+    th = np.pi * (1 + 5 ** 0.5) * k
+    return np.stack([center[0] + radius * np.cos(th) * np.sin(phi),
+                     center[1] + radius * np.sin(th) * np.sin(phi),
+                     center[2] + radius * np.cos(phi)], 1)
+
+
+def developed_device(kernel, walls=WALLS, face=FACE, steps=30):
+    import couplb_amd
+    d = couplb_amd.Device(3, *N3, tau=0.8, walls=walls, face_vel=face, kernel=kernel)
+    d.init_equilibrium(1.0, (0.01, 0.0, 0.002))
+    d.steps_async(steps, (0.0, 0.0, 0.0), 0)
+    d.sync()
+    return d
+
+
+def markers():
+    x = sphere((9.6, 8.7, 7.9), 3.5, 150)
+    r = x - np.array([9.6, 8.7, 7.9])
+    V = np.cross(np.array([0.0, 0.0, 2e-3]), r) + np.array([2e-3, 0.0, 0.0])
+    return x, V
+
+
+def _oracle_grid(d):
+    g = oracle.Grid(3, N3, tau=0.8, walls=WALLS, face_vel=FACE)
+    g.field(oracle.F_F)[:] = d.get_state().reshape(-1)
+    return g
+
+
+@pytest.mark.parametrize("kernel", ["roma", "peskin4"])
+def test_mdf_richardson_matches_oracle_and_next_step(kernel):
+    # a fixed number of Richardson passes: the device and the oracle apply the
+    # same operator the same number of times, so forces agree to rounding
+    import couplb_amd
+    kern = couplb_amd.ROMA if kernel == "roma" else couplb_amd.PESKIN4
+    d = developed_device(kern)
+    x, V = markers()
+    n = len(x)
+    dx, dV, fb = DeviceArray(x), DeviceArray(V), DeviceArray(np.zeros((n, 3)))
+    print("case MDF Richardson vs oracle", kernel, flush=True)
+    st = d.mdf_device(n, dx.addr, dV.addr, None, fb.addr, (0, 0, 0), 1.0, 1.0, 1.0,
+                      tol=1e-30, relax=1.0, max_iter=40, method="richardson")
+    F_dev = -fb.download()
+    g = _oracle_grid(d)
+    F_ref, it, s0, s1 = oracle.mdf_solve(g, x, V, (0, 0, 0), 1.0, kern, 1e-30, relax=1.0,
+                                         max_iter=40, method="richardson")
+    print("device", st, "oracle", it, s0, s1, "rel F", rel_l2(F_dev, F_ref))
+    assert st["iterations"] == it == 40 and not st["converged"]
+    assert st["slip_first"] == pytest.approx(s0, rel=1e-12)
+    assert st["slip_last"] == pytest.approx(s1, rel=1e-9)
+    assert st["slip_last"] < 0.2 * st["slip_first"]
+    assert rel_l2(F_dev, F_ref) < 1e-12
+    # the forces enter the next collision (Guo) on both sides
+    d.step((0.0, 0.0, 0.0), 0)
+    o = g.ops
+    o.compute_macroscopic(g.g, 1)
+    o.collide(g.g)
+    o.exchange(g.g)
+    o.stream(g.g)
+    fd = interior(d.get_state(), 3)
+    fo = interior(g.field(oracle.F_F).reshape(19, N3[2] + 2, N3[1] + 2, N3[0] + 2), 3)
+    print("next step rel-L2(f) =", rel_l2(fd, fo))
+    assert rel_l2(fd, fo) < 1e-12
+
+
+@pytest.mark.parametrize("kernel", ["roma", "peskin4"])
+def test_mdf_cg_converges_like_oracle(kernel):
+    import couplb_amd
+    kern = couplb_amd.ROMA if kernel == "roma" else couplb_amd.PESKIN4
+    d = developed_device(kern)
+    x, V = markers()
+    n = len(x)
+    dx, dV, fb = DeviceArray(x), DeviceArray(V), DeviceArray(np.zeros((n, 3)))
+    print("case MDF CG vs oracle", kernel, flush=True)
+    st = d.mdf_device(n, dx.addr, dV.addr, None, fb.addr, (0, 0, 0), 1.0, 1.0, 1.0,
+                      tol=1e-10, max_iter=400, method="cg")
+    F_dev = -fb.download()
+    F_ref, it, s0, s1 = oracle.mdf_solve(_oracle_grid(d), x, V, (0, 0, 0), 1.0, kern, 1e-10,
+                                         max_iter=400, method="cg")
+    print("device", st, "oracle", it, s0, s1, "rel F", rel_l2(F_dev, F_ref))
+    assert st["converged"] and st["slip_last"] < 1e-10 and s1 < 1e-10
+    # CG on an ill-conditioned A: rounding moves the count a little
+    assert abs(st["iterations"] - it) <= max(3, it // 5)
+    assert st["slip_first"] == pytest.approx(s0, rel=1e-12)
+    assert rel_l2(F_dev, F_ref) < 1e-6
+
+
+def test_mdf_one_iteration_equals_penalty_step():
+    import couplb_amd
+    x, V = markers()
+    n = len(x)
+    states = []
+    for mode in ("penalty", "mdf"):
+        d = developed_device(couplb_amd.PESKIN4)
+        dx, dV = DeviceArray(x), DeviceArray(V)
+        fb = DeviceArray(np.zeros((n, 3)))
+        if mode == "penalty":
+            dm = DeviceArray(np.full(n, 2.0))   # m xi / dt_LB = 2 relax with relax = 1
+            d.ibm_device(n, dx.addr, dV.addr, dm.addr, fb.addr, (0, 0, 0), 1.0, 1.0, 1.0, 1.0, 1.0)
+        else:
+            # slip at F = 0, one Richardson update F = 2 relax slip, slip again
+            st = d.mdf_device(n, dx.addr, dV.addr, None, fb.addr, (0, 0, 0), 1.0, 1.0, 1.0,
+                              tol=1e-30, relax=1.0, max_iter=2, method="richardson")
+            assert st["iterations"] == 2
+        d.step((0.0, 0.0, 0.0), 0)
+        d.sync()
+        states.append((d.get_state(), fb.download()))
+    (f1, a1), (f2, a2) = states
+    assert np.array_equal(a1, a2)           # same force on the markers, bit for bit
+    assert np.array_equal(interior(f1, 3), interior(f2, 3))
+
+
+def test_mdf_momentum_balance_periodic():
+    # fully periodic box: the fluid momentum gained over one LB step equals the
+    # total converged marker force (spreading conserves force, Guo adds F)
+    import couplb_amd
+    d = developed_device(couplb_amd.PESKIN4, walls=((0, 0), (0, 0), (0, 0)), face=None, steps=10)
+    x, V = markers()
+    n = len(x)
+    dx, dV, fb = DeviceArray(x), DeviceArray(V), DeviceArray(np.zeros((n, 3)))
+    m0 = d.diagnostics()["mom"]
+    st = d.mdf_device(n, dx.addr, dV.addr, None, fb.addr, (0, 0, 0), 1.0, 1.0, 1.0,
+                      tol=1e-9, relax=1.0, max_iter=400)
+    assert st["converged"]
+    Ftot = -fb.download().sum(0)
+    d.step((0.0, 0.0, 0.0), 0)
+    m1 = d.diagnostics()["mom"]
+    print("dmom", m1 - m0, "sum F", Ftot)
+    assert np.max(np.abs((m1 - m0) - Ftot)) < 1e-12 * max(1.0, np.abs(Ftot).max()) * 1e3

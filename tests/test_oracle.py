@@ -370,3 +370,79 @@ def test_opposite_moving_walls_couette_closed_form():
     assert np.max(np.abs(ux - exact[:, None])) < 1e-10
     uy = fx.field(oracle.F_UY).reshape(18, 6)[1:-1, 1:-1]
     assert np.max(np.abs(uy)) < 1e-12
+
+
+def _uniform_grid(D, n, u0):
+    g = oracle.Grid(D, n)
+    e, w = oracle.lattice(D)
+    f = g.field(oracle.F_F).reshape(len(w), -1)
+    eu = e @ u0
+    f[:] = (w * (1 + 3 * eu + 4.5 * eu ** 2 - 1.5 * u0 @ u0))[:, None]
+    return g
+
+
+@pytest.mark.parametrize("D", [2, 3])
+@pytest.mark.parametrize("kern", [oracle.ROMA, oracle.PESKIN4])
+def test_mdf_single_marker_closed_form(D, kern):
+    # one marker in uniform flow u0 (rho = 1): U = u0 + c F / 2 with
+    # c = prod_d sum_i delta(r_d - i)^2, so the fixed point is F = 2 (V - u0) / c
+    u0 = np.array([0.02, -0.01, 0.005 if D == 3 else 0.0])
+    g = _uniform_grid(D, (12, 11, 10), u0)
+    x = np.array([[5.3, 6.71, 4.45 if D == 3 else 0.0]])
+    V = np.array([[0.004, 0.001, -0.002 if D == 3 else 0.0]])
+    F, it, s0, s1 = oracle.mdf_solve(g, x, V, (0, 0, 0), 1.0, kern, 1e-13, relax=1.0, max_iter=2000)
+    c = 1.0
+    for d in range(D):
+        l = x[0, d] + 1.0                  # phys_to_local, couplb_ibm.h:49-55
+        i0 = np.floor(l)
+        c *= sum(oracle.delta(kern, l - (i0 - 1 + t)) ** 2 for t in range(4))
+    exact = 2 * (V[0] - u0) / c
+    assert s1 < 1e-13 and s0 > 1e-3
+    np.testing.assert_allclose(F[0, :D], exact[:D], rtol=1e-10, atol=1e-14)
+
+
+def test_mdf_first_iteration_is_the_penalty_force():
+    # from F = 0, one iteration gives F = 2 relax (V - I(u*)): the reference's
+    # penalty force m xi/dt_LB (v - u_f) with m xi/dt_LB = 2 relax
+    rng = np.random.default_rng(4)
+    g = _uniform_grid(3, (10, 9, 8), np.array([0.01, 0.0, -0.01]))
+    f = g.field(oracle.F_F)
+    f *= 1 + 0.01 * rng.standard_normal(f.shape)
+    x = rng.uniform(1.0, 7.0, (6, 3))
+    V = rng.normal(0, 1e-2, (6, 3))
+    F, it, _, _ = oracle.mdf_solve(g, x, V, (0, 0, 0), 1.0, oracle.PESKIN4, 1e-30, relax=0.7,
+                                   max_iter=2)
+    assert it == 2     # the slip at F = 0, one update, the slip after it
+    g.ops.compute_macroscopic(g.g, 0)
+    g.ops.exchange_velocity(g.g)
+    out = (C.c_double * 3)()
+    lo = (C.c_double * 3)(0, 0, 0)
+    for k in range(6):
+        g.ops.interpolate(g.g, *x[k], lo, 1.0, oracle.PESKIN4, out)
+        assert np.array_equal(F[k], 2 * 0.7 * (V[k] - np.array(out[:])))
+
+
+def test_mdf_cg_and_richardson_reach_the_same_forces():
+    # markers on a sphere in a sheared, walled box: both solvers converge to
+    # the one fixed point; conjugate gradients in far fewer passes
+    rng = np.random.default_rng(8)
+    face = [[[0, 0, 0], [0, 0, 0]], [[-0.02, 0, 0], [0.02, 0, 0]], [[0, 0, 0], [0, 0, 0]]]
+    res = {}
+    for method in ("richardson", "cg"):
+        g = oracle.Grid(3, (14, 12, 12), walls=((0, 0), (2, 2), (0, 0)), face_vel=face)
+        e, w = oracle.lattice(3)
+        f = g.field(oracle.F_F).reshape(19, -1)
+        f[:] = w[:, None] * (1 + 0.002 * np.random.default_rng(3).standard_normal(f.shape))
+        k = np.arange(40) + 0.5
+        phi, th = np.arccos(1 - 2 * k / 40), np.pi * (1 + 5 ** 0.5) * k
+        x = np.stack([6.6 + 2.2 * np.cos(th) * np.sin(phi), 5.9 + 2.2 * np.sin(th) * np.sin(phi),
+                      6.1 + 2.2 * np.cos(phi)], 1)
+        V = np.tile([0.003, 0.0, -0.001], (40, 1))
+        res[method] = oracle.mdf_solve(g, x, V, (0, 0, 0), 1.0, oracle.PESKIN4, 1e-8,
+                                       max_iter=8000, method=method, relax=1.5)
+    (Fr, ir, _, sr), (Fc, ic, _, sc) = res["richardson"], res["cg"]
+    assert sr < 1e-8 and sc < 1e-8
+    assert ic < ir / 20
+    # a slip tolerance bounds the force error only by tol / lambda_min(A); with
+    # overlapping Peskin4 stencils A is ill-conditioned, so compare at 1e-3
+    assert np.max(np.abs(Fr - Fc)) < 1e-3 * np.max(np.abs(Fc))

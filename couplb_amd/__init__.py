@@ -30,6 +30,8 @@ STORAGE = {"fp64": FP64, "fp32": FP32, "fp16": FP16, "bf16": BF16}
 RHO_REFRESHED_NEXT = 1
 
 _dp = C.POINTER(C.c_double)
+_fp = C.POINTER(C.c_float)
+MLP_INPUTS, MLP_OUTPUTS, MLP_NPARAM = 67, 3, 8707
 
 
 class Params(C.Structure):
@@ -112,6 +114,14 @@ def lib() -> C.CDLL:
         "couplb_kernel_ms_reset": (ip, [vp]),
         "couplb_ibm_mdf_device": (ip, [vp, ip, vp, vp, vp, vp, C.POINTER(MdfArgs),
                                        C.POINTER(MdfStats)]),
+        "couplb_mlp_create": (ip, [ip, C.c_ulonglong, C.POINTER(vp)]),
+        "couplb_mlp_destroy": (ip, [vp]),
+        "couplb_mlp_last_error": (C.c_char_p, [vp]),
+        "couplb_mlp_get_params": (ip, [vp, _fp]),
+        "couplb_mlp_set_params": (ip, [vp, _fp]),
+        "couplb_mlp_forward_device": (ip, [vp, ip, vp, vp]),
+        "couplb_mlp_gradient_device": (ip, [vp, ip, vp, vp, _fp, dp]),
+        "couplb_mlp_train_device": (ip, [vp, ip, vp, vp, ip, ip, C.c_float, C.c_ulonglong, dp]),
         "couplb_fix_create": (ip, [ip, C.POINTER(C.c_char_p), C.POINTER(Domain), C.POINTER(vp)]),
         "couplb_fix_destroy": (ip, [vp]),
         "couplb_fix_last_error": (C.c_char_p, [vp]),
@@ -294,6 +304,58 @@ class Device:
 
     def kernel_ms_reset(self):
         self._chk(lib().couplb_kernel_ms_reset(self.h), "kernel_ms_reset")
+
+
+class Mlp:
+    """Warm-start network of the multi-direct-forcing IBM (include/couplb_mlp.h).
+    Array arguments of the *_device methods are device pointers."""
+
+    def __init__(self, seed=0, device=0):
+        h = C.c_void_p()
+        rc = lib().couplb_mlp_create(device, seed, C.byref(h))
+        self.h = h
+        if rc != OK:
+            raise CoupLBError(f"mlp_create: {lib().couplb_mlp_last_error(h).decode()} (rc={rc})")
+
+    def _chk(self, rc, what):
+        if rc != OK:
+            raise CoupLBError(f"{what}: {lib().couplb_mlp_last_error(self.h).decode()} (rc={rc})")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().couplb_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def params(self):
+        p = np.zeros(MLP_NPARAM, np.float32)
+        self._chk(lib().couplb_mlp_get_params(self.h, p.ctypes.data_as(_fp)), "get_params")
+        return p
+
+    @params.setter
+    def params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.shape == (MLP_NPARAM,)
+        self._chk(lib().couplb_mlp_set_params(self.h, p.ctypes.data_as(_fp)), "set_params")
+
+    def forward_device(self, n, X_ptr, Y_ptr):
+        self._chk(lib().couplb_mlp_forward_device(self.h, n, X_ptr, Y_ptr), "forward")
+
+    def gradient_device(self, n, X_ptr, T_ptr):
+        g = np.zeros(MLP_NPARAM, np.float32)
+        loss = C.c_double()
+        self._chk(lib().couplb_mlp_gradient_device(self.h, n, X_ptr, T_ptr, g.ctypes.data_as(_fp),
+                                                   C.byref(loss)), "gradient")
+        return loss.value, g
+
+    def train_device(self, n, X_ptr, T_ptr, batch=4096, epochs=1, lr=1e-3, seed=0):
+        loss = C.c_double()
+        self._chk(lib().couplb_mlp_train_device(self.h, n, X_ptr, T_ptr, batch, epochs, lr, seed,
+                                                C.byref(loss)), "train")
+        return loss.value
 
 
 class Fix:

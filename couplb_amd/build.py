@@ -4,7 +4,8 @@
 
 Sources: couplb_amd/csrc/*.hip (HIP for gfx950) and couplb_amd/host/*.cpp
 (the fix's host C++), linked into one shared library whose C ABI is
-declared in include/couplb_hip.h and include/couplb_fix.h.
+declared in include/couplb_hip.h, include/couplb_mlp.h and
+include/couplb_fix.h.
 """
 from __future__ import annotations
 
@@ -34,12 +35,14 @@ def build(verbose: bool = True) -> str:
     bdir = os.path.join(HERE, "_build")
     os.makedirs(bdir, exist_ok=True)
     obj_hip = os.path.join(bdir, "couplb_hip.o")
+    obj_mlp = os.path.join(bdir, "mlp.o")
     obj_fix = os.path.join(bdir, "fix_couplb_core.o")
     _run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(HERE, "csrc", "couplb_hip.hip"), "-o", obj_hip])
+    _run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(HERE, "csrc", "mlp.hip"), "-o", obj_mlp])
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-c",
           os.path.join(HERE, "host", "fix_couplb_core.cpp"), "-o", obj_fix])
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, obj_hip, obj_fix])
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, obj_hip, obj_mlp, obj_fix])
     os.replace(tmp, LIB)
     return LIB
 

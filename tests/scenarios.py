@@ -123,7 +123,8 @@ def rel_l2(a, b):
 
 
 class DeviceArray:
-    """FP64 device buffer for the device-pointer entry points, allocated with
+    """Device buffer (FP64 unless dtype says otherwise) for the device-pointer
+    entry points, allocated with
     the HIP runtime the library itself links (no second runtime in the
     process)."""
     _hip = None
@@ -141,15 +142,15 @@ class DeviceArray:
             cls._hip = h
         return cls._hip
 
-    def __init__(self, a):
+    def __init__(self, a, dtype=np.float64):
         import ctypes as C
-        self.host = np.ascontiguousarray(a, np.float64)
+        self.host = np.ascontiguousarray(a, dtype)
         self.ptr = C.c_void_p()
         assert self.hip().hipMalloc(C.byref(self.ptr), max(8, self.host.nbytes)) == 0
         self.upload(self.host)
 
     def upload(self, a):
-        a = np.ascontiguousarray(a, np.float64)
+        a = np.ascontiguousarray(a, self.host.dtype)
         assert a.shape == self.host.shape
         assert self.hip().hipMemcpy(self.ptr, a.ctypes.data, a.nbytes, 1) == 0
 

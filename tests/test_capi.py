@@ -10,7 +10,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared_symbols():
     names = []
-    for h in ("couplb_hip.h", "couplb_fix.h"):
+    for h in sorted(f for f in os.listdir(os.path.join(ROOT, "include")) if f.endswith(".h")):
         txt = open(os.path.join(ROOT, "include", h)).read()
         txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
         names += re.findall(r"\b(couplb_\w+)\s*\(", txt)
@@ -20,6 +20,7 @@ def declared_symbols():
 def test_headers_declare_entry_points():
     syms = declared_symbols()
     assert "couplb_lbm_step" in syms and "couplb_fix_post_force" in syms
+    assert "couplb_ibm_mdf_device" in syms and "couplb_mlp_train_device" in syms
     assert len(syms) >= 30
 
 

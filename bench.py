@@ -126,6 +126,244 @@ def pmc_traffic(storage, n):
     return None
 
 
+
+# ---------------------------------------------------------------------------
+# Further workloads (not the headline line): python bench.py --workload c2|c4|c5
+# ---------------------------------------------------------------------------
+D3Q19_E = np.array([[0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0],
+                    [0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1],
+                    [0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1]], float).T
+D3Q19_W = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12)
+
+
+def feq_field(rho, u):
+    """D3Q19 equilibrium w rho (1 + 3 e.u + 4.5 (e.u)^2 - 1.5 u.u)
+    (couplb_lattice.h:137-144) for fields rho (...), u (3, ...), velocities
+    in the reference's order (couplb_lattice.h:49-57): returns (19, ...)."""
+    eu = np.tensordot(D3Q19_E, u, axes=(1, 0))
+    uu = (u * u).sum(0)
+    w = D3Q19_W.reshape((19,) + (1,) * np.ndim(uu))
+    return w * rho * (1.0 + 3.0 * eu + 4.5 * eu * eu - 1.5 * uu)
+
+
+def ghost_coords(n):
+    """cell-centre coordinate of ghost-inclusive index 0..n+1 (x = i - 1 + 1/2)"""
+    return np.arange(-1, n + 1) + 0.5
+
+
+def workload_c2(steps):
+    """D3Q19 channel 128^3, walls in y, g_x = 1e-6, initial Poiseuille + seeded
+    uniform noise 1e-4: throughput and profile at FP64 and FP32."""
+    import couplb_amd
+    n, nu, g = 128, 0.1, 1e-6
+    y = ghost_coords(n)
+    prof = g / (2 * nu) * (y + 0.0) * (n - y)           # u(y) with walls at y = 0, n
+    rng = np.random.default_rng(1234)
+    shape = (n + 2, n + 2, n + 2)
+    u = np.zeros((3,) + shape)
+    u[0] = prof[None, :, None]
+    u += rng.uniform(-1e-4, 1e-4, u.shape)
+    f0 = feq_field(np.ones(shape), u)
+    del u
+    out = {}
+    fields = {}
+    exact = g / (2 * nu) * (np.arange(n) + 0.5) * (n - 0.5 - np.arange(n))
+    for storage in ("fp64", "fp32"):
+        d = couplb_amd.Device(3, n, n, n, tau=3 * nu + 0.5, walls=((0, 0), (1, 1), (0, 0)),
+                              storage=storage)
+        d.set_state(f0)
+        d.sync()
+        t0 = time.perf_counter()
+        d.steps_async(steps, (g, 0.0, 0.0), 0)
+        d.sync()
+        dt = time.perf_counter() - t0
+        rho, ux, uy, uz = d.macro()
+        fields[storage] = np.stack([ux, uy, uz])
+        px = ux.mean(axis=(0, 2))
+        out[storage] = dict(mlups=round(n ** 3 * steps / dt / 1e6, 1),
+                            profile_max_rel_err=float(np.max(np.abs(px - exact)) / exact.max()),
+                            mass=float(rho.sum()))
+        d.close()
+    a, b = fields["fp32"], fields["fp64"]
+    out["fp32"]["rel_l2_u_vs_fp64"] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    out["fp32"]["rel_linf_u_vs_fp64"] = float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+    return dict(metric="D3Q19 128^3 channel MLUPS and profile error", unit="MLUPS",
+                value=out["fp64"]["mlups"], steps=steps, data="synthetic (seeded noise)",
+                config={"workload": "D3Q19 128^3 channel, no-slip walls in y, g_x=1e-6, nu=0.1, "
+                                    "initial Poiseuille + uniform noise 1e-4"}, results=out)
+
+
+def workload_c4(steps, n=384):
+    """Taylor-Green vortex n^3 (z-invariant), u0 = 0.03, k = 2 pi / n,
+    nu = 0.02: FP64 against FP32 / FP16 / BF16 deviation storage."""
+    import couplb_amd
+    nu, u0 = 0.02, 0.03
+    k = 2 * np.pi / n
+    c = ghost_coords(n)
+    X, Y = np.meshgrid(c, c, indexing="xy")          # (gy, gx)
+    ux = u0 * np.sin(k * X) * np.cos(k * Y)
+    uy = -u0 * np.cos(k * X) * np.sin(k * Y)
+    rho = 1.0 + 0.75 * u0 * u0 * (np.cos(2 * k * X) + np.cos(2 * k * Y))
+    f2 = feq_field(rho, np.stack([ux, uy, np.zeros_like(ux)]))          # (19, gy, gx)
+    f0 = np.ascontiguousarray(np.broadcast_to(f2[:, None], (19, n + 2, n + 2, n + 2)))
+    del f2
+    decay = float(np.exp(-4 * nu * k * k * steps))   # KE(t)/KE(0), analytic
+    res, ref = {}, None
+
+    def ke(d):
+        r, a, b, cz = d.macro()
+        e = 0.5 * float((r * (a * a + b * b + cz * cz)).sum())
+        return e, np.stack([a, b, cz])
+
+    for storage in ("fp64", "fp32", "fp16", "bf16"):
+        d = couplb_amd.Device(3, n, n, n, tau=3 * nu + 0.5, storage=storage)
+        d.set_state(f0)
+        e0, _ = ke(d)
+        d.sync()
+        t0 = time.perf_counter()
+        d.steps_async(steps, (0.0, 0.0, 0.0), 0)
+        d.sync()
+        dt = time.perf_counter() - t0
+        e1, u = ke(d)
+        mass = d.diagnostics()["mass"]
+        d.close()
+        r = dict(mlups=round(n ** 3 * steps / dt / 1e6, 1), ke_ratio=e1 / e0,
+                 ke_ratio_rel_err_vs_analytic=(e1 / e0) / decay - 1.0, mass=mass)
+        if ref is None:
+            ref = u
+        else:
+            r["rel_l2_u_vs_fp64"] = float(np.linalg.norm(u - ref) / np.linalg.norm(ref))
+        res[storage] = r
+        del u
+    return dict(metric=f"TGV {n}^3 precision study: KE decay and rel-L2(u) vs FP64 GPU",
+                unit="MLUPS", value=res["fp16"]["mlups"], steps=steps,
+                data="synthetic (analytic initial field)",
+                config={"workload": f"D3Q19 {n}^3 periodic Taylor-Green, u0=0.03, k=2pi/{n}, nu=0.02",
+                        "analytic_ke_ratio": decay}, results=res)
+
+
+def c5_spheres(N, count=64, D=16.0, seed=2026):
+    """Non-overlapping sphere centres at least D/2 + 2 from the y walls."""
+    rng = np.random.default_rng(seed)
+    cs = []
+    while len(cs) < count:
+        c = np.array([rng.uniform(0, N), rng.uniform(D / 2 + 2, N - D / 2 - 2), rng.uniform(0, N)])
+        ok = True
+        for o in cs:
+            d = np.abs(c - o)
+            d[0] = min(d[0], N - d[0])
+            d[2] = min(d[2], N - d[2])
+            if np.linalg.norm(d) < D + 2:
+                ok = False
+                break
+        if ok:
+            cs.append(c)
+    ns = int(round(math.pi * D * D))                      # 804 markers, ~1 dx spacing
+    x = np.concatenate([fibonacci_sphere(ns, D / 2, c) % [N, 1e9, N] for c in cs])
+    return x, ns
+
+
+def workload_c5(steps, train_steps=20, N=256):
+    """Shear flow between walls moving at -/+0.04 with 64 fixed spheres of
+    diameter 16 (Peskin4 markers), multi-direct forcing to slip 1e-6 (CG).
+    Compares the solve started from zero (FP64, plain), from the previous
+    step's forces, and from the trained network (FP64 and FP16 storage)."""
+    import torch
+    import couplb_amd
+    cuda = torch.device("cuda:0")
+    walls = ((0, 0), (2, 2), (0, 0))
+    face = [[[0, 0, 0], [0, 0, 0]], [[-0.04, 0, 0], [0.04, 0, 0]], [[0, 0, 0], [0, 0, 0]]]
+    x, ns = c5_spheres(N)
+    n = len(x)
+    tx = torch.from_numpy(x).to(cuda)
+    tV = torch.zeros((n, 3), dtype=torch.float64, device=cuda)
+    tf = torch.zeros((n, 3), dtype=torch.float64, device=cuda)
+    kw = dict(lo=(0, 0, 0), inv_dx=1.0, vel_scale=1.0, force_scale=1.0, tol=1e-6, max_iter=4000,
+              method="cg")
+    # initial Couette profile between the walls
+    y = ghost_coords(N)
+    prof = -0.04 + 0.08 * y / N
+    fy = feq_field(np.ones((N + 2, 1)), np.stack([prof[:, None], np.zeros((N + 2, 1)),
+                                                  np.zeros((N + 2, 1))]))      # (19, gy, 1)
+    f0 = np.ascontiguousarray(np.broadcast_to(fy[:, None, :, :], (19, N + 2, N + 2, N + 2)))
+    del fy
+
+    def device(storage):
+        d = couplb_amd.Device(3, N, N, N, tau=0.8, walls=walls, face_vel=face,
+                              kernel=couplb_amd.PESKIN4, storage=storage)
+        return d
+
+    A = device("fp64")
+    A.set_state(f0)
+    del f0
+    # training data: the solver's converged forces over the first train_steps LB steps
+    feats = torch.zeros((train_steps * n, 67), dtype=torch.float32, device=cuda)
+    targs = torch.zeros((train_steps * n, 3), dtype=torch.float32, device=cuda)
+    t0 = time.perf_counter()
+    it_tr = []
+    for s in range(train_steps):
+        st = A.mdf_device(n, tx.data_ptr(), tV.data_ptr(), None, tf.data_ptr(),
+                          positions_unchanged=s > 0, features_ptr=feats[s * n].data_ptr(),
+                          targets_ptr=targs[s * n].data_ptr(), **kw)
+        it_tr.append(st["iterations"])
+        A.step((0.0, 0.0, 0.0), 0)
+    A.sync()
+    gen_s = time.perf_counter() - t0
+    net = couplb_amd.Mlp(seed=11)
+    t0 = time.perf_counter()
+    loss0, _ = net.gradient_device(len(feats), feats.data_ptr(), targs.data_ptr())
+    lossE = net.train_device(len(feats), feats.data_ptr(), targs.data_ptr(), batch=4096, epochs=8,
+                             lr=2e-3, seed=5)
+    loss1, _ = net.gradient_device(len(feats), feats.data_ptr(), targs.data_ptr())
+    train_s = time.perf_counter() - t0
+    start = A.get_state()
+    A.close()
+
+    def run(storage, mode):
+        d = device(storage)
+        d.set_state(start)
+        d.sync()
+        iters, F0 = [], None
+        fprev = torch.zeros_like(tf)
+        t = time.perf_counter()
+        for s in range(steps):
+            st = d.mdf_device(n, tx.data_ptr(), tV.data_ptr(),
+                              fprev.data_ptr() if (mode == "previous" and s > 0) else None,
+                              tf.data_ptr(), positions_unchanged=s > 0,
+                              warm_start=net if mode == "network" else None, **kw)
+            iters.append(st["iterations"])
+            if mode == "previous":
+                fprev.copy_(-tf)          # F (lattice) = -f_body with force_scale 1
+            d.step((0.0, 0.0, 0.0), 0)
+        d.sync()
+        wall = time.perf_counter() - t
+        r, a, b, c = d.macro()
+        d.close()
+        return dict(mlups=round(N ** 3 * steps / wall / 1e6, 1), mean_iterations=float(np.mean(iters)),
+                    iterations=iters), np.stack([a, b, c])
+
+    res = {}
+    res["fp64_plain"], u_ref = run("fp64", "zero")
+    for name, storage, mode in (("fp64_previous", "fp64", "previous"),
+                                ("fp64_network", "fp64", "network"),
+                                ("fp16_network", "fp16", "network"),
+                                ("fp16_plain", "fp16", "zero")):
+        r, u = run(storage, mode)
+        r["rel_l2_u_vs_fp64_plain"] = float(np.linalg.norm(u - u_ref) / np.linalg.norm(u_ref))
+        res[name] = r
+        del u
+    return dict(metric="C5 learned MDF warm start: MLUPS, MDF passes, rel-L2(u) vs FP64 plain",
+                unit="MLUPS", value=res["fp16_network"]["mlups"], steps=steps,
+                data="synthetic (seeded sphere centres, Couette initial field)",
+                config={"workload": f"D3Q19 {N}^3 shear flow, walls y at -/+0.04, 64 fixed spheres "
+                                    f"D=16 ({ns} markers each, {n} total), Peskin4, MDF (CG) to "
+                                    f"slip 1e-6, network 67-64-64-3 trained on {len(feats)} "
+                                    f"samples from {train_steps} solves"},
+                training=dict(samples=len(feats), generate_s=round(gen_s, 2), train_s=round(train_s, 2),
+                              loss_initial=loss0, loss_last_epoch=lossE, loss_final=loss1,
+                              train_phase_iterations=it_tr),
+                results=res)
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -139,6 +377,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="time a plain device copy of MB megabytes (PMC byte-counter scale)")
+    ap.add_argument("--workload", default="c3", choices=["c3", "c2", "c4", "c5"],
+                    help="c3 (default): the headline line; c2 channel, c4 precision study, "
+                         "c5 learned MDF warm start")
+    ap.add_argument("--grid", type=int, default=0, help="grid edge for c4 (default 384)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -151,6 +393,16 @@ def main():
 
     if args.copy_calibration:
         copy_calibration(args.copy_calibration, args.steps)
+        return
+    if args.workload != "c3":
+        if args.workload == "c2":
+            out = workload_c2(args.steps)
+        elif args.workload == "c4":
+            out = workload_c4(args.steps, args.grid or 384)
+        else:
+            out = workload_c5(args.steps)
+        out.update(n_gpus=1, higher_is_better=True, dtype="f64 / reduced storage as named")
+        print(json.dumps(out), flush=True)
         return
     N, md = args.n, 4
     log(f"bench: D3Q19 {N}^3 {args.storage} IBM md_per_lb={md} particles={args.particles}")

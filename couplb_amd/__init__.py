@@ -58,7 +58,11 @@ class MdfArgs(C.Structure):
 
 class MdfStats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("slip_first", C.c_double),
-                ("slip_last", C.c_double)]
+                ("slip_last", C.c_double), ("slip_zero", C.c_double)]
+
+
+class MdfLearn(C.Structure):
+    _fields_ = [("warm_start", C.c_void_p), ("features", C.c_void_p), ("targets", C.c_void_p)]
 
 
 class Domain(C.Structure):
@@ -113,7 +117,7 @@ def lib() -> C.CDLL:
         "couplb_kernel_ms_total": (ip, [vp, ip, dp, C.POINTER(C.c_long)]),
         "couplb_kernel_ms_reset": (ip, [vp]),
         "couplb_ibm_mdf_device": (ip, [vp, ip, vp, vp, vp, vp, C.POINTER(MdfArgs),
-                                       C.POINTER(MdfStats)]),
+                                       C.POINTER(MdfLearn), C.POINTER(MdfStats)]),
         "couplb_mlp_create": (ip, [ip, C.c_ulonglong, C.POINTER(vp)]),
         "couplb_mlp_destroy": (ip, [vp]),
         "couplb_mlp_last_error": (C.c_char_p, [vp]),
@@ -250,10 +254,12 @@ class Device:
                   "ibm_substep_device")
 
     def mdf_device(self, n, x_ptr, V_ptr, F0_ptr, fbody_ptr, lo, inv_dx, vel_scale, force_scale,
-                   tol=1e-6, relax=1.0, max_iter=200, positions_unchanged=False, method="cg"):
+                   tol=1e-6, relax=1.0, max_iter=200, positions_unchanged=False, method="cg",
+                   warm_start=None, features_ptr=None, targets_ptr=None):
         """Multi-direct-forcing solve for markers with prescribed velocity V
-        (device pointers; F0_ptr may be None); method "cg" or "richardson".
-        Returns the solver statistics."""
+        (device pointers; F0_ptr may be None); method "cg" or "richardson";
+        warm_start an Mlp; features_ptr / targets_ptr optional float32 device
+        outputs for training.  Returns the solver statistics."""
         a = MdfArgs()
         for d in range(3):
             a.ibm.lo[d] = lo[d]
@@ -263,10 +269,15 @@ class Device:
         a.tol, a.relax, a.max_iter = tol, relax, max_iter
         a.method = MDF_CG if method == "cg" else MDF_RICHARDSON
         st = MdfStats()
+        learn = None
+        if warm_start is not None or features_ptr or targets_ptr:
+            learn = MdfLearn(warm_start.h.value if warm_start is not None else None,
+                             features_ptr, targets_ptr)
         self._chk(lib().couplb_ibm_mdf_device(self.h, n, x_ptr, V_ptr, F0_ptr, fbody_ptr,
-                                              C.byref(a), C.byref(st)), "ibm_mdf_device")
+                                              C.byref(a), C.byref(learn) if learn else None,
+                                              C.byref(st)), "ibm_mdf_device")
         return dict(iterations=st.iterations, converged=bool(st.converged),
-                    slip_first=st.slip_first, slip_last=st.slip_last)
+                    slip_first=st.slip_first, slip_last=st.slip_last, slip_zero=st.slip_zero)
 
     def refresh(self):
         self._chk(lib().couplb_refresh_macro(self.h), "refresh_macro")

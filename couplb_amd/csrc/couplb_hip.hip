@@ -15,6 +15,7 @@
 #include "ibm.hpp"
 #include "kernels.hpp"
 #include "mdf.hpp"
+#include "mlp.hpp"
 #include "spread_pairs.hpp"
 
 using namespace couplb;
@@ -102,6 +103,8 @@ struct couplb_ctx {
   int mdf_cap = 0;
   MdfScalars* mdf_S = nullptr;
   MdfScalars* mdf_host = nullptr;  // pinned
+  float *mlp_x = nullptr, *mlp_y = nullptr;  // network inputs / outputs, sorted order
+  double* mlp_sig = nullptr;                 // |slip at F = 0| per sorted marker
   int ugather_cycle = -1; // state counter of the last velocity gather
   std::vector<SlotHost> slots;
   int nslots = 0;
@@ -521,15 +524,20 @@ int build_pairs(couplb_ctx* c, const SlotHost& sl) {
 }
 
 int mdf_impl(couplb_ctx* c, int n, const double* x, const double* V, const double* F0,
-             double* f_body, const couplb_mdf_args* a, couplb_mdf_stats* st) {
+             double* f_body, const couplb_mdf_args* a, const couplb_mdf_learn* learn,
+             couplb_mdf_stats* st) {
   if (n < 0 || !a || !x || !V || !f_body) return fail(c, COUPLB_EINVAL, "bad MDF arguments");
   if (!(a->tol > 0) || a->max_iter < 1 || !(a->relax > 0) ||
       (a->method != COUPLB_MDF_RICHARDSON && a->method != COUPLB_MDF_CG))
     return fail(c, COUPLB_EINVAL, "MDF needs tol > 0, max_iter >= 1, relax > 0 and a known method");
   if (c->nslots != 0)
     return fail(c, COUPLB_ESTATE, "MDF must be the only IBM call between two LB steps");
-  if (st) *st = couplb_mdf_stats{0, 0, 0.0, 0.0};
+  if (st) *st = couplb_mdf_stats{0, 0, 0.0, 0.0, -1.0};
   if (n == 0) return COUPLB_OK;
+  const bool use_net = learn && learn->warm_start;
+  const bool want_x = learn && (learn->warm_start || learn->features || learn->targets);
+  if (use_net && learn->warm_start->device != c->p.device)
+    return fail(c, COUPLB_EINVAL, "the warm-start network lives on another device");
   const int S = (c->D == 3) ? stencil_size<3>() : stencil_size<2>();
   if ((long)n * S >= (long)INT32_MAX) return fail(c, COUPLB_EINVAL, "too many markers for MDF");
   int rc = ensure_ibm(c, n);
@@ -552,6 +560,12 @@ int mdf_impl(couplb_ctx* c, int n, const double* x, const double* V, const doubl
     HIPCHK(c, hipMalloc(&c->mdf_R, sizeof(double) * 3 * cap));
     HIPCHK(c, hipMalloc(&c->mdf_Ap, sizeof(double) * 3 * cap));
     HIPCHK(c, hipMalloc(&c->mdf_part, sizeof(double) * 2 * nbmax));
+    (void)hipFree(c->mlp_x); (void)hipFree(c->mlp_y); (void)hipFree(c->mlp_sig);
+    c->mlp_x = c->mlp_y = nullptr;
+    c->mlp_sig = nullptr;
+    HIPCHK(c, hipMalloc(&c->mlp_x, sizeof(float) * (size_t)MLP_IN * cap));
+    HIPCHK(c, hipMalloc(&c->mlp_y, sizeof(float) * 3 * (size_t)cap));
+    HIPCHK(c, hipMalloc(&c->mlp_sig, sizeof(double) * (size_t)cap));
     c->mdf_cap = cap;
   }
   double* part_a = c->mdf_part;
@@ -566,11 +580,12 @@ int mdf_impl(couplb_ctx* c, int n, const double* x, const double* V, const doubl
   SlotHost& sl = c->slots[0];
   MdfScalars* Sd = c->mdf_S;
   hipLaunchKernelGGL(k_mdf_reset, dim3(1), dim3(1), 0, c->stream, Sd, 1);
+  const double* F0i = want_x ? nullptr : F0;  // the network path starts from F = 0
   if (c->D == 3)
-    hipLaunchKernelGGL(k_mdf_init<3>, dim3(nb), dim3(tb), 0, c->stream, n, c->perm, x, V, F0, ip, c->geo,
+    hipLaunchKernelGGL(k_mdf_init<3>, dim3(nb), dim3(tb), 0, c->stream, n, c->perm, x, V, F0i, ip, c->geo,
                        c->walls, sl.rec, c->mdf_v, c->mdf_F);
   else
-    hipLaunchKernelGGL(k_mdf_init<2>, dim3(nb), dim3(tb), 0, c->stream, n, c->perm, x, V, F0, ip, c->geo,
+    hipLaunchKernelGGL(k_mdf_init<2>, dim3(nb), dim3(tb), 0, c->stream, n, c->perm, x, V, F0i, ip, c->geo,
                        c->walls, sl.rec, c->mdf_v, c->mdf_F);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipMemcpyAsync(sl.rowstart, c->rowstart_cur, sizeof(int) * (c->geo.nrows + 1),
@@ -618,6 +633,32 @@ int mdf_impl(couplb_ctx* c, int n, const double* x, const double* V, const doubl
   constexpr int CHUNK = 4;  // iterations queued between two host checks
   rc = true_residual(1);
   if (rc) return rc;
+  double slip_zero = -1.0;
+  if (want_x) {
+    slip_zero = c->mdf_host->smax;
+    if (c->D == 3)
+      hipLaunchKernelGGL(k_mlp_features<3>, dim3(nb), dim3(tb), 0, c->stream, n, c->geo, c->walls,
+                         (const MarkerRec*)sl.rec, (const double*)c->u4, (const double*)c->mdf_v,
+                         (const double*)c->mdf_R, c->mlp_x, c->mlp_sig);
+    else
+      hipLaunchKernelGGL(k_mlp_features<2>, dim3(nb), dim3(tb), 0, c->stream, n, c->geo, c->walls,
+                         (const MarkerRec*)sl.rec, (const double*)c->u4, (const double*)c->mdf_v,
+                         (const double*)c->mdf_R, c->mlp_x, c->mlp_sig);
+    HIPCHK(c, hipGetLastError());
+    if (learn->features) {
+      const long ne = (long)n * MLP_IN;
+      hipLaunchKernelGGL(k_mlp_unsort, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, c->stream, n,
+                         (const int*)c->perm, (const float*)c->mlp_x, learn->features);
+    }
+    if (use_net && c->mdf_host->done == 0) {
+      couplb::mlp::forward(n, c->mlp_x, learn->warm_start->params, c->mlp_y, c->stream);
+      hipLaunchKernelGGL(k_mdf_warm, dim3(nb), dim3(tb), 0, c->stream, n, (const float*)c->mlp_y,
+                         (const double*)c->mlp_sig, c->mdf_F, sl.rec);
+      HIPCHK(c, hipGetLastError());
+      rc = true_residual(1);  // slip at the prediction: where the iteration starts
+      if (rc) return rc;
+    }
+  }
   int method = a->method;
   while (c->mdf_host->done == 0) {
     if (method == COUPLB_MDF_CG) {
@@ -660,7 +701,11 @@ int mdf_impl(couplb_ctx* c, int n, const double* x, const double* V, const doubl
     }
   }
   const MdfScalars& hs = *c->mdf_host;
-  couplb_mdf_stats out{hs.iters, hs.done == 1 ? 1 : 0, hs.smax_first, hs.smax};
+  couplb_mdf_stats out{hs.iters, hs.done == 1 ? 1 : 0, hs.smax_first, hs.smax, slip_zero};
+  if (learn && learn->targets)
+    hipLaunchKernelGGL(k_mdf_targets, dim3(nb), dim3(tb), 0, c->stream, n, (const int*)c->perm,
+                       (const double*)c->mdf_F, (const double*)c->mlp_sig, (const MarkerRec*)sl.rec,
+                       learn->targets);
   hipLaunchKernelGGL(k_mdf_setf, dim3(nb), dim3(tb), 0, c->stream, n, (const double*)c->mdf_F, sl.rec);
   hipLaunchKernelGGL(k_mdf_out, dim3(nb), dim3(tb), 0, c->stream, n, c->perm, (const double*)c->mdf_F,
                      (const MarkerRec*)sl.rec, a->ibm.force_scale, f_body);
@@ -885,6 +930,7 @@ int couplb_destroy(couplb_ctx* c) {
   (void)hipFree(c->pp); (void)hipFree(c->pw); (void)hipFree(c->ptmp);
   (void)hipFree(c->ut); (void)hipFree(c->mdf_v); (void)hipFree(c->mdf_S);
   (void)hipFree(c->mdf_F); (void)hipFree(c->mdf_R); (void)hipFree(c->mdf_Ap); (void)hipFree(c->mdf_part);
+  (void)hipFree(c->mlp_x); (void)hipFree(c->mlp_y); (void)hipFree(c->mlp_sig);
   if (c->mdf_host) (void)hipHostFree(c->mdf_host);
   for (auto& s : c->slots) { (void)hipFree(s.rowstart); (void)hipFree(s.rec); }
   (void)hipFree(c->slot_table);
@@ -988,9 +1034,10 @@ int couplb_ibm_substep_device(couplb_ctx* c, int n, const double* x, const doubl
 }
 
 int couplb_ibm_mdf_device(couplb_ctx* c, int n, const double* x, const double* V, const double* F0,
-                          double* f_body, const couplb_mdf_args* a, couplb_mdf_stats* st) {
+                          double* f_body, const couplb_mdf_args* a, const couplb_mdf_learn* learn,
+                          couplb_mdf_stats* st) {
   if (!c) return COUPLB_EINVAL;
-  return mdf_impl(c, n, x, V, F0, f_body, a, st);
+  return mdf_impl(c, n, x, V, F0, f_body, a, learn, st);
 }
 
 int couplb_ibm_substep(couplb_ctx* c, int n, const double* x, const double* v, const double* m,

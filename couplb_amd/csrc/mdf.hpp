@@ -342,3 +342,102 @@ __global__ void __launch_bounds__(256) k_mdf_out(int n, const int* perm, const d
 }
 
 }  // namespace couplb
+
+namespace couplb {
+
+// Inputs of the warm-start network for every marker (sorted order), from the
+// slip at F = 0, s = V - I(u*) (R), sigma = |s|:
+//   x[l]      = ((V - u_l) . s) / sigma^2   for the 64 stencil slots
+//               l = t + 4u + 16v -> node (i0-1+t, j0-1+u, k0-1+v); u_l is u*
+//               at fluid nodes, the wall velocity at moving-wall nodes, 0 at
+//               other wall nodes; slots beyond the ghost layer (and the 48
+//               unused slots in 2D) are 0
+//   x[64..66] = s / sigma
+// Markers that are not valid or have sigma < 1e-14 get zero inputs.
+constexpr int MLP_IN = 67;
+constexpr double MLP_FSCALE = 16.0;  // network output = F / (16 sigma)
+
+template <int D>
+__global__ void __launch_bounds__(256) k_mlp_features(int n, Geo g, Walls w, const MarkerRec* rec,
+                                                      const double* u4, const double* Vs,
+                                                      const double* R, float* X, double* sig) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  float* xr = X + (size_t)p * MLP_IN;
+  const MarkerRec& r = rec[p];
+  const double s0 = R[3 * (size_t)p + 0], s1 = R[3 * (size_t)p + 1], s2 = R[3 * (size_t)p + 2];
+  const double ss = s0 * s0 + s1 * s1 + s2 * s2;
+  const double sg = sqrt(ss);
+  sig[p] = sg;
+  if (!r.valid || !(sg >= 1e-14)) {
+    for (int l = 0; l < MLP_IN; l++) xr[l] = 0.0f;
+    return;
+  }
+  const double v0 = Vs[3 * (size_t)p + 0], v1 = Vs[3 * (size_t)p + 1], v2 = Vs[3 * (size_t)p + 2];
+  const double inv = 1.0 / ss;
+  for (int l = 0; l < 64; l++) {
+    const int t = l & 3, u = (l >> 2) & 3, vv = l >> 4;
+    float val = 0.0f;
+    const int a = r.i0 - 1 + t, b = r.j0 - 1 + u, c = (D == 3) ? r.k0 - 1 + vv : 0;
+    const bool in = (D == 3 || vv == 0) && a >= 0 && a <= g.nx + 1 && b >= 0 && b <= g.ny + 1 &&
+                    (D == 2 || (c >= 0 && c <= g.nz + 1));
+    if (in) {
+      int wd, wsd;
+      const int nt = node_type(w, g, a, b, c, wd, wsd);
+      double u0 = 0.0, u1 = 0.0, u2 = 0.0;
+      if (nt == 0) {
+        const unsigned node = (unsigned)wrap_ghost(a, g.nx) +
+                              (unsigned)g.nx * ((unsigned)wrap_ghost(b, g.ny) +
+                                                (unsigned)g.ny * (unsigned)((D == 3) ? wrap_ghost(c, g.nz) : 0));
+        const double4 q = *reinterpret_cast<const double4*>(u4 + 4 * (size_t)node);
+        u0 = q.x; u1 = q.y; u2 = q.z;
+      } else if (nt == 2) {
+        u0 = w.vel[wd][wsd][0]; u1 = w.vel[wd][wsd][1]; u2 = (D == 3) ? w.vel[wd][wsd][2] : 0.0;
+      }
+      val = (float)(((v0 - u0) * s0 + (v1 - u1) * s1 + (v2 - u2) * s2) * inv);
+    }
+    xr[l] = val;
+  }
+  xr[64] = (float)(s0 / sg);
+  xr[65] = (float)(s1 / sg);
+  xr[66] = (float)(s2 / sg);
+}
+
+// F0 = 16 sigma y from the network's outputs (sorted order) into Fm and rec.f
+__global__ void __launch_bounds__(256) k_mdf_warm(int n, const float* Y, const double* sig,
+                                                  double* Fm, MarkerRec* rec) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  MarkerRec& r = rec[p];
+  const bool ok = r.valid && sig[p] >= 1e-14;
+  const double k = MLP_FSCALE * sig[p];
+  double f[3];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    f[c] = ok ? k * (double)Y[3 * (size_t)p + c] : 0.0;
+    Fm[3 * (size_t)p + c] = f[c];
+  }
+  r.fx = f[0]; r.fy = f[1]; r.fz = f[2];
+}
+
+// network inputs (sorted) -> caller order
+__global__ void __launch_bounds__(256) k_mlp_unsort(int n, const int* perm, const float* Xs, float* X) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)n * MLP_IN) return;
+  const int p = (int)(e / MLP_IN), l = (int)(e % MLP_IN);
+  X[(size_t)perm[p] * MLP_IN + l] = Xs[e];
+}
+
+// training targets F / (16 sigma), caller order (0 where sigma is tiny)
+__global__ void __launch_bounds__(256) k_mdf_targets(int n, const int* perm, const double* Fm,
+                                                     const double* sig, const MarkerRec* rec,
+                                                     float* T) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const bool ok = rec[p].valid && sig[p] >= 1e-14;
+  const double k = ok ? 1.0 / (MLP_FSCALE * sig[p]) : 0.0;
+#pragma unroll
+  for (int c = 0; c < 3; c++) T[3 * (size_t)perm[p] + c] = (float)(Fm[3 * (size_t)p + c] * k);
+}
+
+}  // namespace couplb

@@ -159,14 +159,33 @@ typedef struct couplb_mdf_args {
   int method;           /* COUPLB_MDF_RICHARDSON or COUPLB_MDF_CG */
 } couplb_mdf_args;
 typedef struct couplb_mdf_stats {
-  int iterations;       /* interpolations performed */
+  int iterations;       /* interpolation passes, including the one at F = 0
+                           when the network is used */
   int converged;        /* 1 when max|slip| < tol was reached */
-  double slip_first;    /* max|slip| at the initial force */
-  double slip_last;     /* max|slip| before the last update */
+  double slip_first;    /* max|slip| at the force the iteration starts from */
+  double slip_last;     /* max|slip| of the returned force */
+  double slip_zero;     /* max|slip| at F = 0 when evaluated, else -1 */
 } couplb_mdf_stats;
+
+/* Learned warm start (include/couplb_mlp.h).  The network's inputs for
+ * marker k come from the slip at F = 0, s = V/vel_scale - I(u*), and the
+ * velocities of its 4x4x4 stencil: 64 values ((V - u_l).s)/|s|^2 in stencil
+ * order t + 4u + 16v (u_l = u* at fluid nodes, the wall velocity at moving
+ * walls, 0 at other walls and outside the ghost layer) and s/|s|; its 3
+ * outputs are F/(16 |s|).  warm_start: start the iteration from the
+ * network's prediction (instead of F0); features / targets (device, caller
+ * order, optional): the inputs of this solve and the converged forces
+ * scaled as outputs, for training. */
+struct couplb_mlp;
+typedef struct couplb_mdf_learn {
+  struct couplb_mlp* warm_start;  /* may be NULL */
+  float* features;                /* [n][67] or NULL */
+  float* targets;                 /* [n][3] or NULL */
+} couplb_mdf_learn;
+
 int couplb_ibm_mdf_device(couplb_ctx* ctx, int n, const double* x, const double* V,
                           const double* F0, double* f_body, const couplb_mdf_args* a,
-                          couplb_mdf_stats* stats);
+                          const couplb_mdf_learn* learn, couplb_mdf_stats* stats);
 
 /* compute_macroscopic(false) semantics without copying anything out: marks
  * the density as refreshed for the next step's gravity (fix_couplb.cpp:706,

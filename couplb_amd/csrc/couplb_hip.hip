@@ -9,6 +9,8 @@
 
 #include <hip/hip_runtime.h>
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_select.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
 
 #include "../../include/couplb_hip.h"
 #include "gather.hpp"
@@ -92,8 +94,16 @@ struct couplb_ctx {
   double* pw = nullptr;
   void* ptmp = nullptr;
   size_t ptmp_bytes = 0;
-  long pcap = 0, pne = 0;
+  long pcap = 0, pne = 0, pcap_runs = 0;
   int pair_epoch = -1;       // ordering the pairs (and the node flags) were built for
+  int *pheads = nullptr, *phlen = nullptr;  // node runs of the sorted pairs
+  unsigned* phnode = nullptr;
+  unsigned char* phflag = nullptr;
+  int* pcount = nullptr;           // device count of runs
+  int pnh = 0;                     // number of runs
+  void* seltmp = nullptr;
+  size_t seltmp_bytes = 0;
+  double* fsrc = nullptr;          // record slot 0's force per sorted marker (3*cap)
   int prev_step_epoch = -1;  // ordering of the single record slot at the previous LB step
   // multi-direct forcing (mdf.hpp)
   double* ut = nullptr;            // operator node values at pair nodes, 3 per node
@@ -337,6 +347,9 @@ int ensure_ibm(couplb_ctx* c, int n) {
   HIPCHK(c, hipMalloc(&c->dm, sizeof(double) * cap));
   HIPCHK(c, hipMalloc(&c->df, sizeof(double) * 3 * cap));
   HIPCHK(c, hipMalloc(&c->ufc, sizeof(double) * 3 * cap));
+  (void)hipFree(c->fsrc);
+  c->fsrc = nullptr;
+  HIPCHK(c, hipMalloc(&c->fsrc, sizeof(double) * 3 * cap));
   (void)hipFree(c->mdf_v);
   c->mdf_v = nullptr;
   HIPCHK(c, hipMalloc(&c->mdf_v, sizeof(double) * 3 * cap));
@@ -439,7 +452,7 @@ int ibm_impl(couplb_ctx* c, int n, const double* x, const double* v, const doubl
   if (merge) {
     SlotHost& sl = c->slots[c->nslots - 1];
     hipLaunchKernelGGL(k_marker_force, dim3(nb), dim3(tb), 0, c->stream, n, c->perm, v, m, f_dev, fmode,
-                       ip, (const double*)c->ufc, sl.rec);
+                       ip, (const double*)c->ufc, sl.rec, c->nslots == 1 ? c->fsrc : nullptr);
     HIPCHK(c, hipGetLastError());
     c->rho_fresh = 1;  // compute_macroscopic(false) at fix_couplb.cpp:758
     return timer_end(c, tslot);
@@ -452,10 +465,10 @@ int ibm_impl(couplb_ctx* c, int n, const double* x, const double* v, const doubl
   if (c->D == 3) {
     const int nb4 = (4 * n + tb - 1) / tb;  // four lanes per marker
     hipLaunchKernelGGL((k_marker<3>), dim3(nb4), dim3(tb), 0, c->stream, n, c->perm, x, v, m, f_dev,
-                       fmode, ip, c->geo, c->walls, c->u4, sl.rec, c->ufc);
+                       fmode, ip, c->geo, c->walls, c->u4, sl.rec, c->ufc, s == 0 ? c->fsrc : nullptr);
   } else {
     hipLaunchKernelGGL((k_marker<2>), dim3(nb), dim3(tb), 0, c->stream, n, c->perm, x, v, m, f_dev,
-                       fmode, ip, c->geo, c->walls, c->u4, sl.rec, c->ufc);
+                       fmode, ip, c->geo, c->walls, c->u4, sl.rec, c->ufc, s == 0 ? c->fsrc : nullptr);
   }
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipMemcpyAsync(sl.rowstart, c->rowstart_cur, sizeof(int) * (c->geo.nrows + 1),
@@ -518,6 +531,38 @@ int build_pairs(couplb_ctx* c, const SlotHost& sl) {
     hipLaunchKernelGGL(k_pair_weights<2>, dim3(nb), dim3(tb), 0, c->stream, ne, c->pkey_s, c->pval_s,
                        sentinel, sl.rec, c->pp, c->pw, c->flag);
   HIPCHK(c, hipGetLastError());
+  // node runs: first pair of each run, its length and node
+  if (ne > c->pcap_runs) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    (void)hipFree(c->pheads); (void)hipFree(c->phlen); (void)hipFree(c->phnode); (void)hipFree(c->phflag);
+    (void)hipFree(c->seltmp);
+    c->pheads = c->phlen = nullptr; c->phnode = nullptr; c->phflag = nullptr; c->seltmp = nullptr;
+    c->pcap_runs = 0;
+    HIPCHK(c, hipMalloc(&c->pheads, sizeof(int) * ne));
+    HIPCHK(c, hipMalloc(&c->phlen, sizeof(int) * ne));
+    HIPCHK(c, hipMalloc(&c->phnode, sizeof(unsigned) * ne));
+    HIPCHK(c, hipMalloc(&c->phflag, ne));
+    if (!c->pcount) HIPCHK(c, hipMalloc(&c->pcount, sizeof(int)));
+    size_t bytes = 0;
+    HIPCHK(c, rocprim::select(nullptr, bytes, rocprim::counting_iterator<int>(0), c->phflag, c->pheads,
+                              c->pcount, (size_t)ne, c->stream));
+    HIPCHK(c, hipMalloc(&c->seltmp, bytes));
+    c->seltmp_bytes = bytes;
+    c->pcap_runs = ne;
+  }
+  hipLaunchKernelGGL(k_pair_heads, dim3(nb), dim3(tb), 0, c->stream, ne, (const unsigned*)c->pkey_s, sentinel,
+                     c->phflag);
+  size_t sbytes = c->seltmp_bytes;
+  HIPCHK(c, rocprim::select(c->seltmp, sbytes, rocprim::counting_iterator<int>(0), c->phflag, c->pheads,
+                            c->pcount, (size_t)ne, c->stream));
+  int nh = 0;
+  HIPCHK(c, hipMemcpyAsync(&nh, c->pcount, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (nh > 0)
+    hipLaunchKernelGGL(k_head_info, dim3((nh + tb - 1) / tb), dim3(tb), 0, c->stream, nh,
+                       (const int*)c->pheads, (const unsigned*)c->pkey_s, ne, c->phlen, c->phnode);
+  HIPCHK(c, hipGetLastError());
+  c->pnh = nh;
   c->pne = ne;
   c->pair_epoch = sl.epoch;
   return COUPLB_OK;
@@ -583,10 +628,10 @@ int mdf_impl(couplb_ctx* c, int n, const double* x, const double* V, const doubl
   const double* F0i = want_x ? nullptr : F0;  // the network path starts from F = 0
   if (c->D == 3)
     hipLaunchKernelGGL(k_mdf_init<3>, dim3(nb), dim3(tb), 0, c->stream, n, c->perm, x, V, F0i, ip, c->geo,
-                       c->walls, sl.rec, c->mdf_v, c->mdf_F);
+                       c->walls, sl.rec, c->mdf_v, c->mdf_F, c->fsrc);
   else
     hipLaunchKernelGGL(k_mdf_init<2>, dim3(nb), dim3(tb), 0, c->stream, n, c->perm, x, V, F0i, ip, c->geo,
-                       c->walls, sl.rec, c->mdf_v, c->mdf_F);
+                       c->walls, sl.rec, c->mdf_v, c->mdf_F, c->fsrc);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipMemcpyAsync(sl.rowstart, c->rowstart_cur, sizeof(int) * (c->geo.nrows + 1),
                            hipMemcpyDeviceToDevice, c->stream));
@@ -596,41 +641,43 @@ int mdf_impl(couplb_ctx* c, int n, const double* x, const double* V, const doubl
     rc = build_pairs(c, sl);
     if (rc) return rc;
   }
-  const unsigned nbp = (unsigned)std::max<long>(1, (c->pne + 255) / 256);
-  // node values ut = [u*] + S(rec.f)/(2 rho), then the interpolation pass
+  const unsigned nbp = (unsigned)std::max(1, (c->pnh + 255) / 256);
+  // node values ut = [u*] + S(P)/(2 rho), then the interpolation pass
   auto apply = [&](int affine) {
-    hipLaunchKernelGGL(k_mdf_ut, dim3(nbp), dim3(256), 0, c->stream, c->pne, (const unsigned*)c->pkey_s,
-                       (const int*)c->pp, (const double*)c->pw, (unsigned)c->N, (const MarkerRec*)sl.rec,
-                       (const double*)c->u4, affine, c->ut, (const MdfScalars*)Sd);
+    hipLaunchKernelGGL(k_mdf_ut, dim3(nbp), dim3(256), 0, c->stream, c->pnh, (const int*)c->pheads,
+                       (const int*)c->phlen, (const unsigned*)c->phnode, (const int*)c->pp,
+                       (const double*)c->pw, (const double*)c->fsrc, (const double*)c->u4, affine, c->ut,
+                       (const MdfScalars*)Sd);
     double* out = affine ? c->mdf_R : c->mdf_Ap;
     if (c->D == 3)
       hipLaunchKernelGGL(k_mdf_interp<3>, dim3(nbi), dim3(MDF_THREADS), 0, c->stream, n, c->geo, c->walls,
                          (const double*)c->ut, affine, (const MarkerRec*)sl.rec, (const double*)c->mdf_v,
-                         out, part_a, part_b, (const MdfScalars*)Sd);
+                         (const double*)c->fsrc, out, part_a, part_b, (const MdfScalars*)Sd);
     else
       hipLaunchKernelGGL(k_mdf_interp<2>, dim3(nbi), dim3(MDF_THREADS), 0, c->stream, n, c->geo, c->walls,
                          (const double*)c->ut, affine, (const MarkerRec*)sl.rec, (const double*)c->mdf_v,
-                         out, part_a, part_b, (const MdfScalars*)Sd);
+                         (const double*)c->fsrc, out, part_a, part_b, (const MdfScalars*)Sd);
   };
   auto scalar = [&](int phase, int nblk, int first, int respect) {
     hipLaunchKernelGGL(k_mdf_scalar, dim3(1), dim3(MDF_THREADS), 0, c->stream, (const double*)part_a,
-                       (const double*)part_b, nblk, phase, a->tol, first, a->max_iter, respect, Sd);
+                       (const double*)part_b, nblk, phase, a->tol, first, a->max_iter, respect, Sd,
+                       c->mdf_host);
   };
   auto read = [&]() -> int {
     HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipMemcpyAsync(c->mdf_host, Sd, sizeof(MdfScalars), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));  // k_mdf_scalar wrote the pinned mirror
     return COUPLB_OK;
   };
   // true residual of F (rec.f = F): sets done = 1 when converged
   auto true_residual = [&](int first) -> int {
-    hipLaunchKernelGGL(k_mdf_setf, dim3(nb), dim3(tb), 0, c->stream, n, (const double*)c->mdf_F, sl.rec);
+    hipLaunchKernelGGL(k_mdf_setf, dim3(nb), dim3(tb), 0, c->stream, n, (const double*)c->mdf_F, sl.rec,
+                       c->fsrc);
     hipLaunchKernelGGL(k_mdf_reset, dim3(1), dim3(1), 0, c->stream, Sd, 0);
     apply(1);
     scalar(MDF_RESID, nbi, first, 0);
     return read();
   };
-  constexpr int CHUNK = 4;  // iterations queued between two host checks
+  constexpr int CHUNK = 8;  // iterations queued between two host checks
   rc = true_residual(1);
   if (rc) return rc;
   double slip_zero = -1.0;
@@ -653,7 +700,7 @@ int mdf_impl(couplb_ctx* c, int n, const double* x, const double* V, const doubl
     if (use_net && c->mdf_host->done == 0) {
       couplb::mlp::forward(n, c->mlp_x, learn->warm_start->params, c->mlp_y, c->stream);
       hipLaunchKernelGGL(k_mdf_warm, dim3(nb), dim3(tb), 0, c->stream, n, (const float*)c->mlp_y,
-                         (const double*)c->mlp_sig, c->mdf_F, sl.rec);
+                         (const double*)c->mlp_sig, c->mdf_F, sl.rec, c->fsrc);
       HIPCHK(c, hipGetLastError());
       rc = true_residual(1);  // slip at the prediction: where the iteration starts
       if (rc) return rc;
@@ -663,17 +710,17 @@ int mdf_impl(couplb_ctx* c, int n, const double* x, const double* V, const doubl
   while (c->mdf_host->done == 0) {
     if (method == COUPLB_MDF_CG) {
       hipLaunchKernelGGL(k_cg_direction, dim3(nb), dim3(tb), 0, c->stream, n, (const double*)c->mdf_R,
-                         sl.rec, (const MdfScalars*)Sd, 1);
+                         (const MarkerRec*)sl.rec, c->fsrc, (const MdfScalars*)Sd, 1);
       do {
         for (int k = 0; k < CHUNK; k++) {
           apply(0);
           scalar(MDF_ALPHA, nbi, 0, 1);
           hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(MDF_THREADS), 0, c->stream, n,
-                             (const MarkerRec*)sl.rec, (const double*)c->mdf_Ap, c->mdf_F, c->mdf_R, part_a,
-                             part_b, (const MdfScalars*)Sd);
+                             (const MarkerRec*)sl.rec, (const double*)c->fsrc, (const double*)c->mdf_Ap,
+                             c->mdf_F, c->mdf_R, part_a, part_b, (const MdfScalars*)Sd);
           scalar(MDF_BETA, nb, 0, 1);
           hipLaunchKernelGGL(k_cg_direction, dim3(nb), dim3(tb), 0, c->stream, n, (const double*)c->mdf_R,
-                             sl.rec, (const MdfScalars*)Sd, 0);
+                             (const MarkerRec*)sl.rec, c->fsrc, (const MdfScalars*)Sd, 0);
         }
         rc = read();
         if (rc) return rc;
@@ -691,7 +738,7 @@ int mdf_impl(couplb_ctx* c, int n, const double* x, const double* V, const doubl
       do {
         for (int k = 0; k < CHUNK; k++) {
           hipLaunchKernelGGL(k_mdf_relax, dim3(nb), dim3(tb), 0, c->stream, n, (const double*)c->mdf_R,
-                             a->relax, c->mdf_F, sl.rec, (const MdfScalars*)Sd);
+                             a->relax, c->mdf_F, (const MarkerRec*)sl.rec, c->fsrc, (const MdfScalars*)Sd);
           apply(1);
           scalar(MDF_RESID, nbi, 0, 1);
         }
@@ -706,7 +753,8 @@ int mdf_impl(couplb_ctx* c, int n, const double* x, const double* V, const doubl
     hipLaunchKernelGGL(k_mdf_targets, dim3(nb), dim3(tb), 0, c->stream, n, (const int*)c->perm,
                        (const double*)c->mdf_F, (const double*)c->mlp_sig, (const MarkerRec*)sl.rec,
                        learn->targets);
-  hipLaunchKernelGGL(k_mdf_setf, dim3(nb), dim3(tb), 0, c->stream, n, (const double*)c->mdf_F, sl.rec);
+  hipLaunchKernelGGL(k_mdf_setf, dim3(nb), dim3(tb), 0, c->stream, n, (const double*)c->mdf_F, sl.rec,
+                     c->fsrc);
   hipLaunchKernelGGL(k_mdf_out, dim3(nb), dim3(tb), 0, c->stream, n, c->perm, (const double*)c->mdf_F,
                      (const MarkerRec*)sl.rec, a->ibm.force_scale, f_body);
   HIPCHK(c, hipGetLastError());
@@ -749,9 +797,11 @@ int step_impl(couplb_ctx* c, const double g[3], int flags, bool timed) {
       const int rc = build_pairs(c, c->slots[0]);
       if (rc) return rc;
     }
-    hipLaunchKernelGGL(k_pair_spread, dim3((unsigned)((c->pne + 255) / 256)), dim3(256), 0, c->stream,
-                       c->pne, (const unsigned*)c->pkey_s, (const int*)c->pp, (const double*)c->pw,
-                       (unsigned)c->N, (const MarkerRec*)c->slots[0].rec, c->Fx, c->Fy, c->Fz);
+    if (c->pnh > 0)
+      hipLaunchKernelGGL(k_pair_spread, dim3((unsigned)((c->pnh + 255) / 256)), dim3(256), 0, c->stream,
+                         c->pnh, (const int*)c->pheads, (const int*)c->phlen, (const unsigned*)c->phnode,
+                         (const int*)c->pp, (const double*)c->pw, (const double*)c->fsrc, c->Fx, c->Fy,
+                         c->Fz);
     HIPCHK(c, hipGetLastError());
   } else if (c->nslots > 0) {
     // spread the pending marker records into the per-node force (k_spread)
@@ -928,6 +978,8 @@ int couplb_destroy(couplb_ctx* c) {
   (void)hipFree(c->staging);
   (void)hipFree(c->pkey); (void)hipFree(c->pkey_s); (void)hipFree(c->pval); (void)hipFree(c->pval_s);
   (void)hipFree(c->pp); (void)hipFree(c->pw); (void)hipFree(c->ptmp);
+  (void)hipFree(c->pheads); (void)hipFree(c->phlen); (void)hipFree(c->phnode); (void)hipFree(c->phflag);
+  (void)hipFree(c->pcount); (void)hipFree(c->seltmp); (void)hipFree(c->fsrc);
   (void)hipFree(c->ut); (void)hipFree(c->mdf_v); (void)hipFree(c->mdf_S);
   (void)hipFree(c->mdf_F); (void)hipFree(c->mdf_R); (void)hipFree(c->mdf_Ap); (void)hipFree(c->mdf_part);
   (void)hipFree(c->mlp_x); (void)hipFree(c->mlp_y); (void)hipFree(c->mlp_sig);

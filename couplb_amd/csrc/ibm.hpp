@@ -296,7 +296,8 @@ template <int D>
 __global__ void __launch_bounds__(256) k_marker(int n, const int* perm, const double* x,
                                                 const double* v, const double* m, double* f_atom,
                                                 int fmode, IbmParams ip, Geo g, Walls w,
-                                                const double* u4, MarkerRec* rec, double* ufc) {
+                                                const double* u4, MarkerRec* rec, double* ufc,
+                                                double* fsrc) {
 #pragma clang fp contract(off)
   constexpr int LPM = (D == 3) ? 4 : 1;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -358,6 +359,11 @@ __global__ void __launch_bounds__(256) k_marker(int n, const int* perm, const do
   r.valid = (fin && !(wsf < DELTA_TOL)) ? 1 : 0;
   r.sc = r.valid ? 1.0 / wsf : 0.0;
   rec[p] = r;
+  if (fsrc) {  // compact copy of the reaction for the fixed-marker map
+    fsrc[3 * (size_t)p + 0] = r.fx;
+    fsrc[3 * (size_t)p + 1] = r.fy;
+    fsrc[3 * (size_t)p + 2] = r.fz;
+  }
 }
 
 // A later sub-step with unchanged positions in the same LB state: the
@@ -370,7 +376,7 @@ __global__ void __launch_bounds__(256) k_marker(int n, const int* perm, const do
 __global__ void __launch_bounds__(256) k_marker_force(int n, const int* perm, const double* v,
                                                       const double* m, double* f_atom, int fmode,
                                                       IbmParams ip, const double* ufc,
-                                                      MarkerRec* rec) {
+                                                      MarkerRec* rec, double* fsrc) {
 #pragma clang fp contract(off)
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
@@ -388,6 +394,11 @@ __global__ void __launch_bounds__(256) k_marker_force(int n, const int* perm, co
   r.fx = r.fx + -Fx * ip.inv_N / ip.force_scale;
   r.fy = r.fy + -Fy * ip.inv_N / ip.force_scale;
   r.fz = r.fz + -Fz * ip.inv_N / ip.force_scale;
+  if (fsrc) {
+    fsrc[3 * (size_t)p + 0] = r.fx;
+    fsrc[3 * (size_t)p + 1] = r.fy;
+    fsrc[3 * (size_t)p + 2] = r.fz;
+  }
 }
 
 }  // namespace couplb

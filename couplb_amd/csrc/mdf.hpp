@@ -27,6 +27,7 @@
 // order: results do not depend on scheduling.
 #pragma once
 #include "ibm.hpp"
+#include "spread_pairs.hpp"
 
 namespace couplb {
 
@@ -56,12 +57,13 @@ __device__ inline double block_reduce(double v, int op, double* sh) {
 
 // stencil geometry and weights of each marker (as k_marker, no velocity
 // gather), the initial force F0 (caller order, lattice units; null = zero)
-// into rec.f and Fm, and the target velocity in sorted order, lattice units
+// into rec.f, Fm and P (the vector the operator is applied to), and the
+// target velocity in sorted order, lattice units
 template <int D>
 __global__ void __launch_bounds__(256) k_mdf_init(int n, const int* perm, const double* x,
                                                   const double* V, const double* F0, IbmParams ip,
                                                   Geo g, Walls w, MarkerRec* rec, double* Vs,
-                                                  double* Fm) {
+                                                  double* Fm, double* P) {
 #pragma clang fp contract(off)
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
@@ -112,45 +114,40 @@ __global__ void __launch_bounds__(256) k_mdf_init(int n, const int* perm, const 
   r.fy = (r.valid && F0) ? F0[3 * (size_t)idx + 1] : 0.0;
   r.fz = (r.valid && F0) ? F0[3 * (size_t)idx + 2] : 0.0;
   rec[p] = r;
-  Fm[3 * (size_t)p + 0] = r.fx;
-  Fm[3 * (size_t)p + 1] = r.fy;
-  Fm[3 * (size_t)p + 2] = r.fz;
+  Fm[3 * (size_t)p + 0] = P[3 * (size_t)p + 0] = r.fx;
+  Fm[3 * (size_t)p + 1] = P[3 * (size_t)p + 1] = r.fy;
+  Fm[3 * (size_t)p + 2] = P[3 * (size_t)p + 2] = r.fz;
 #pragma unroll
   for (int c = 0; c < 3; c++) Vs[3 * (size_t)p + c] = V[3 * (size_t)idx + c] / ip.vel_scale;
 }
 
-// node values of the operator: ut = [u* +] S(rec.f)/(2 rho) at every pair node
-__global__ void __launch_bounds__(256) k_mdf_ut(long ne, const unsigned* skey, const int* pp,
-                                                const double* pw, unsigned sentinel,
-                                                const MarkerRec* rec, const double* u4, int affine,
-                                                double* ut, const MdfScalars* S) {
+// node values of the operator, ut = [u*] + S(P)/(2 rho), one thread per
+// node run of the sorted pairs
+__global__ void __launch_bounds__(256) k_mdf_ut(int nh, const int* heads, const int* hlen,
+                                                const unsigned* hnode, const int* pp,
+                                                const double* pw, const double* P, const double* u4,
+                                                int affine, double* ut, const MdfScalars* S) {
   if (S->done) return;
-  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= ne) return;
-  const unsigned k = skey[e];
-  if (k == sentinel || (e > 0 && skey[e - 1] == k)) return;
-  double ax = 0.0, ay = 0.0, az = 0.0;
-  for (long f = e; f < ne && skey[f] == k; f++) {
-    const MarkerRec& r = rec[pp[f]];
-    const double wgt = pw[f];
-    ax += r.fx * wgt;
-    ay += r.fy * wgt;
-    az += r.fz * wgt;
-  }
+  const int h = blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= nh) return;
+  double ax, ay, az;
+  run_sum(heads[h], hlen[h], pp, pw, P, ax, ay, az);
+  const unsigned k = hnode[h];
   const double4 q = *reinterpret_cast<const double4*>(u4 + 4 * (size_t)k);
-  const double h = 0.5 / q.w;
-  ut[3 * (size_t)k + 0] = (affine ? q.x : 0.0) + ax * h;
-  ut[3 * (size_t)k + 1] = (affine ? q.y : 0.0) + ay * h;
-  ut[3 * (size_t)k + 2] = (affine ? q.z : 0.0) + az * h;
+  const double hf = 0.5 / q.w;
+  ut[3 * (size_t)k + 0] = (affine ? q.x : 0.0) + ax * hf;
+  ut[3 * (size_t)k + 1] = (affine ? q.y : 0.0) + ay * hf;
+  ut[3 * (size_t)k + 2] = (affine ? q.z : 0.0) + az * hf;
 }
 
 // Interpolate ut at every marker.  affine: out = r = Vs - I(ut), block
 // partials (sum r.r, max |r|).  linear: out = A p = I0(ut), block partial
-// sum p.Ap (p = rec.f).
+// sum p.Ap.
 template <int D>
 __global__ void __launch_bounds__(MDF_THREADS) k_mdf_interp(int n, Geo g, Walls w, const double* ut,
                                                             int affine, const MarkerRec* rec,
-                                                            const double* Vs, double* out,
+                                                            const double* Vs, const double* P,
+                                                            double* out,
                                                             double* part_a, double* part_b,
                                                             const MdfScalars* S) {
   __shared__ double sh[MDF_THREADS];
@@ -193,8 +190,7 @@ __global__ void __launch_bounds__(MDF_THREADS) k_mdf_interp(int n, Geo g, Walls 
         b = sqrt(a);
       } else {
         o0 = u0; o1 = u1; o2 = u2;
-        const MarkerRec& r = rec[p];
-        a = r.fx * o0 + r.fy * o1 + r.fz * o2;
+        a = P[3 * (size_t)p + 0] * o0 + P[3 * (size_t)p + 1] * o1 + P[3 * (size_t)p + 2] * o2;
       }
     }
     out[3 * (size_t)p + 0] = o0;
@@ -218,11 +214,13 @@ __global__ void __launch_bounds__(MDF_THREADS) k_mdf_interp(int n, Geo g, Walls 
 enum { MDF_RESID = 0, MDF_ALPHA = 1, MDF_BETA = 2 };
 //   Phases 0 and 2 set done = 3 once maxit applications are reached.
 //   respect_done: phase 0 also returns at once when done is set (loops).
+//   mirror: copy of the scalars in pinned host memory, read by the host
+//   after a stream synchronisation without a device-to-host copy.
 __global__ void __launch_bounds__(MDF_THREADS) k_mdf_scalar(const double* part_a,
                                                             const double* part_b, int nblk,
                                                             int phase, double tol, int first,
                                                             int maxit, int respect_done,
-                                                            MdfScalars* S) {
+                                                            MdfScalars* S, MdfScalars* mirror) {
   __shared__ double sh[MDF_THREADS];
   if ((phase != MDF_RESID || respect_done) && S->done) return;
   double a = 0.0, b = 0.0;
@@ -252,21 +250,21 @@ __global__ void __launch_bounds__(MDF_THREADS) k_mdf_scalar(const double* part_a
     if (b < tol) S->done = 1;
     else if (S->iters >= maxit) S->done = 3;
   }
+  if (mirror) *mirror = *S;
 }
 
 // F += alpha p, r -= alpha Ap; block partials of r.r and max |r|
 __global__ void __launch_bounds__(MDF_THREADS) k_cg_update(int n, const MarkerRec* rec,
-                                                           const double* Ap, double* Fm, double* R,
-                                                           double* part_a, double* part_b,
-                                                           const MdfScalars* S) {
+                                                           const double* P, const double* Ap,
+                                                           double* Fm, double* R, double* part_a,
+                                                           double* part_b, const MdfScalars* S) {
   __shared__ double sh[MDF_THREADS];
   if (S->done) return;
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   double a = 0.0, b = 0.0;
   if (p < n && rec[p].valid) {
     const double al = S->alpha;
-    const MarkerRec& r = rec[p];
-    const double pv[3] = {r.fx, r.fy, r.fz};
+    const double pv[3] = {P[3 * (size_t)p + 0], P[3 * (size_t)p + 1], P[3 * (size_t)p + 2]};
 #pragma unroll
     for (int c = 0; c < 3; c++) {
       Fm[3 * (size_t)p + c] += al * pv[c];
@@ -284,43 +282,45 @@ __global__ void __launch_bounds__(MDF_THREADS) k_cg_update(int n, const MarkerRe
   }
 }
 
-// p = r + beta p (beta = 0: restart with p = r), p lives in rec.f
-__global__ void __launch_bounds__(256) k_cg_direction(int n, const double* R, MarkerRec* rec,
-                                                      const MdfScalars* S, int restart) {
+// p = r + beta p (beta = 0: restart with p = r)
+__global__ void __launch_bounds__(256) k_cg_direction(int n, const double* R, const MarkerRec* rec,
+                                                      double* P, const MdfScalars* S, int restart) {
   if (!restart && S->done) return;
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n || !rec[p].valid) return;
   const double be = restart ? 0.0 : S->beta;
-  MarkerRec& r = rec[p];
-  r.fx = R[3 * (size_t)p + 0] + be * r.fx;
-  r.fy = R[3 * (size_t)p + 1] + be * r.fy;
-  r.fz = R[3 * (size_t)p + 2] + be * r.fz;
+#pragma unroll
+  for (int c = 0; c < 3; c++) P[3 * (size_t)p + c] = R[3 * (size_t)p + c] + be * P[3 * (size_t)p + c];
 }
 
 // Richardson step F += 2 omega r (after a true-residual evaluation that did
-// not converge); rec.f carries F
+// not converge); P carries F
 __global__ void __launch_bounds__(256) k_mdf_relax(int n, const double* R, double omega, double* Fm,
-                                                   MarkerRec* rec, const MdfScalars* S) {
+                                                   const MarkerRec* rec, double* P, const MdfScalars* S) {
   if (S->done) return;
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n || !rec[p].valid) return;
   const double k2 = 2.0 * omega;
-  MarkerRec& r = rec[p];
-  const double f0 = Fm[3 * (size_t)p + 0] + k2 * R[3 * (size_t)p + 0];
-  const double f1 = Fm[3 * (size_t)p + 1] + k2 * R[3 * (size_t)p + 1];
-  const double f2 = Fm[3 * (size_t)p + 2] + k2 * R[3 * (size_t)p + 2];
-  Fm[3 * (size_t)p + 0] = f0; Fm[3 * (size_t)p + 1] = f1; Fm[3 * (size_t)p + 2] = f2;
-  r.fx = f0; r.fy = f1; r.fz = f2;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const double f = Fm[3 * (size_t)p + c] + k2 * R[3 * (size_t)p + c];
+    Fm[3 * (size_t)p + c] = f;
+    P[3 * (size_t)p + c] = f;
+  }
 }
 
-// rec.f = F (before a true residual and for the LB step's spreading)
-__global__ void __launch_bounds__(256) k_mdf_setf(int n, const double* Fm, MarkerRec* rec) {
+// P = F (before a true residual) and rec.f = F (the LB step's spreading)
+__global__ void __launch_bounds__(256) k_mdf_setf(int n, const double* Fm, MarkerRec* rec, double* P) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   MarkerRec& r = rec[p];
-  r.fx = r.valid ? Fm[3 * (size_t)p + 0] : 0.0;
-  r.fy = r.valid ? Fm[3 * (size_t)p + 1] : 0.0;
-  r.fz = r.valid ? Fm[3 * (size_t)p + 2] : 0.0;
+  double f[3];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    f[c] = r.valid ? Fm[3 * (size_t)p + c] : 0.0;
+    P[3 * (size_t)p + c] = f[c];
+  }
+  r.fx = f[0]; r.fy = f[1]; r.fz = f[2];
 }
 
 __global__ void k_mdf_reset(MdfScalars* S, int clear_iters) {
@@ -403,9 +403,9 @@ __global__ void __launch_bounds__(256) k_mlp_features(int n, Geo g, Walls w, con
   xr[66] = (float)(s2 / sg);
 }
 
-// F0 = 16 sigma y from the network's outputs (sorted order) into Fm and rec.f
+// F0 = 16 sigma y from the network's outputs (sorted order) into Fm, P and rec.f
 __global__ void __launch_bounds__(256) k_mdf_warm(int n, const float* Y, const double* sig,
-                                                  double* Fm, MarkerRec* rec) {
+                                                  double* Fm, MarkerRec* rec, double* P) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
   MarkerRec& r = rec[p];
@@ -416,6 +416,7 @@ __global__ void __launch_bounds__(256) k_mdf_warm(int n, const float* Y, const d
   for (int c = 0; c < 3; c++) {
     f[c] = ok ? k * (double)Y[3 * (size_t)p + c] : 0.0;
     Fm[3 * (size_t)p + c] = f[c];
+    P[3 * (size_t)p + c] = f[c];
   }
   r.fx = f[0]; r.fy = f[1]; r.fz = f[2];
 }

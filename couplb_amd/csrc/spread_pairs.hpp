@@ -66,24 +66,72 @@ __global__ void __launch_bounds__(256) k_pair_weights(long ne, const unsigned* s
   if (e == 0 || skey[e - 1] != k) flag[k] = 1;
 }
 
-// F_node = sum over the node's run of F_marker * w (couplb_ibm.h:125-130)
-__global__ void __launch_bounds__(256) k_pair_spread(long ne, const unsigned* skey, const int* pp,
-                                                     const double* pw, unsigned sentinel,
-                                                     const MarkerRec* rec, double* Fx, double* Fy,
-                                                     double* Fz) {
-#pragma clang fp contract(off)
+// first pair of every node's run
+__global__ void __launch_bounds__(256) k_pair_heads(long ne, const unsigned* skey, unsigned sentinel,
+                                                    unsigned char* hflag) {
   const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= ne) return;
   const unsigned k = skey[e];
-  if (k == sentinel || (e > 0 && skey[e - 1] == k)) return;
-  double ax = 0.0, ay = 0.0, az = 0.0;
-  for (long f = e; f < ne && skey[f] == k; f++) {
-    const MarkerRec& r = rec[pp[f]];
-    const double wgt = pw[f];
-    ax += r.fx * wgt;
-    ay += r.fy * wgt;
-    az += r.fz * wgt;
+  hflag[e] = (k != sentinel && (e == 0 || skey[e - 1] != k)) ? 1 : 0;
+}
+
+// node and length of each run (heads in increasing pair order)
+__global__ void __launch_bounds__(256) k_head_info(int nh, const int* heads, const unsigned* skey,
+                                                   long ne, int* hlen, unsigned* hnode) {
+  const int h = blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= nh) return;
+  const long s = heads[h];
+  const unsigned k = skey[s];
+  long f = s + 1;
+  while (f < ne && skey[f] == k) f++;
+  hlen[h] = (int)(f - s);
+  hnode[h] = k;
+}
+
+// Sum of one node's run, sum_f src[pp[f]] * pw[f] in pair order
+// (couplb_ibm.h:125-130: F_node += F_marker * w); src holds three values
+// per sorted marker.  Four pairs are loaded per round.
+__device__ inline void run_sum(int s, int len, const int* pp, const double* pw, const double* src,
+                               double& ax, double& ay, double& az) {
+#pragma clang fp contract(off)
+  ax = ay = az = 0.0;
+  int f = s;
+  const int end = s + len;
+  for (; f + 4 <= end; f += 4) {
+    int q[4];
+    double w[4], v[4][3];
+#pragma unroll
+    for (int i = 0; i < 4; i++) { q[i] = pp[f + i]; w[i] = pw[f + i]; }
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) v[i][c] = src[3 * (size_t)q[i] + c];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      ax += v[i][0] * w[i];
+      ay += v[i][1] * w[i];
+      az += v[i][2] * w[i];
+    }
   }
+  for (; f < end; f++) {
+    const int q = pp[f];
+    const double w = pw[f];
+    ax += src[3 * (size_t)q + 0] * w;
+    ay += src[3 * (size_t)q + 1] * w;
+    az += src[3 * (size_t)q + 2] * w;
+  }
+}
+
+// node forces of the fixed-marker map: F_node = run sum of the marker forces
+__global__ void __launch_bounds__(256) k_pair_spread(int nh, const int* heads, const int* hlen,
+                                                     const unsigned* hnode, const int* pp,
+                                                     const double* pw, const double* src, double* Fx,
+                                                     double* Fy, double* Fz) {
+  const int h = blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= nh) return;
+  double ax, ay, az;
+  run_sum(heads[h], hlen[h], pp, pw, src, ax, ay, az);
+  const unsigned k = hnode[h];
   Fx[k] = ax;
   Fy[k] = ay;
   Fz[k] = az;

@@ -571,7 +571,7 @@ int build_pairs(couplb_ctx* c, const SlotHost& sl) {
 int mdf_impl(couplb_ctx* c, int n, const double* x, const double* V, const double* F0,
              double* f_body, const couplb_mdf_args* a, const couplb_mdf_learn* learn,
              couplb_mdf_stats* st) {
-  if (n < 0 || !a || !x || !V || !f_body) return fail(c, COUPLB_EINVAL, "bad MDF arguments");
+  if (n < 0 || !a || (n > 0 && (!x || !V || !f_body))) return fail(c, COUPLB_EINVAL, "bad MDF arguments");
   if (!(a->tol > 0) || a->max_iter < 1 || !(a->relax > 0) ||
       (a->method != COUPLB_MDF_RICHARDSON && a->method != COUPLB_MDF_CG))
     return fail(c, COUPLB_EINVAL, "MDF needs tol > 0, max_iter >= 1, relax > 0 and a known method");

@@ -1,0 +1,77 @@
+"""Edge cases of the device IBM entry points: empty marker sets, markers
+that cannot be placed (non-finite or far outside the box), rows longer than
+the IBM tile limit, and misuse of the multi-direct-forcing call."""
+import numpy as np
+import pytest
+
+from tests.scenarios import DeviceArray
+
+pytestmark = pytest.mark.gpu
+
+KW = dict(lo=(0.0, 0.0, 0.0), inv_dx=1.0, xi_over_dtLB=1.0, vel_scale=1.0, inv_N=1.0,
+          force_scale=1.0)
+
+
+def test_empty_marker_set_is_a_no_op():
+    import couplb_amd
+    d = couplb_amd.Device(3, 8, 8, 8, tau=0.8)
+    d.init_equilibrium(1.0, (0.01, 0.0, 0.0))
+    f0 = d.get_state()
+    empty = np.zeros((0, 3))
+    d.ibm(empty, empty, np.zeros(0), np.zeros((0, 3)), **KW)
+    st = d.mdf_device(0, None, None, None, None, (0, 0, 0), 1.0, 1.0, 1.0)
+    assert st["iterations"] == 0
+    d.step((0.0, 0.0, 0.0), 0)
+    d.sync()
+    # uniform equilibrium without force is a fixed point of the step
+    assert np.allclose(d.get_state()[:, 1:-1, 1:-1, 1:-1], f0[:, 1:-1, 1:-1, 1:-1], rtol=0, atol=1e-15)
+
+
+def test_unplaceable_markers_get_the_pure_damping_force():
+    # no stencil weight: the interpolated velocity is 0 (couplb_ibm.h:88-89), so
+    # F = m xi (0 - v), and nothing is spread
+    import couplb_amd
+    d = couplb_amd.Device(3, 8, 8, 8, tau=0.8, kernel=couplb_amd.PESKIN4)
+    d.init_equilibrium(1.0, (0.01, 0.0, 0.0))
+    x = np.array([[np.nan, 1.0, 1.0], [1e12, 2.0, 2.0], [3.0, np.inf, 3.0]])
+    v = np.array([[1e-3, 2e-3, 3e-3], [0.0, -1e-3, 0.0], [5e-4, 0.0, 0.0]])
+    m = np.array([1.0, 2.0, 3.0])
+    f = np.zeros((3, 3))
+    d.ibm(x, v, m, f, **KW)
+    assert np.array_equal(f, -m[:, None] * v)
+    f_before = d.get_state()
+    d.step((0.0, 0.0, 0.0), 0)
+    d.sync()
+    g = couplb_amd.Device(3, 8, 8, 8, tau=0.8)
+    g.set_state(f_before)
+    g.step((0.0, 0.0, 0.0), 0)
+    assert np.array_equal(d.get_state()[:, 1:-1, 1:-1, 1:-1], g.get_state()[:, 1:-1, 1:-1, 1:-1])
+
+
+def test_rows_longer_than_the_ibm_tile_limit_are_rejected():
+    import couplb_amd
+    d = couplb_amd.Device(3, 1100, 4, 4, tau=0.8)
+    d.init_equilibrium(1.0, (0.0, 0.0, 0.0))
+    x = np.array([[10.5, 2.0, 2.0]])
+    with pytest.raises(couplb_amd.CoupLBError, match="1024"):
+        d.ibm(x, np.zeros((1, 3)), np.ones(1), np.zeros((1, 3)), **KW)
+    d.step((0.0, 0.0, 0.0), 0)      # the LB step itself has no such limit
+    d.sync()
+
+
+def test_mdf_must_be_the_only_ibm_call_of_an_interval():
+    import couplb_amd
+    d = couplb_amd.Device(3, 12, 12, 12, tau=0.8, kernel=couplb_amd.PESKIN4)
+    d.init_equilibrium(1.0, (0.01, 0.0, 0.0))
+    x = DeviceArray(np.array([[6.2, 6.1, 5.9], [5.1, 6.6, 6.3]]))
+    V = DeviceArray(np.zeros((2, 3)))
+    fb = DeviceArray(np.zeros((2, 3)))
+    d.mdf_device(2, x.addr, V.addr, None, fb.addr, (0, 0, 0), 1.0, 1.0, 1.0, tol=1e-8)
+    with pytest.raises(couplb_amd.CoupLBError, match="only IBM call"):
+        d.mdf_device(2, x.addr, V.addr, None, fb.addr, (0, 0, 0), 1.0, 1.0, 1.0, tol=1e-8)
+    with pytest.raises(couplb_amd.CoupLBError):
+        d.mdf_device(2, x.addr, V.addr, None, fb.addr, (0, 0, 0), 1.0, 1.0, 1.0, tol=-1.0)
+    d.step((0.0, 0.0, 0.0), 0)
+    st = d.mdf_device(2, x.addr, V.addr, None, fb.addr, (0, 0, 0), 1.0, 1.0, 1.0, tol=1e-8,
+                      positions_unchanged=True)
+    assert st["converged"]

@@ -111,9 +111,11 @@ struct PullCtx {
   float scale, inv_scale;  // storage scale of reduced formats
 };
 
-template <int D, typename S>
-__device__ inline void pull_node(const PullCtx& pc, int i, int j, int k,
-                                 typename Store<S>::T (&f)[Lat<D>::Q]) {
+// fluid(q, si, sj, sk) returns the stored value of plane q at fluid source
+// node (si, sj, sk); the other rules read the node itself from global memory
+template <int D, typename S, typename Fluid>
+__device__ inline void pull_node_with(const PullCtx& pc, int i, int j, int k,
+                                      typename Store<S>::T (&f)[Lat<D>::Q], Fluid fluid) {
   using L = Lat<D>;
   using T = typename Store<S>::T;
   constexpr int Q = L::Q;
@@ -170,8 +172,7 @@ __device__ inline void pull_node(const PullCtx& pc, int i, int j, int k,
     const int sj = ey == 1 ? jm : (ey == -1 ? jp : j);
     const int sk = (D == 3) ? (ez == 1 ? km : (ez == -1 ? kp : k)) : 0;
     if (wt == 0) {
-      const unsigned addr = (unsigned)si + (unsigned)nx * ((unsigned)sj + (unsigned)ny * (unsigned)sk);
-      f[q] = Store<S>::load(src + q * pc.pitch + addr);
+      f[q] = fluid(q, si, sj, sk);
     } else {
       int qs = q;
       if (wt == 1 || wt == 2) qs = L::opp(q);
@@ -196,6 +197,17 @@ __device__ inline void pull_node(const PullCtx& pc, int i, int j, int k,
     }
     if (Store<S>::deviation) f[q] *= (T)pc.inv_scale;
   }
+}
+
+template <int D, typename S>
+__device__ inline void pull_node(const PullCtx& pc, int i, int j, int k,
+                                 typename Store<S>::T (&f)[Lat<D>::Q]) {
+  const S* src = static_cast<const S*>(pc.src);
+  const int nx = pc.g.nx, ny = pc.g.ny;
+  pull_node_with<D, S>(pc, i, j, k, f, [&](int q, int si, int sj, int sk) {
+    const unsigned addr = (unsigned)si + (unsigned)nx * ((unsigned)sj + (unsigned)ny * (unsigned)sk);
+    return Store<S>::load(src + q * pc.pitch + addr);
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -277,29 +289,24 @@ struct StepArgs {
   const double* Fz;
 };
 
-template <int D, typename S>
-__global__ void __launch_bounds__(256) k_step(StepArgs a) {
+// Everything of the step after the pull: IBM force, gravity, macroscopic
+// with the Guo half-force, density bookkeeping, collision.  out(q, v) stores
+// the post-collision value of plane q (already scaled for the reduced
+// formats).
+template <int D, typename S, typename Out>
+__device__ inline void step_node(const StepArgs& a, int i, int j, int k, unsigned node,
+                                 typename Store<S>::T (&f)[Lat<D>::Q], Out out) {
   using L = Lat<D>;
   using T = typename Store<S>::T;
   constexpr int Q = L::Q;
   const Geo& g = a.pc.g;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y, k = blockIdx.z;
-  if (i >= g.nx) return;
-  const unsigned node = (unsigned)i + (unsigned)g.nx * ((unsigned)j + (unsigned)g.ny * (unsigned)k);
-
-  // IBM force spread by the last cycle's markers (k_spread), present only
-  // at flagged nodes
+  // IBM force spread by the last cycle's markers, present only at flagged nodes
   T Fx = 0, Fy = 0, Fz = 0;
   if (a.has_ibm && a.flag[node]) {
     Fx = (T)a.Fx[node];
     Fy = (T)a.Fy[node];
     Fz = (T)a.Fz[node];
   }
-
-  T f[Q];
-  pull_node<D, S>(a.pc, i, j, k, f);
-
   // density (deviation form for the reduced formats: A = rho - base)
   T A = 0, jx = 0, jy = 0, jz = 0;
 #pragma unroll
@@ -330,10 +337,9 @@ __global__ void __launch_bounds__(256) k_step(StepArgs a) {
         (D == 3 && ((k == 0 && a.pc.w.type[2][0] == 2) || (k == g.nz - 1 && a.pc.w.type[2][1] == 2)));
     if (wall_adj) a.rho_out[node] = (double)rho;
   }
-  S* dst = static_cast<S*>(a.dst);
   if constexpr (!Store<S>::deviation) {
-    collide_exact<D>(f, rho, jx, jy, jz, Fx, Fy, Fz, a.omega, dst, a.pc.pitch, node);
-    return;
+    collide_exact<D>(f, rho, jx, jy, jz, Fx, Fy, Fz, a.omega, static_cast<S*>(a.dst), a.pc.pitch, node);
+    (void)out;
   } else {
     const T inv_r = (T)1 / rho;
     const T ux = (jx + (T)0.5 * Fx) * inv_r;
@@ -355,9 +361,82 @@ __global__ void __launch_bounds__(256) k_step(StepArgs a) {
       // Guo source w(1-om/2)[(e.F - u.F)/cs^2 + (e.u)(e.F)/cs^4]  (couplb_collision.h:51-52)
       const T Si = wq * hom * ((T)3 * (ef - uf) + (T)9 * eu * ef);
       T fn = f[q] - om * (f[q] - feq) + Si;
-      fn *= scale;
-      Store<S>::store(dst + q * a.pc.pitch + node, fn);
+      out(q, fn * scale);
     }
+  }
+}
+
+template <int D, typename S>
+__global__ void __launch_bounds__(256) k_step(StepArgs a) {
+  using L = Lat<D>;
+  using T = typename Store<S>::T;
+  constexpr int Q = L::Q;
+  const Geo& g = a.pc.g;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y, k = blockIdx.z;
+  if (i >= g.nx) return;
+  const unsigned node = (unsigned)i + (unsigned)g.nx * ((unsigned)j + (unsigned)g.ny * (unsigned)k);
+  T f[Q];
+  pull_node<D, S>(a.pc, i, j, k, f);
+  S* dst = static_cast<S*>(a.dst);
+  step_node<D, S>(a, i, j, k, node, f,
+                  [&](int q, T v) { Store<S>::store(dst + q * a.pc.pitch + node, v); });
+}
+
+// 16-bit formats: the same step with the 19 source rows of a 256-node row
+// chunk staged through LDS in 8-byte groups (and the results written back the
+// same way), so that each global access moves 8 bytes per lane instead of
+// one 2-byte population.  Needs nx % (8 / sizeof(S)) == 0, a plane pitch
+// that keeps the groups aligned, and a state in pull form (not the
+// post-import local form).
+constexpr int LDS_XC = 256;
+template <int D, typename S>
+__global__ void __launch_bounds__(LDS_XC) k_step_lds(StepArgs a) {
+  using L = Lat<D>;
+  using T = typename Store<S>::T;
+  constexpr int Q = L::Q;
+  constexpr int G = 8 / (int)sizeof(S);   // elements per 8-byte group
+  constexpr int W = LDS_XC + 2 * G;       // window [x0 - G, x0 + XC + G)
+  __shared__ __attribute__((aligned(16))) S win[Q][W];
+  __shared__ __attribute__((aligned(16))) S res[Q][LDS_XC];
+  const Geo& g = a.pc.g;
+  const int x0 = blockIdx.x * LDS_XC, j = blockIdx.y, k = blockIdx.z;
+  const int xc = (g.nx - x0 < LDS_XC) ? g.nx - x0 : LDS_XC;
+  const S* src = static_cast<const S*>(a.pc.src);
+  const int jm = (j == 0) ? g.ny - 1 : j - 1, jp = (j == g.ny - 1) ? 0 : j + 1;
+  int km = 0, kp = 0;
+  if (D == 3) { km = (k == 0) ? g.nz - 1 : k - 1; kp = (k == g.nz - 1) ? 0 : k + 1; }
+  const int ngr = (xc + 2 * G) / G;
+  for (int e = threadIdx.x; e < Q * ngr; e += LDS_XC) {
+    const int q = e / ngr, gi = e - q * ngr;
+    int x = x0 - G + G * gi;
+    x = (x < 0) ? x + g.nx : (x >= g.nx ? x - g.nx : x);
+    const int ey = L::ey(q), ez = L::ez(q);
+    const int sj = ey == 1 ? jm : (ey == -1 ? jp : j);
+    const int sk = (D == 3) ? (ez == 1 ? km : (ez == -1 ? kp : k)) : 0;
+    const size_t addr = (size_t)q * a.pc.pitch + (unsigned)x + (unsigned)g.nx * ((unsigned)sj + (unsigned)g.ny * (unsigned)sk);
+    const uint64_t v = __builtin_nontemporal_load(reinterpret_cast<const uint64_t*>(src + addr));
+    *reinterpret_cast<uint64_t*>(&win[q][G * gi]) = v;
+  }
+  __syncthreads();
+  const int t = threadIdx.x;
+  if (t < xc) {
+    const int i = x0 + t;
+    const unsigned node = (unsigned)i + (unsigned)g.nx * ((unsigned)j + (unsigned)g.ny * (unsigned)k);
+    T f[Q];
+    pull_node_with<D, S>(a.pc, i, j, k, f, [&](int q, int, int, int) {
+      return Store<S>::decode(win[q][t + G - L::ex(q)]);
+    });
+    step_node<D, S>(a, i, j, k, node, f, [&](int q, T v) { res[q][t] = Store<S>::encode(v); });
+  }
+  __syncthreads();
+  S* dst = static_cast<S*>(a.dst);
+  const int ngo = xc / G;
+  const size_t row = (size_t)(unsigned)g.nx * ((unsigned)j + (unsigned)g.ny * (unsigned)k);
+  for (int e = threadIdx.x; e < Q * ngo; e += LDS_XC) {
+    const int q = e / ngo, gi = e - q * ngo;
+    const uint64_t v = *reinterpret_cast<const uint64_t*>(&res[q][G * gi]);
+    __builtin_nontemporal_store(v, reinterpret_cast<uint64_t*>(dst + (size_t)q * a.pc.pitch + row + x0 + G * gi));
   }
 }
 

@@ -105,6 +105,8 @@ template <> struct Store<double> {
   static constexpr bool deviation = false;
   __device__ static inline T load(const double* p) { return __builtin_nontemporal_load(p); }
   __device__ static inline void store(double* p, T v) { __builtin_nontemporal_store(v, p); }
+  __device__ static inline T decode(double v) { return v; }
+  __device__ static inline double encode(T v) { return v; }
 };
 
 template <> struct Store<float> {
@@ -112,6 +114,8 @@ template <> struct Store<float> {
   static constexpr bool deviation = true;
   __device__ static inline T load(const float* p) { return __builtin_nontemporal_load(p); }
   __device__ static inline void store(float* p, T v) { __builtin_nontemporal_store(v, p); }
+  __device__ static inline T decode(float v) { return v; }
+  __device__ static inline float encode(T v) { return v; }
 };
 
 template <> struct Store<_Float16> {
@@ -123,6 +127,8 @@ template <> struct Store<_Float16> {
   __device__ static inline void store(_Float16* p, T v) {
     __builtin_nontemporal_store((_Float16)v, p);
   }
+  __device__ static inline T decode(_Float16 v) { return (float)v; }
+  __device__ static inline _Float16 encode(T v) { return (_Float16)v; }
 };
 
 template <> struct Store<bf16_t> {
@@ -133,10 +139,12 @@ template <> struct Store<bf16_t> {
     return __uint_as_float((uint32_t)b << 16);
   }
   // round to nearest even; inputs are finite in normal operation
-  __device__ static inline void store(bf16_t* p, T v) {
+  __device__ static inline void store(bf16_t* p, T v) { __builtin_nontemporal_store(encode(v).bits, &p->bits); }
+  __device__ static inline T decode(bf16_t v) { return __uint_as_float((uint32_t)v.bits << 16); }
+  __device__ static inline bf16_t encode(T v) {
     const uint32_t u = __float_as_uint(v);
     const uint32_t r = u + 0x7FFFu + ((u >> 16) & 1u);
-    __builtin_nontemporal_store((uint16_t)(r >> 16), &p->bits);
+    return bf16_t{(uint16_t)(r >> 16)};
   }
 };
 

@@ -73,3 +73,31 @@ def test_reduced_storage_error_vs_fp64(fp64, storage):
     err = np.linalg.norm(u - u64) / np.linalg.norm(u64)
     print(f"{storage}: rel-L2(u) vs FP64 = {err:.3e}, mass = {diag['mass']:.12e}")
     assert err < BOUNDS[storage]
+
+
+@pytest.mark.parametrize("storage", ["fp32", "fp16", "bf16"])
+def test_staged_step_equals_per_node_step(storage, monkeypatch):
+    """The reduced formats run the LDS-staged step (8-byte row groups); a
+    plane pitch that is not a multiple of 4 elements selects the per-node
+    step instead.  Both must produce identical bits, walls and IBM included."""
+    from tests.scenarios import DeviceArray
+    walls = ((0, 0), (2, 1), (3, 3))
+    face = [[[0, 0, 0], [0, 0, 0]], [[0.01, 0, 0], [0, 0, 0]], [[0, 0, 0], [0, 0, 0]]]
+    rng = np.random.default_rng(6)
+    x = rng.uniform(2.0, 14.0, (40, 3))
+    states = []
+    for pad in ("256", "257"):
+        monkeypatch.setenv("COUPLB_PLANE_PAD", pad)
+        d = couplb_amd.Device(3, 32, 16, 16, tau=0.7, walls=walls, face_vel=face,
+                              kernel=couplb_amd.PESKIN4, storage=storage)
+        d.init_equilibrium(1.0, (0.01, 0.002, 0.0))
+        dx, dv = DeviceArray(x), DeviceArray(np.zeros((40, 3)))
+        dm, df = DeviceArray(np.ones(40)), DeviceArray(np.zeros((40, 3)))
+        for s in range(30):
+            d.step((1e-5, 0.0, 0.0), 0)
+            d.ibm_device(40, dx.addr, dv.addr, dm.addr, df.addr, (0, 0, 0), 1.0, 1.0, 1.0, 1.0, 1.0,
+                         positions_unchanged=s > 0)
+        d.step((1e-5, 0.0, 0.0), 0)
+        d.sync()
+        states.append(d.get_state())
+    assert np.array_equal(states[0], states[1])

@@ -232,9 +232,9 @@ template <int D, typename S> struct LaunchStep {
   static void run(couplb_ctx* c, const StepArgs& a) {
     if constexpr (Store<S>::deviation && sizeof(S) == 2) {
       // (FP32 is faster per node: its staged windows halve the occupancy)
-      constexpr int G = 8 / (int)sizeof(S);
-      if (!a.pc.local && c->geo.nx % G == 0 && c->pitch % 4 == 0) {
-        // reduced formats: 8-byte staged rows (k_step_lds)
+      constexpr int G = LDS_GB / (int)sizeof(S);
+      if (!a.pc.local && c->geo.nx % G == 0 && c->pitch % G == 0) {
+        // 16-bit formats: 16-byte staged row groups (k_step_lds)
         const dim3 grid((unsigned)((c->geo.nx + LDS_XC - 1) / LDS_XC), (unsigned)c->geo.ny,
                         (unsigned)(c->D == 3 ? c->geo.nz : 1));
         hipLaunchKernelGGL((k_step_lds<D, S>), grid, dim3(LDS_XC), 0, c->stream, a);

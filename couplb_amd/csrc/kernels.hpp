@@ -384,39 +384,54 @@ __global__ void __launch_bounds__(256) k_step(StepArgs a) {
 }
 
 // 16-bit formats: the same step with the 19 source rows of a 256-node row
-// chunk staged through LDS in 8-byte groups (and the results written back the
-// same way), so that each global access moves 8 bytes per lane instead of
-// one 2-byte population.  Needs nx % (8 / sizeof(S)) == 0, a plane pitch
-// that keeps the groups aligned, and a state in pull form (not the
-// post-import local form).
+// chunk staged through LDS in 16-byte groups (and the results written back
+// the same way), so that each global access moves 16 bytes per lane instead
+// of one 2-byte population.  Needs nx % 8 == 0, a plane pitch that keeps
+// the groups aligned (a multiple of 8 elements), and a state in pull form
+// (not the post-import local form).
 constexpr int LDS_XC = 256;
+constexpr int LDS_GB = 16;  // bytes per staged group
 template <int D, typename S>
 __global__ void __launch_bounds__(LDS_XC) k_step_lds(StepArgs a) {
   using L = Lat<D>;
   using T = typename Store<S>::T;
+  typedef unsigned int V __attribute__((ext_vector_type(4)));  // one 16-byte group
   constexpr int Q = L::Q;
-  constexpr int G = 8 / (int)sizeof(S);   // elements per 8-byte group
-  constexpr int W = LDS_XC + 2 * G;       // window [x0 - G, x0 + XC + G)
+  constexpr int G = LDS_GB / (int)sizeof(S);  // elements per group
+  constexpr int W = LDS_XC + 2 * G;          // window [x0 - G, x0 + XC + G)
+  constexpr int NGR = W / G;                 // groups per staged row (full chunk)
+  constexpr int NGO = LDS_XC / G;            // groups per result row (full chunk)
   __shared__ __attribute__((aligned(16))) S win[Q][W];
   __shared__ __attribute__((aligned(16))) S res[Q][LDS_XC];
-  const Geo& g = a.pc.g;
+  // copies, not references: the lambdas below must not take the address of
+  // the kernel argument (that would move it to scratch)
+  const Geo g = a.pc.g;
+  const size_t pitch = a.pc.pitch;
   const int x0 = blockIdx.x * LDS_XC, j = blockIdx.y, k = blockIdx.z;
   const int xc = (g.nx - x0 < LDS_XC) ? g.nx - x0 : LDS_XC;
   const S* src = static_cast<const S*>(a.pc.src);
   const int jm = (j == 0) ? g.ny - 1 : j - 1, jp = (j == g.ny - 1) ? 0 : j + 1;
   int km = 0, kp = 0;
   if (D == 3) { km = (k == 0) ? g.nz - 1 : k - 1; kp = (k == g.nz - 1) ? 0 : k + 1; }
-  const int ngr = (xc + 2 * G) / G;
+  // signs of e_y, e_z as bit masks over q (no per-lane table lookups)
+  constexpr unsigned YP = [] { unsigned m = 0; for (int q = 0; q < Q; q++) m |= (L::ey(q) == 1) << q; return m; }();
+  constexpr unsigned YN = [] { unsigned m = 0; for (int q = 0; q < Q; q++) m |= (L::ey(q) == -1) << q; return m; }();
+  constexpr unsigned ZP = [] { unsigned m = 0; for (int q = 0; q < Q; q++) m |= (L::ez(q) == 1) << q; return m; }();
+  constexpr unsigned ZN = [] { unsigned m = 0; for (int q = 0; q < Q; q++) m |= (L::ez(q) == -1) << q; return m; }();
+  const int ngr = (xc == LDS_XC) ? NGR : (xc + 2 * G) / G;
   for (int e = threadIdx.x; e < Q * ngr; e += LDS_XC) {
-    const int q = e / ngr, gi = e - q * ngr;
+    const int q = (xc == LDS_XC) ? e / NGR : e / ngr;
+    const int gi = e - q * ngr;
     int x = x0 - G + G * gi;
     x = (x < 0) ? x + g.nx : (x >= g.nx ? x - g.nx : x);
-    const int ey = L::ey(q), ez = L::ez(q);
+    const int ey = (int)((YP >> q) & 1u) - (int)((YN >> q) & 1u);
+    const int ez = (int)((ZP >> q) & 1u) - (int)((ZN >> q) & 1u);
     const int sj = ey == 1 ? jm : (ey == -1 ? jp : j);
     const int sk = (D == 3) ? (ez == 1 ? km : (ez == -1 ? kp : k)) : 0;
-    const size_t addr = (size_t)q * a.pc.pitch + (unsigned)x + (unsigned)g.nx * ((unsigned)sj + (unsigned)g.ny * (unsigned)sk);
-    const uint64_t v = __builtin_nontemporal_load(reinterpret_cast<const uint64_t*>(src + addr));
-    *reinterpret_cast<uint64_t*>(&win[q][G * gi]) = v;
+    const size_t addr = (size_t)q * pitch + (unsigned)x +
+                        (unsigned)g.nx * ((unsigned)sj + (unsigned)g.ny * (unsigned)sk);
+    const V v = __builtin_nontemporal_load(reinterpret_cast<const V*>(src + addr));
+    *reinterpret_cast<V*>(&win[q][G * gi]) = v;
   }
   __syncthreads();
   const int t = threadIdx.x;
@@ -431,12 +446,13 @@ __global__ void __launch_bounds__(LDS_XC) k_step_lds(StepArgs a) {
   }
   __syncthreads();
   S* dst = static_cast<S*>(a.dst);
-  const int ngo = xc / G;
   const size_t row = (size_t)(unsigned)g.nx * ((unsigned)j + (unsigned)g.ny * (unsigned)k);
+  const int ngo = (xc == LDS_XC) ? NGO : xc / G;
   for (int e = threadIdx.x; e < Q * ngo; e += LDS_XC) {
-    const int q = e / ngo, gi = e - q * ngo;
-    const uint64_t v = *reinterpret_cast<const uint64_t*>(&res[q][G * gi]);
-    __builtin_nontemporal_store(v, reinterpret_cast<uint64_t*>(dst + (size_t)q * a.pc.pitch + row + x0 + G * gi));
+    const int q = (xc == LDS_XC) ? e / NGO : e / ngo;
+    const int gi = e - q * ngo;
+    const V v = *reinterpret_cast<const V*>(&res[q][G * gi]);
+    __builtin_nontemporal_store(v, reinterpret_cast<V*>(dst + (size_t)q * pitch + row + x0 + G * gi));
   }
 }
 

@@ -72,7 +72,7 @@ struct couplb_ctx {
   // IBM
   double* u4 = nullptr;   // IBM fluid velocity and density (ux, uy, uz, rho) per node
   int* stamp = nullptr;
-  double *Fx = nullptr, *Fy = nullptr, *Fz = nullptr;  // spread force
+  double* F4 = nullptr;  // spread force (fx, fy, fz, unused) per node
   unsigned char *flag = nullptr, *cover = nullptr;
   int cap = 0;
   unsigned *keys = nullptr, *skeys = nullptr;
@@ -336,9 +336,7 @@ int ensure_ibm(couplb_ctx* c, int n) {
     HIPCHK(c, hipMalloc(&c->stamp, sizeof(int) * c->N));
     HIPCHK(c, hipMemsetAsync(c->stamp, 0xFF, sizeof(int) * c->N, c->stream));
     HIPCHK(c, hipMalloc(&c->rowstart_cur, sizeof(int) * (c->geo.nrows + 1)));
-    HIPCHK(c, hipMalloc(&c->Fx, sizeof(double) * c->N));
-    HIPCHK(c, hipMalloc(&c->Fy, sizeof(double) * c->N));
-    HIPCHK(c, hipMalloc(&c->Fz, sizeof(double) * c->N));
+    HIPCHK(c, hipMalloc(&c->F4, sizeof(double) * 4 * c->N));
     HIPCHK(c, hipMalloc(&c->flag, c->N));
     HIPCHK(c, hipMalloc(&c->cover, c->N));
   }
@@ -793,7 +791,7 @@ int step_impl(couplb_ctx* c, const double g[3], int flags, bool timed) {
   a.rho_out = c->rho;
   a.has_ibm = c->nslots > 0;
   a.flag = c->flag;
-  a.Fx = c->Fx; a.Fy = c->Fy; a.Fz = c->Fz;
+  a.F4 = c->F4;
   const int S = (c->D == 3) ? stencil_size<3>() : stencil_size<2>();
   // markers at the positions of the previous LB interval: constant map
   const bool fixed = c->nslots == 1 && c->slots[0].epoch == c->prev_step_epoch &&
@@ -811,8 +809,7 @@ int step_impl(couplb_ctx* c, const double g[3], int flags, bool timed) {
     if (c->pnh > 0)
       hipLaunchKernelGGL(k_pair_spread, dim3((unsigned)((c->pnh + 255) / 256)), dim3(256), 0, c->stream,
                          c->pnh, (const int*)c->pheads, (const int*)c->phlen, (const unsigned*)c->phnode,
-                         (const int*)c->pp, (const double*)c->pw, (const double*)c->fsrc, c->Fx, c->Fy,
-                         c->Fz);
+                         (const int*)c->pp, (const double*)c->pw, (const double*)c->fsrc, c->F4);
     HIPCHK(c, hipGetLastError());
   } else if (c->nslots > 0) {
     // spread the pending marker records into the per-node force (k_spread)
@@ -832,7 +829,7 @@ int step_impl(couplb_ctx* c, const double g[3], int flags, bool timed) {
     sa.w = c->walls;
     sa.nslots = c->nslots;
     sa.slots = c->slot_table;
-    sa.Fx = c->Fx; sa.Fy = c->Fy; sa.Fz = c->Fz;
+    sa.F4 = c->F4;
     sa.flag = c->flag;
     if (timed) {
       const int rc = timer_begin(c, 2, &tspread);
@@ -980,7 +977,7 @@ int couplb_destroy(couplb_ctx* c) {
   (void)hipFree(c->f[0]); (void)hipFree(c->f[1]); (void)hipFree(c->rho);
   (void)hipFree(c->mac); (void)hipFree(c->partial); (void)hipFree(c->red);
   (void)hipFree(c->u4); (void)hipFree(c->stamp);
-  (void)hipFree(c->Fx); (void)hipFree(c->Fy); (void)hipFree(c->Fz);
+  (void)hipFree(c->F4);
   (void)hipFree(c->flag); (void)hipFree(c->cover);
   (void)hipFree(c->keys); (void)hipFree(c->skeys); (void)hipFree(c->vals); (void)hipFree(c->perm);
   (void)hipFree(c->rowstart_cur); (void)hipFree(c->bases); (void)hipFree(c->sort_tmp);

@@ -185,9 +185,7 @@ struct SpreadArgs {
   Walls w;
   int nslots;
   const Slot* slots;    // device table of nslots descriptors
-  double* Fx;
-  double* Fy;
-  double* Fz;
+  double* F4;  // node force (fx, fy, fz, unused), 32 B per node
   unsigned char* flag;  // per node: 1 where some record contributes
 };
 
@@ -303,9 +301,8 @@ __global__ void __launch_bounds__(TILE_THREADS) k_spread(SpreadArgs a) {
         const unsigned node = (unsigned)tn.x[m] + (unsigned)g.nx * ((unsigned)tn.j[r] + (unsigned)g.ny * (unsigned)tn.k[q]);
         a.flag[node] = hit[m][r][q] ? 1 : 0;
         if (hit[m][r][q]) {
-          a.Fx[node] = acc[m][r][q][0];
-          a.Fy[node] = acc[m][r][q][1];
-          a.Fz[node] = acc[m][r][q][2];
+          *reinterpret_cast<double4*>(a.F4 + 4 * (size_t)node) =
+              double4{acc[m][r][q][0], acc[m][r][q][1], acc[m][r][q][2], 0.0};
         }
       }
 }

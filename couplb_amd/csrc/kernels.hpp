@@ -284,9 +284,7 @@ struct StepArgs {
   double* rho_out;      // may alias pc.rho_coll (same node read then written)
   int has_ibm;          // spread force of the last cycle present
   const unsigned char* flag;
-  const double* Fx;
-  const double* Fy;
-  const double* Fz;
+  const double* F4;    // node force (fx, fy, fz, unused) where flag is set
 };
 
 // Everything of the step after the pull: IBM force, gravity, macroscopic
@@ -303,9 +301,10 @@ __device__ inline void step_node(const StepArgs& a, int i, int j, int k, unsigne
   // IBM force spread by the last cycle's markers, present only at flagged nodes
   T Fx = 0, Fy = 0, Fz = 0;
   if (a.has_ibm && a.flag[node]) {
-    Fx = (T)a.Fx[node];
-    Fy = (T)a.Fy[node];
-    Fz = (T)a.Fz[node];
+    const double4 F = *reinterpret_cast<const double4*>(a.F4 + 4 * (size_t)node);
+    Fx = (T)F.x;
+    Fy = (T)F.y;
+    Fz = (T)F.z;
   }
   // density (deviation form for the reduced formats: A = rho - base)
   T A = 0, jx = 0, jy = 0, jz = 0;

@@ -125,16 +125,14 @@ __device__ inline void run_sum(int s, int len, const int* pp, const double* pw, 
 // node forces of the fixed-marker map: F_node = run sum of the marker forces
 __global__ void __launch_bounds__(256) k_pair_spread(int nh, const int* heads, const int* hlen,
                                                      const unsigned* hnode, const int* pp,
-                                                     const double* pw, const double* src, double* Fx,
-                                                     double* Fy, double* Fz) {
+                                                     const double* pw, const double* src,
+                                                     double* F4) {
   const int h = blockIdx.x * blockDim.x + threadIdx.x;
   if (h >= nh) return;
   double ax, ay, az;
   run_sum(heads[h], hlen[h], pp, pw, src, ax, ay, az);
   const unsigned k = hnode[h];
-  Fx[k] = ax;
-  Fy[k] = ay;
-  Fz[k] = az;
+  *reinterpret_cast<double4*>(F4 + 4 * (size_t)k) = double4{ax, ay, az, 0.0};
 }
 
 __global__ void k_fill_u8(unsigned char* p, size_t n, unsigned char v) {

@@ -230,14 +230,13 @@ void dispatch(couplb_ctx* c, Args&&... args) {
 
 template <int D, typename S> struct LaunchStep {
   static void run(couplb_ctx* c, const StepArgs& a) {
-    if constexpr (Store<S>::deviation && sizeof(S) == 2) {
-      // (FP32 is faster per node: its staged windows halve the occupancy)
-      constexpr int G = LDS_GB / (int)sizeof(S);
+    if constexpr (Store<S>::deviation) {
+      constexpr int G = LDS_GB / (int)sizeof(S), XC = lds_chunk<S>();
       if (!a.pc.local && c->geo.nx % G == 0 && c->pitch % G == 0) {
-        // 16-bit formats: 16-byte staged row groups (k_step_lds)
-        const dim3 grid((unsigned)((c->geo.nx + LDS_XC - 1) / LDS_XC), (unsigned)c->geo.ny,
+        // reduced formats: 16-byte staged row groups (k_step_lds)
+        const dim3 grid((unsigned)((c->geo.nx + XC - 1) / XC), (unsigned)c->geo.ny,
                         (unsigned)(c->D == 3 ? c->geo.nz : 1));
-        hipLaunchKernelGGL((k_step_lds<D, S>), grid, dim3(LDS_XC), 0, c->stream, a);
+        hipLaunchKernelGGL((k_step_lds<D, S>), grid, dim3(XC), 0, c->stream, a);
         return;
       }
     }

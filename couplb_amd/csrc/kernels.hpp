@@ -382,16 +382,19 @@ __global__ void __launch_bounds__(256) k_step(StepArgs a) {
                   [&](int q, T v) { Store<S>::store(dst + q * a.pc.pitch + node, v); });
 }
 
-// 16-bit formats: the same step with the 19 source rows of a 256-node row
-// chunk staged through LDS in 16-byte groups (and the results written back
-// the same way), so that each global access moves 16 bytes per lane instead
-// of one 2-byte population.  Needs nx % 8 == 0, a plane pitch that keeps
-// the groups aligned (a multiple of 8 elements), and a state in pull form
-// (not the post-import local form).
-constexpr int LDS_XC = 256;
+// Reduced formats: the same step with the 19 source rows of a 256-node row
+// chunk staged through LDS in 16-byte groups, and the results written back
+// the same way (through the same LDS buffer), so that each
+// global access moves 16 bytes per lane instead of one 2- or 4-byte
+// population.  Needs nx and the plane pitch to be multiples of the group
+// (8 or 4 elements) and a state in pull form (not the post-import local
+// form).
 constexpr int LDS_GB = 16;  // bytes per staged group
+// nodes per block
+template <typename S> constexpr int lds_chunk() { return 256; }
 template <int D, typename S>
-__global__ void __launch_bounds__(LDS_XC) k_step_lds(StepArgs a) {
+__global__ void __launch_bounds__(lds_chunk<S>()) k_step_lds(StepArgs a) {
+  constexpr int LDS_XC = lds_chunk<S>();
   using L = Lat<D>;
   using T = typename Store<S>::T;
   typedef unsigned int V __attribute__((ext_vector_type(4)));  // one 16-byte group
@@ -400,8 +403,12 @@ __global__ void __launch_bounds__(LDS_XC) k_step_lds(StepArgs a) {
   constexpr int W = LDS_XC + 2 * G;          // window [x0 - G, x0 + XC + G)
   constexpr int NGR = W / G;                 // groups per staged row (full chunk)
   constexpr int NGO = LDS_XC / G;            // groups per result row (full chunk)
-  __shared__ __attribute__((aligned(16))) S win[Q][W];
-  __shared__ __attribute__((aligned(16))) S res[Q][LDS_XC];
+  // one LDS buffer: the staged windows, then (after every lane has pulled
+  // its populations into registers) the results
+  constexpr int LDS_ELEMS = (Q * W > Q * LDS_XC) ? Q * W : Q * LDS_XC;
+  __shared__ __attribute__((aligned(16))) S lds[LDS_ELEMS];
+  S (*win)[W] = reinterpret_cast<S (*)[W]>(lds);
+  S (*res)[LDS_XC] = reinterpret_cast<S (*)[LDS_XC]>(lds);
   // copies, not references: the lambdas below must not take the address of
   // the kernel argument (that would move it to scratch)
   const Geo g = a.pc.g;
@@ -434,15 +441,16 @@ __global__ void __launch_bounds__(LDS_XC) k_step_lds(StepArgs a) {
   }
   __syncthreads();
   const int t = threadIdx.x;
-  if (t < xc) {
-    const int i = x0 + t;
-    const unsigned node = (unsigned)i + (unsigned)g.nx * ((unsigned)j + (unsigned)g.ny * (unsigned)k);
-    T f[Q];
+  const bool live = t < xc;
+  const int i = x0 + t;
+  const unsigned node = (unsigned)i + (unsigned)g.nx * ((unsigned)j + (unsigned)g.ny * (unsigned)k);
+  T f[Q];
+  if (live)
     pull_node_with<D, S>(a.pc, i, j, k, f, [&](int q, int, int, int) {
       return Store<S>::decode(win[q][t + G - L::ex(q)]);
     });
-    step_node<D, S>(a, i, j, k, node, f, [&](int q, T v) { res[q][t] = Store<S>::encode(v); });
-  }
+  __syncthreads();
+  if (live) step_node<D, S>(a, i, j, k, node, f, [&](int q, T v) { res[q][t] = Store<S>::encode(v); });
   __syncthreads();
   S* dst = static_cast<S*>(a.dst);
   const size_t row = (size_t)(unsigned)g.nx * ((unsigned)j + (unsigned)g.ny * (unsigned)k);

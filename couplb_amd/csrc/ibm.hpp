@@ -178,11 +178,15 @@ __global__ void __launch_bounds__(TILE_THREADS) k_cover(Geo g, Walls w, BaseSlot
       }
 }
 
-// Fluid velocity u = j/rho (compute_macroscopic(false), couplb_lattice.h:
-// 155-202) at covered nodes, once per state (stamp = state counter).  u is
-// stored node-interleaved (ux, uy, uz): a stencil row is one contiguous
-// segment for the marker gathers.
-template <int D, typename S>
+// Fluid velocity u = j/rho and density (compute_macroscopic(false),
+// couplb_lattice.h:155-202) at covered nodes, once per state (stamp = state
+// counter).  Stored node-interleaved (ux, uy, uz, rho): a stencil row is one
+// contiguous segment for the marker gathers.
+// DENSE: every lane pulls (full-row, fully coalesced reads) and only covered
+// nodes are written; for coverage high enough that nearly every cache line
+// of the populations is needed anyway.  Otherwise only covered nodes not yet
+// computed for this state (stamp) pull.
+template <int D, typename S, bool DENSE>
 __global__ void __launch_bounds__(256) k_upull(PullCtx pc, const unsigned char* cover, double* u4,
                                                int* stamp, int cycle) {
   using L = Lat<D>;
@@ -193,7 +197,8 @@ __global__ void __launch_bounds__(256) k_upull(PullCtx pc, const unsigned char* 
   const int j = blockIdx.y, k = blockIdx.z;
   if (i >= g.nx) return;
   const unsigned node = (unsigned)i + (unsigned)g.nx * ((unsigned)j + (unsigned)g.ny * (unsigned)k);
-  if (!cover[node] || stamp[node] == cycle) return;
+  const bool want = cover[node] && (DENSE || stamp[node] != cycle);
+  if (!DENSE && !want) return;
   T f[Q];
   pull_node<D, S>(pc, i, j, k, f);
   double A = 0, jx = 0, jy = 0, jz = 0;
@@ -214,6 +219,7 @@ __global__ void __launch_bounds__(256) k_upull(PullCtx pc, const unsigned char* 
   out.y = jy * inv_r;
   out.z = jz * inv_r;
   out.w = r;
+  if (!want) return;
   *reinterpret_cast<double4*>(u4 + 4 * (size_t)node) = out;
   stamp[node] = cycle;
 }

@@ -9,6 +9,7 @@
 
 #include <hip/hip_runtime.h>
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 #include <rocprim/device/device_select.hpp>
 #include <rocprim/iterator/counting_iterator.hpp>
 
@@ -17,6 +18,7 @@
 #include "ibm.hpp"
 #include "kernels.hpp"
 #include "mdf.hpp"
+#include "mdf_matrix.hpp"
 #include "mlp.hpp"
 #include "spread_pairs.hpp"
 
@@ -112,9 +114,22 @@ struct couplb_ctx {
   double* mdf_part = nullptr;      // block partials
   int mdf_cap = 0;
   MdfScalars* mdf_S = nullptr;
-  MdfScalars* mdf_host = nullptr;  // pinned
+  MdfScalars* mdf_host = nullptr;  // pinned, two slots: chunks of passes alternate
+  hipEvent_t mdf_ev[2] = {nullptr, nullptr};  // end of the last two queued chunks
   float *mlp_x = nullptr, *mlp_y = nullptr;  // network inputs / outputs, sorted order
   double* mlp_sig = nullptr;                 // |slip at F = 0| per sorted marker
+  // explicit marker-by-marker MDF operator (mdf_matrix.hpp)
+  double* mm_gam = nullptr;  // interpolation / spreading weight ratio per sorted marker
+  double* mm_B = nullptr;    // b = V - I(u*) per sorted marker
+  int mm_epoch = -1;         // pair map the matrix was built for (-1: none)
+  int mm_seen = -1;          // pair map of the previous MDF solve
+  bool mm_ok = false;        // built (false: over the size limit, node path)
+  int mm_n = 0, mm_nnz = 0;
+  long long mm_T = 0;        // triples
+  unsigned long long* mm_val = nullptr;  // sorted triples: pair of k << 32 | pair of j
+  int *mm_ent = nullptr, *mm_elen = nullptr, *mm_erow = nullptr, *mm_col = nullptr;
+  int* mm_rowptr = nullptr;
+  double* mm_aval = nullptr;  // entries for the current density
   int ugather_cycle = -1; // state counter of the last velocity gather
   std::vector<SlotHost> slots;
   int nslots = 0;
@@ -585,6 +600,117 @@ int build_pairs(couplb_ctx* c, const SlotHost& sl) {
   return COUPLB_OK;
 }
 
+void mm_free(couplb_ctx* c) {
+  (void)hipFree(c->mm_val); (void)hipFree(c->mm_ent); (void)hipFree(c->mm_elen);
+  (void)hipFree(c->mm_erow); (void)hipFree(c->mm_col); (void)hipFree(c->mm_rowptr);
+  (void)hipFree(c->mm_aval);
+  c->mm_val = nullptr;
+  c->mm_ent = c->mm_elen = c->mm_erow = c->mm_col = c->mm_rowptr = nullptr;
+  c->mm_aval = nullptr;
+  c->mm_ok = false;
+  c->mm_nnz = 0;
+  c->mm_T = 0;
+}
+
+// Largest triple count the explicit operator is built for (32 bytes a triple
+// while building, 8 kept); beyond it the solve stays on the node path.
+constexpr long long MM_MAX_TRIPLES = 1ll << 27;
+
+// Structure of the explicit operator for the current pair map of n sorted
+// markers: triples (k, j, pair of k, pair of j) of every node run, sorted by
+// (k, j); one matrix entry per distinct (k, j).
+int mm_build(couplb_ctx* c, int n) {
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  mm_free(c);
+  c->mm_epoch = c->pair_epoch;
+  c->mm_n = n;
+  const int nh = c->pnh;
+  if (nh <= 0) return COUPLB_OK;
+  struct Tmp {  // build temporaries, freed on every exit
+    hipStream_t s;
+    std::vector<void*> p;
+    ~Tmp() {
+      (void)hipStreamSynchronize(s);
+      for (void* q : p) (void)hipFree(q);
+    }
+  } tmp{c->stream, {}};
+  auto alloc = [&](void** q, size_t bytes) -> int {
+    HIPCHK(c, hipMalloc(q, bytes));
+    tmp.p.push_back(*q);
+    return COUPLB_OK;
+  };
+  const int tb = 256;
+  unsigned long long *cnt = nullptr, *off = nullptr;
+  int rc = alloc((void**)&cnt, 8 * ((size_t)nh + 1));
+  if (!rc) rc = alloc((void**)&off, 8 * ((size_t)nh + 1));
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_mm_count, dim3((unsigned)((nh + tb) / tb)), dim3(tb), 0, c->stream, nh,
+                     (const int*)c->phlen, cnt);
+  size_t bytes = 0;
+  HIPCHK(c, rocprim::exclusive_scan(nullptr, bytes, cnt, off, 0ull, (size_t)nh + 1,
+                                    rocprim::plus<unsigned long long>(), c->stream));
+  void* scan_tmp = nullptr;
+  if ((rc = alloc(&scan_tmp, bytes))) return rc;
+  HIPCHK(c, rocprim::exclusive_scan(scan_tmp, bytes, cnt, off, 0ull, (size_t)nh + 1,
+                                    rocprim::plus<unsigned long long>(), c->stream));
+  unsigned long long T = 0;
+  HIPCHK(c, hipMemcpyAsync(&T, off + nh, sizeof(T), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (T == 0 || T > (unsigned long long)MM_MAX_TRIPLES) return COUPLB_OK;
+  unsigned long long *key = nullptr, *key_s = nullptr, *val = nullptr;
+  unsigned char* flag = nullptr;
+  int *ent = nullptr, *cnt_d = nullptr;
+  if ((rc = alloc((void**)&key, 8 * T)) || (rc = alloc((void**)&key_s, 8 * T)) ||
+      (rc = alloc((void**)&val, 8 * T)) || (rc = alloc((void**)&flag, T)) ||
+      (rc = alloc((void**)&ent, 4 * T)) || (rc = alloc((void**)&cnt_d, sizeof(int))))
+    return rc;
+  HIPCHK(c, hipMalloc(&c->mm_val, 8 * T));
+  const unsigned nbt = (unsigned)((T + tb - 1) / tb);
+  const unsigned long long nn = (unsigned long long)n;
+  hipLaunchKernelGGL(k_mm_emit, dim3((unsigned)((nh + tb - 1) / tb)), dim3(tb), 0, c->stream, nh,
+                     (const int*)c->pheads, (const int*)c->phlen, (const int*)c->pp, nn,
+                     (const unsigned long long*)off, key, val);
+  unsigned bits = 1;
+  while (bits < 64 && (1ull << bits) < nn * nn) bits++;
+  bytes = 0;
+  HIPCHK(c, rocprim::radix_sort_pairs(nullptr, bytes, key, key_s, val, c->mm_val, (size_t)T, 0u, bits,
+                                      c->stream));
+  void* sort_tmp = nullptr;
+  if ((rc = alloc(&sort_tmp, bytes))) return rc;
+  HIPCHK(c, rocprim::radix_sort_pairs(sort_tmp, bytes, key, key_s, val, c->mm_val, (size_t)T, 0u, bits,
+                                      c->stream));
+  hipLaunchKernelGGL(k_mm_heads, dim3(nbt), dim3(tb), 0, c->stream, (long long)T,
+                     (const unsigned long long*)key_s, flag);
+  bytes = 0;
+  HIPCHK(c, rocprim::select(nullptr, bytes, rocprim::counting_iterator<int>(0), flag, ent, cnt_d,
+                            (size_t)T, c->stream));
+  void* sel_tmp = nullptr;
+  if ((rc = alloc(&sel_tmp, bytes))) return rc;
+  HIPCHK(c, rocprim::select(sel_tmp, bytes, rocprim::counting_iterator<int>(0), flag, ent, cnt_d,
+                            (size_t)T, c->stream));
+  int nnz = 0;
+  HIPCHK(c, hipMemcpyAsync(&nnz, cnt_d, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMalloc(&c->mm_ent, sizeof(int) * (size_t)nnz));
+  HIPCHK(c, hipMalloc(&c->mm_elen, sizeof(int) * (size_t)nnz));
+  HIPCHK(c, hipMalloc(&c->mm_erow, sizeof(int) * (size_t)nnz));
+  HIPCHK(c, hipMalloc(&c->mm_col, sizeof(int) * (size_t)nnz));
+  HIPCHK(c, hipMalloc(&c->mm_aval, sizeof(double) * (size_t)nnz));
+  HIPCHK(c, hipMalloc(&c->mm_rowptr, sizeof(int) * ((size_t)n + 1)));
+  HIPCHK(c, hipMemcpyAsync(c->mm_ent, ent, sizeof(int) * (size_t)nnz, hipMemcpyDeviceToDevice, c->stream));
+  hipLaunchKernelGGL(k_mm_entries, dim3((unsigned)((nnz + tb - 1) / tb)), dim3(tb), 0, c->stream, nnz,
+                     (const int*)c->mm_ent, (const unsigned long long*)key_s, (long long)T, nn, c->mm_col,
+                     c->mm_erow, c->mm_elen);
+  hipLaunchKernelGGL(k_mm_rowptr, dim3((unsigned)((n + tb) / tb)), dim3(tb), 0, c->stream, n, nnz,
+                     (const int*)c->mm_erow, c->mm_rowptr);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->mm_ok = true;
+  c->mm_T = (long long)T;
+  c->mm_nnz = nnz;
+  return COUPLB_OK;
+}
+
 int mdf_impl(couplb_ctx* c, int n, const double* x, const double* V, const double* F0,
              double* f_body, const couplb_mdf_args* a, const couplb_mdf_learn* learn,
              couplb_mdf_stats* st) {
@@ -609,19 +735,24 @@ int mdf_impl(couplb_ctx* c, int n, const double* x, const double* V, const doubl
   const int tb = 256, nb = (n + tb - 1) / tb;
   const int nbi = (c->D == 3) ? (4 * n + tb - 1) / tb : nb;
   if (!c->ut) HIPCHK(c, hipMalloc(&c->ut, sizeof(double) * 3 * c->N));
-  if (!c->mdf_S) HIPCHK(c, hipMalloc(&c->mdf_S, sizeof(MdfScalars)));
-  if (!c->mdf_host) HIPCHK(c, hipHostMalloc((void**)&c->mdf_host, sizeof(MdfScalars)));
+  if (!c->mdf_S) HIPCHK(c, hipMalloc(&c->mdf_S, 2 * sizeof(MdfScalars)));  // CG passes alternate
+  if (!c->mdf_host) HIPCHK(c, hipHostMalloc((void**)&c->mdf_host, 2 * sizeof(MdfScalars)));
+  for (auto& e : c->mdf_ev)
+    if (!e) HIPCHK(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
   if (c->mdf_cap < n) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     (void)hipFree(c->mdf_F); (void)hipFree(c->mdf_R); (void)hipFree(c->mdf_Ap); (void)hipFree(c->mdf_part);
-    c->mdf_F = c->mdf_R = c->mdf_Ap = c->mdf_part = nullptr;
+    (void)hipFree(c->mm_gam); (void)hipFree(c->mm_B);
+    c->mdf_F = c->mdf_R = c->mdf_Ap = c->mdf_part = c->mm_gam = c->mm_B = nullptr;
     c->mdf_cap = 0;
     const int cap = std::max(n, c->cap);
-    const int nbmax = (4 * cap + tb - 1) / tb;
+    const int nbmax = (MM_LANES * cap + tb - 1) / tb;  // the widest reduction: k_mm_spmv
     HIPCHK(c, hipMalloc(&c->mdf_F, sizeof(double) * 3 * cap));
     HIPCHK(c, hipMalloc(&c->mdf_R, sizeof(double) * 3 * cap));
     HIPCHK(c, hipMalloc(&c->mdf_Ap, sizeof(double) * 3 * cap));
-    HIPCHK(c, hipMalloc(&c->mdf_part, sizeof(double) * 2 * nbmax));
+    HIPCHK(c, hipMalloc(&c->mdf_part, sizeof(double) * 4 * nbmax));
+    HIPCHK(c, hipMalloc(&c->mm_gam, sizeof(double) * cap));
+    HIPCHK(c, hipMalloc(&c->mm_B, sizeof(double) * 3 * cap));
     (void)hipFree(c->mlp_x); (void)hipFree(c->mlp_y); (void)hipFree(c->mlp_sig);
     c->mlp_x = c->mlp_y = nullptr;
     c->mlp_sig = nullptr;
@@ -631,7 +762,10 @@ int mdf_impl(couplb_ctx* c, int n, const double* x, const double* V, const doubl
     c->mdf_cap = cap;
   }
   double* part_a = c->mdf_part;
-  double* part_b = c->mdf_part + (4 * c->mdf_cap + tb - 1) / tb;
+  const int nbmax = (MM_LANES * c->mdf_cap + tb - 1) / tb;
+  double* part_b = c->mdf_part + nbmax;
+  double* part_x = c->mdf_part + 2 * nbmax;  // operator partials inside a CG pass
+  double* part_y = c->mdf_part + 3 * nbmax;
   const IbmParams ip = make_ibm_params(c, &a->ibm);
   int tslot = 0;
   rc = timer_begin(c, 3, &tslot);
@@ -645,10 +779,10 @@ int mdf_impl(couplb_ctx* c, int n, const double* x, const double* V, const doubl
   const double* F0i = want_x ? nullptr : F0;  // the network path starts from F = 0
   if (c->D == 3)
     hipLaunchKernelGGL(k_mdf_init<3>, dim3(nb), dim3(tb), 0, c->stream, n, c->perm, x, V, F0i, ip, c->geo,
-                       c->walls, sl.rec, c->mdf_v, c->mdf_F, c->fsrc);
+                       c->walls, sl.rec, c->mdf_v, c->mdf_F, c->fsrc, c->mm_gam);
   else
     hipLaunchKernelGGL(k_mdf_init<2>, dim3(nb), dim3(tb), 0, c->stream, n, c->perm, x, V, F0i, ip, c->geo,
-                       c->walls, sl.rec, c->mdf_v, c->mdf_F, c->fsrc);
+                       c->walls, sl.rec, c->mdf_v, c->mdf_F, c->fsrc, c->mm_gam);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipMemcpyAsync(sl.rowstart, c->rowstart_cur, sizeof(int) * (c->geo.nrows + 1),
                            hipMemcpyDeviceToDevice, c->stream));
@@ -659,26 +793,63 @@ int mdf_impl(couplb_ctx* c, int n, const double* x, const double* V, const doubl
     if (rc) return rc;
   }
   const unsigned nbp = (unsigned)std::max(1, (c->pnh + 255) / 256);
-  // node values ut = [u*] + S(P)/(2 rho), then the interpolation pass
-  auto apply = [&](int affine) {
+  // The explicit operator (mdf_matrix.hpp) when the pair map is reused from
+  // the previous solve (fixed markers): its build is a sort of all triples,
+  // paid back over the solves of later steps.  COUPLB_MDF_MATRIX=0 keeps the
+  // node path, =1 builds it at the first solve of a pair map.
+  const char* mm_env = getenv("COUPLB_MDF_MATRIX");
+  const int mm_mode = mm_env ? atoi(mm_env) : -1;
+  bool use_mm = false;
+  if (mm_mode != 0 && c->pnh > 0 && (mm_mode == 1 || c->mm_seen == c->pair_epoch)) {
+    if (c->mm_epoch != c->pair_epoch || c->mm_n != n) {
+      rc = mm_build(c, n);
+      if (rc) return rc;
+    }
+    use_mm = c->mm_ok;
+  }
+  c->mm_seen = c->pair_epoch;  // pairs rebuilt for one ordering are the same pairs
+  const int nbs = (MM_LANES * n + MDF_THREADS - 1) / MDF_THREADS;
+  const int nbr = use_mm ? nbs : nbi;  // partials of a residual / p.Ap pass
+  // node values ut = [u*] + S(P)/(2 rho), then the interpolation pass:
+  // out = V - I(ut) (affine) or I0(ut); P null stands for zero forces
+  auto node_apply = [&](int affine, const double* P, double* out, const MdfScalars* Sx, double* pa,
+                        double* pb) {
     hipLaunchKernelGGL(k_mdf_ut, dim3(nbp), dim3(256), 0, c->stream, c->pnh, (const int*)c->pheads,
                        (const int*)c->phlen, (const unsigned*)c->phnode, (const int*)c->pp,
-                       (const double*)c->pw, (const double*)c->fsrc, (const double*)c->u4, affine, c->ut,
-                       (const MdfScalars*)Sd);
-    double* out = affine ? c->mdf_R : c->mdf_Ap;
+                       (const double*)c->pw, P, (const double*)c->u4, affine, c->ut, Sx);
     if (c->D == 3)
       hipLaunchKernelGGL(k_mdf_interp<3>, dim3(nbi), dim3(MDF_THREADS), 0, c->stream, n, c->geo, c->walls,
                          (const double*)c->ut, affine, (const MarkerRec*)sl.rec, (const double*)c->mdf_v,
-                         (const double*)c->fsrc, out, part_a, part_b, (const MdfScalars*)Sd);
+                         (const double*)c->fsrc, out, pa, pb, Sx);
     else
       hipLaunchKernelGGL(k_mdf_interp<2>, dim3(nbi), dim3(MDF_THREADS), 0, c->stream, n, c->geo, c->walls,
                          (const double*)c->ut, affine, (const MarkerRec*)sl.rec, (const double*)c->mdf_v,
-                         (const double*)c->fsrc, out, part_a, part_b, (const MdfScalars*)Sd);
+                         (const double*)c->fsrc, out, pa, pb, Sx);
   };
-  auto scalar = [&](int phase, int nblk, int first, int respect) {
+  // residual R = b - A F (affine, F = rec.f = fsrc on the node path, Fm on the
+  // matrix path) or Ap = A P (P = fsrc), with the block partials in pa, pb
+  auto apply = [&](int affine, const MdfScalars* Sx, double* pa, double* pb) {
+    if (use_mm)
+      hipLaunchKernelGGL(k_mm_spmv, dim3(nbs), dim3(MDF_THREADS), 0, c->stream, n,
+                         (const int*)c->mm_rowptr, (const int*)c->mm_col, (const double*)c->mm_aval,
+                         (const double*)(affine ? c->mdf_F : c->fsrc), (const double*)c->mm_B,
+                         affine ? c->mdf_R : c->mdf_Ap, affine, pa, pb, Sx);
+    else
+      node_apply(affine, c->fsrc, affine ? c->mdf_R : c->mdf_Ap, Sx, pa, pb);
+  };
+  if (use_mm) {
+    // entries for this density, and b = V - I(u*) on the node path
+    hipLaunchKernelGGL(k_mm_values, dim3((unsigned)((c->mm_nnz + 255) / 256)), dim3(256), 0, c->stream,
+                       c->mm_nnz, (const int*)c->mm_ent, (const int*)c->mm_elen, (const int*)c->mm_erow,
+                       (const unsigned long long*)c->mm_val, (const double*)c->pw,
+                       (const unsigned*)c->pkey_s, (const double*)c->u4, (const double*)c->mm_gam,
+                       c->mm_aval);
+    node_apply(1, nullptr, c->mm_B, Sd, part_a, part_b);
+    HIPCHK(c, hipGetLastError());
+  }
+  auto scalar = [&](int nblk, int first, int respect, MdfScalars* mirror) {
     hipLaunchKernelGGL(k_mdf_scalar, dim3(1), dim3(MDF_THREADS), 0, c->stream, (const double*)part_a,
-                       (const double*)part_b, nblk, phase, a->tol, first, a->max_iter, respect, Sd,
-                       c->mdf_host);
+                       (const double*)part_b, nblk, a->tol, first, a->max_iter, respect, Sd, mirror);
   };
   auto read = [&]() -> int {
     HIPCHK(c, hipGetLastError());
@@ -690,11 +861,37 @@ int mdf_impl(couplb_ctx* c, int n, const double* x, const double* V, const doubl
     hipLaunchKernelGGL(k_mdf_setf, dim3(nb), dim3(tb), 0, c->stream, n, (const double*)c->mdf_F, sl.rec,
                        c->fsrc);
     hipLaunchKernelGGL(k_mdf_reset, dim3(1), dim3(1), 0, c->stream, Sd, 0);
-    apply(1);
-    scalar(MDF_RESID, nbi, first, 0);
+    apply(1, Sd, part_a, part_b);
+    scalar(nbr, first, 0, c->mdf_host);
     return read();
   };
   constexpr int CHUNK = 8;  // iterations queued between two host checks
+  static_assert(CHUNK % 2 == 0, "a CG chunk ends on the first scalar buffer");
+  // Chunks of passes until the device sets done: chunk j + 1 is queued before
+  // the host waits for chunk j, so the device does not idle on the check; the
+  // passes of a chunk queued after convergence return at once.  enqueue(m)
+  // queues one chunk whose last kernel writes the scalars to mirror slot m.
+  // Ends with the final scalars in slot 0.
+  auto run_chunks = [&](auto&& enqueue) -> int {
+    MdfScalars* m = c->mdf_host;
+    enqueue(m + 0);
+    HIPCHK(c, hipEventRecord(c->mdf_ev[0], c->stream));
+    for (int j = 0;; j++) {
+      enqueue(m + ((j + 1) & 1));
+      HIPCHK(c, hipGetLastError());
+      HIPCHK(c, hipEventRecord(c->mdf_ev[(j + 1) & 1], c->stream));
+      HIPCHK(c, hipEventSynchronize(c->mdf_ev[j & 1]));
+      if (m[j & 1].done) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        m[0] = m[j & 1];
+        return COUPLB_OK;
+      }
+      if (j > a->max_iter / CHUNK + 4) {  // every pass counts, so max_iter ends the loop first
+        (void)hipStreamSynchronize(c->stream);
+        return fail(c, COUPLB_ESTATE, "MDF iteration did not end");
+      }
+    }
+  };
   rc = true_residual(1);
   if (rc) return rc;
   double slip_zero = -1.0;
@@ -726,22 +923,23 @@ int mdf_impl(couplb_ctx* c, int n, const double* x, const double* V, const doubl
   int method = a->method;
   while (c->mdf_host->done == 0) {
     if (method == COUPLB_MDF_CG) {
-      hipLaunchKernelGGL(k_cg_direction, dim3(nb), dim3(tb), 0, c->stream, n, (const double*)c->mdf_R,
-                         (const MarkerRec*)sl.rec, c->fsrc, (const MdfScalars*)Sd, 1);
-      do {
+      hipLaunchKernelGGL(k_cg_restart, dim3((unsigned)((3 * n + tb - 1) / tb)), dim3(tb), 0, c->stream, n,
+                         (const double*)c->mdf_R, c->fsrc);
+      // pass k reads the scalars Sd[k % 2] and writes Sd[(k + 1) % 2]
+      rc = run_chunks([&](MdfScalars* mirror) {
         for (int k = 0; k < CHUNK; k++) {
-          apply(0);
-          scalar(MDF_ALPHA, nbi, 0, 1);
+          MdfScalars* Sk = Sd + (k & 1);
+          MdfScalars* Sn = Sd + ((k + 1) & 1);
+          apply(0, Sk, part_x, part_y);
           hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(MDF_THREADS), 0, c->stream, n,
-                             (const MarkerRec*)sl.rec, (const double*)c->fsrc, (const double*)c->mdf_Ap,
-                             c->mdf_F, c->mdf_R, part_a, part_b, (const MdfScalars*)Sd);
-          scalar(MDF_BETA, nb, 0, 1);
-          hipLaunchKernelGGL(k_cg_direction, dim3(nb), dim3(tb), 0, c->stream, n, (const double*)c->mdf_R,
-                             (const MarkerRec*)sl.rec, c->fsrc, (const MdfScalars*)Sd, 0);
+                             (const double*)c->fsrc, (const double*)c->mdf_Ap, c->mdf_F, c->mdf_R,
+                             (const double*)part_x, nbr, part_a, part_b, Sk);
+          hipLaunchKernelGGL(k_cg_next, dim3(nb), dim3(MDF_THREADS), 0, c->stream, n, (const double*)c->mdf_R,
+                             c->fsrc, (const double*)part_a, (const double*)part_b, nb, a->tol, a->max_iter,
+                             (const MdfScalars*)Sk, Sn, k == CHUNK - 1 ? mirror : nullptr);
         }
-        rc = read();
-        if (rc) return rc;
-      } while (c->mdf_host->done == 0);
+      });
+      if (rc) return rc;
       const int why = c->mdf_host->done;
       if (c->mdf_host->iters >= a->max_iter) {
         rc = true_residual(0);
@@ -752,16 +950,15 @@ int mdf_impl(couplb_ctx* c, int n, const double* x, const double* V, const doubl
       if (rc) return rc;
       if (c->mdf_host->done == 0 && why == 2) method = COUPLB_MDF_RICHARDSON;
     } else {
-      do {
+      rc = run_chunks([&](MdfScalars* mirror) {
         for (int k = 0; k < CHUNK; k++) {
           hipLaunchKernelGGL(k_mdf_relax, dim3(nb), dim3(tb), 0, c->stream, n, (const double*)c->mdf_R,
                              a->relax, c->mdf_F, (const MarkerRec*)sl.rec, c->fsrc, (const MdfScalars*)Sd);
-          apply(1);
-          scalar(MDF_RESID, nbi, 0, 1);
+          apply(1, Sd, part_a, part_b);
+          scalar(nbr, 0, 1, k == CHUNK - 1 ? mirror : nullptr);
         }
-        rc = read();
-        if (rc) return rc;
-      } while (c->mdf_host->done == 0);
+      });
+      if (rc) return rc;
     }
   }
   const MdfScalars& hs = *c->mdf_host;
@@ -999,7 +1196,11 @@ int couplb_destroy(couplb_ctx* c) {
   (void)hipFree(c->ut); (void)hipFree(c->mdf_v); (void)hipFree(c->mdf_S);
   (void)hipFree(c->mdf_F); (void)hipFree(c->mdf_R); (void)hipFree(c->mdf_Ap); (void)hipFree(c->mdf_part);
   (void)hipFree(c->mlp_x); (void)hipFree(c->mlp_y); (void)hipFree(c->mlp_sig);
+  (void)hipFree(c->mm_gam); (void)hipFree(c->mm_B);
+  mm_free(c);
   if (c->mdf_host) (void)hipHostFree(c->mdf_host);
+  for (auto& e : c->mdf_ev)
+    if (e) (void)hipEventDestroy(e);
   for (auto& s : c->slots) { (void)hipFree(s.rowstart); (void)hipFree(s.rec); }
   (void)hipFree(c->slot_table);
   for (int e = 0; e < TimerRing::R; e++) {

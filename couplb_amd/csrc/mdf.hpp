@@ -63,7 +63,7 @@ template <int D>
 __global__ void __launch_bounds__(256) k_mdf_init(int n, const int* perm, const double* x,
                                                   const double* V, const double* F0, IbmParams ip,
                                                   Geo g, Walls w, MarkerRec* rec, double* Vs,
-                                                  double* Fm, double* P) {
+                                                  double* Fm, double* P, double* gam) {
 #pragma clang fp contract(off)
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
@@ -84,12 +84,12 @@ __global__ void __launch_bounds__(256) k_mdf_init(int n, const int* perm, const 
   }
   // fluid weight sum per stencil plane, planes combined in order: the same
   // rounding as the four-lane sums of k_marker
-  double wsf = 0.0;
+  double wsf = 0.0, wsa = 0.0;  // fluid weight, weight of every counted node
   if (fin) {
 #pragma unroll
     for (int v = 0; v < ((D == 3) ? 4 : 1); v++) {
       const int c = (D == 3) ? r.k0 - 1 + v : 0;
-      double pl = 0.0;
+      double pl = 0.0, pla = 0.0;
       if (!(D == 3 && (c < 0 || c > g.nz + 1))) {
 #pragma unroll
         for (int u = 0; u < 4; u++) {
@@ -100,12 +100,15 @@ __global__ void __launch_bounds__(256) k_mdf_init(int n, const int* perm, const 
             const int a = r.i0 - 1 + t;
             if (a < 0 || a > g.nx + 1) continue;
             int wd, wsd;
-            if (node_type(w, g, a, b, c, wd, wsd) != 0) continue;
-            pl += (D == 3) ? (r.wx[t] * r.wy[u]) * r.wz[v] : r.wx[t] * r.wy[u];
+            const int nt = node_type(w, g, a, b, c, wd, wsd);
+            const double d = (D == 3) ? (r.wx[t] * r.wy[u]) * r.wz[v] : r.wx[t] * r.wy[u];
+            if (nt == 0) pl += d;
+            if (nt >= 0 && nt <= 3) pla += d;  // IBM::interpolate counts types 0-3
           }
         }
       }
       wsf += pl;
+      wsa += pla;
     }
   }
   r.valid = (fin && !(wsf < DELTA_TOL)) ? 1 : 0;
@@ -114,6 +117,7 @@ __global__ void __launch_bounds__(256) k_mdf_init(int n, const int* perm, const 
   r.fy = (r.valid && F0) ? F0[3 * (size_t)idx + 1] : 0.0;
   r.fz = (r.valid && F0) ? F0[3 * (size_t)idx + 2] : 0.0;
   rec[p] = r;
+  gam[p] = r.valid ? wsf / wsa : 0.0;  // turns spreading weights into interpolation weights
   Fm[3 * (size_t)p + 0] = P[3 * (size_t)p + 0] = r.fx;
   Fm[3 * (size_t)p + 1] = P[3 * (size_t)p + 1] = r.fy;
   Fm[3 * (size_t)p + 2] = P[3 * (size_t)p + 2] = r.fz;
@@ -130,8 +134,8 @@ __global__ void __launch_bounds__(256) k_mdf_ut(int nh, const int* heads, const 
   if (S->done) return;
   const int h = blockIdx.x * blockDim.x + threadIdx.x;
   if (h >= nh) return;
-  double ax, ay, az;
-  run_sum(heads[h], hlen[h], pp, pw, P, ax, ay, az);
+  double ax = 0.0, ay = 0.0, az = 0.0;
+  if (P) run_sum(heads[h], hlen[h], pp, pw, P, ax, ay, az);  // P null: F = 0
   const unsigned k = hnode[h];
   const double4 q = *reinterpret_cast<const double4*>(u4 + 4 * (size_t)k);
   const double hf = 0.5 / q.w;
@@ -205,69 +209,80 @@ __global__ void __launch_bounds__(MDF_THREADS) k_mdf_interp(int n, Geo g, Walls 
   }
 }
 
-// One block: fold the block partials (fixed order) into the scalars.
-//   phase 0  true residual:   rr, smax (first time also smax_first);
-//                              done = smax < tol; counts one application
-//   phase 1  alpha:            pAp; alpha = rr/pAp, done = 2 if pAp <= 0
-//   phase 2  beta:             rr' ; beta = rr'/rr, rr = rr', smax;
-//                              done = smax < tol; counts one application
-enum { MDF_RESID = 0, MDF_ALPHA = 1, MDF_BETA = 2 };
-//   Phases 0 and 2 set done = 3 once maxit applications are reached.
-//   respect_done: phase 0 also returns at once when done is set (loops).
-//   mirror: copy of the scalars in pinned host memory, read by the host
-//   after a stream synchronisation without a device-to-host copy.
+// Partials folded in a fixed order: thread t sums entries t, t + 256, ...,
+// then the block tree of block_reduce.  Every block that folds the same
+// partials finds the same value.
+__device__ inline double fold_sum(const double* part, int nblk, double* sh) {
+  double a = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += MDF_THREADS) a += part[i];
+  return block_reduce(a, 0, sh);
+}
+__device__ inline double fold_max(const double* part, int nblk, double* sh) {
+  double b = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += MDF_THREADS) b = b > part[i] ? b : part[i];
+  return block_reduce(b, 1, sh);
+}
+
+// One block: the true residual's partials into the scalars: rr, smax (the
+// first time also smax_first); done = 1 when smax < tol, else 3 once maxit
+// operator applications are counted.  respect_done: when done is already
+// set (inside the Richardson loop), only copy the scalars.  mirror: copy of the
+// scalars in pinned host memory, read by the host after a stream
+// synchronisation without a device-to-host copy.
 __global__ void __launch_bounds__(MDF_THREADS) k_mdf_scalar(const double* part_a,
                                                             const double* part_b, int nblk,
-                                                            int phase, double tol, int first,
-                                                            int maxit, int respect_done,
-                                                            MdfScalars* S, MdfScalars* mirror) {
+                                                            double tol, int first, int maxit,
+                                                            int respect_done, MdfScalars* S,
+                                                            MdfScalars* mirror) {
   __shared__ double sh[MDF_THREADS];
-  if ((phase != MDF_RESID || respect_done) && S->done) return;
-  double a = 0.0, b = 0.0;
-  for (int i = threadIdx.x; i < nblk; i += MDF_THREADS) {
-    a += part_a[i];
-    if (phase != MDF_ALPHA) b = b > part_b[i] ? b : part_b[i];
+  if (respect_done && S->done) {
+    if (threadIdx.x == 0 && mirror) *mirror = *S;
+    return;
   }
-  a = block_reduce(a, 0, sh);
-  b = block_reduce(b, 1, sh);
+  const double a = fold_sum(part_a, nblk, sh);
+  const double b = fold_max(part_b, nblk, sh);
   if (threadIdx.x != 0) return;
-  if (phase == MDF_RESID) {
-    S->rr = a;
-    S->smax = b;
-    if (first) S->smax_first = b;
-    S->done = (b < tol) ? 1 : 0;
-    S->iters += 1;
-    if (!S->done && S->iters >= maxit) S->done = 3;
-  } else if (phase == MDF_ALPHA) {
-    S->pAp = a;
-    if (!(a > 0.0)) S->done = 2;
-    else S->alpha = S->rr / a;
-  } else {
-    S->beta = a / S->rr;
-    S->rr = a;
-    S->smax = b;
-    S->iters += 1;
-    if (b < tol) S->done = 1;
-    else if (S->iters >= maxit) S->done = 3;
-  }
+  S->rr = a;
+  S->smax = b;
+  if (first) S->smax_first = b;
+  S->done = (b < tol) ? 1 : 0;
+  S->iters += 1;
+  if (!S->done && S->iters >= maxit) S->done = 3;
   if (mirror) *mirror = *S;
 }
 
-// F += alpha p, r -= alpha Ap; block partials of r.r and max |r|
-__global__ void __launch_bounds__(MDF_THREADS) k_cg_update(int n, const MarkerRec* rec,
-                                                           const double* P, const double* Ap,
-                                                           double* Fm, double* R, double* part_a,
-                                                           double* part_b, const MdfScalars* S) {
+// Conjugate-gradient pass in three kernels: the operator (Ap = A p and the
+// block partials of p.Ap), k_cg_update and k_cg_next.  A pass reads the
+// scalars of its start, Sk, and k_cg_next writes the whole state after it
+// into Sn; passes alternate between two buffers, so no block reads a field
+// that another block of the same kernel writes.  Every block folds the
+// partials it needs itself (fold_sum / fold_max), so all blocks agree on
+// alpha and beta.  Markers without a valid stencil hold r = p = Ap = 0 and
+// stay unchanged.
+
+// alpha = rr / p.Ap; F += alpha p, r -= alpha Ap; block partials of r.r and
+// max |r|.  p.Ap <= 0 (breakdown) leaves F and r and is seen by k_cg_next.
+__global__ void __launch_bounds__(MDF_THREADS) k_cg_update(int n, const double* P, const double* Ap,
+                                                           double* Fm, double* R, const double* px,
+                                                           int npx, double* part_a, double* part_b,
+                                                           MdfScalars* Sk) {
   __shared__ double sh[MDF_THREADS];
-  if (S->done) return;
+  if (Sk->done) return;
+  const double rr = Sk->rr;
+  const double pAp = fold_sum(px, npx, sh);
+  const bool ok = pAp > 0.0;
+  const double al = ok ? rr / pAp : 0.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    Sk->pAp = pAp;  // fields no block of this kernel reads
+    Sk->alpha = al;
+  }
+  if (!ok) return;
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   double a = 0.0, b = 0.0;
-  if (p < n && rec[p].valid) {
-    const double al = S->alpha;
-    const double pv[3] = {P[3 * (size_t)p + 0], P[3 * (size_t)p + 1], P[3 * (size_t)p + 2]};
+  if (p < n) {
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-      Fm[3 * (size_t)p + c] += al * pv[c];
+      Fm[3 * (size_t)p + c] += al * P[3 * (size_t)p + c];
       const double rc = R[3 * (size_t)p + c] - al * Ap[3 * (size_t)p + c];
       R[3 * (size_t)p + c] = rc;
       a += rc * rc;
@@ -282,15 +297,43 @@ __global__ void __launch_bounds__(MDF_THREADS) k_cg_update(int n, const MarkerRe
   }
 }
 
-// p = r + beta p (beta = 0: restart with p = r)
-__global__ void __launch_bounds__(256) k_cg_direction(int n, const double* R, const MarkerRec* rec,
-                                                      double* P, const MdfScalars* S, int restart) {
-  if (!restart && S->done) return;
+// rr' and smax from k_cg_update's partials; done = 2 after a breakdown, 1
+// when smax < tol, 3 at maxit applications; beta = rr'/rr and p = r + beta p
+// while running.  Block 0 writes the new state to Sn (and the mirror).
+__global__ void __launch_bounds__(MDF_THREADS) k_cg_next(int n, const double* R, double* P,
+                                                         const double* part_a, const double* part_b,
+                                                         int nblk, double tol, int maxit,
+                                                         const MdfScalars* Sk, MdfScalars* Sn,
+                                                         MdfScalars* mirror) {
+  __shared__ double sh[MDF_THREADS];
+  MdfScalars s = *Sk;
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  if (!s.done && !(s.pAp > 0.0)) s.done = 2;
+  if (!s.done) {
+    const double rr = fold_sum(part_a, nblk, sh);
+    const double smax = fold_max(part_b, nblk, sh);
+    s.beta = rr / s.rr;
+    s.rr = rr;
+    s.smax = smax;
+    s.iters += 1;
+    if (smax < tol) s.done = 1;
+    else if (s.iters >= maxit) s.done = 3;
+  }
+  if (lead) {
+    *Sn = s;
+    if (mirror) *mirror = s;
+  }
+  if (s.done) return;
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n || !rec[p].valid) return;
-  const double be = restart ? 0.0 : S->beta;
+  if (p >= n) return;
 #pragma unroll
-  for (int c = 0; c < 3; c++) P[3 * (size_t)p + c] = R[3 * (size_t)p + c] + be * P[3 * (size_t)p + c];
+  for (int c = 0; c < 3; c++) P[3 * (size_t)p + c] = R[3 * (size_t)p + c] + s.beta * P[3 * (size_t)p + c];
+}
+
+// restart: p = r
+__global__ void __launch_bounds__(256) k_cg_restart(int n, const double* R, double* P) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 3 * n) P[i] = R[i];
 }
 
 // Richardson step F += 2 omega r (after a true-residual evaluation that did

@@ -47,17 +47,24 @@ def _oracle_grid(d):
     return g
 
 
+# "node": spreading + interpolation over the grid per pass; "matrix": the
+# explicit marker-by-marker operator (built here at the first solve)
+OPERATORS = {"node": "0", "matrix": "1"}
+
+
+@pytest.mark.parametrize("operator", list(OPERATORS))
 @pytest.mark.parametrize("kernel", ["roma", "peskin4"])
-def test_mdf_richardson_matches_oracle_and_next_step(kernel):
+def test_mdf_richardson_matches_oracle_and_next_step(kernel, operator, monkeypatch):
     # a fixed number of Richardson passes: the device and the oracle apply the
     # same operator the same number of times, so forces agree to rounding
     import couplb_amd
+    monkeypatch.setenv("COUPLB_MDF_MATRIX", OPERATORS[operator])
     kern = couplb_amd.ROMA if kernel == "roma" else couplb_amd.PESKIN4
     d = developed_device(kern)
     x, V = markers()
     n = len(x)
     dx, dV, fb = DeviceArray(x), DeviceArray(V), DeviceArray(np.zeros((n, 3)))
-    print("case MDF Richardson vs oracle", kernel, flush=True)
+    print("case MDF Richardson vs oracle", kernel, operator, flush=True)
     st = d.mdf_device(n, dx.addr, dV.addr, None, fb.addr, (0, 0, 0), 1.0, 1.0, 1.0,
                       tol=1e-30, relax=1.0, max_iter=40, method="richardson")
     F_dev = -fb.download()
@@ -83,15 +90,17 @@ def test_mdf_richardson_matches_oracle_and_next_step(kernel):
     assert rel_l2(fd, fo) < 1e-12
 
 
+@pytest.mark.parametrize("operator", list(OPERATORS))
 @pytest.mark.parametrize("kernel", ["roma", "peskin4"])
-def test_mdf_cg_converges_like_oracle(kernel):
+def test_mdf_cg_converges_like_oracle(kernel, operator, monkeypatch):
     import couplb_amd
+    monkeypatch.setenv("COUPLB_MDF_MATRIX", OPERATORS[operator])
     kern = couplb_amd.ROMA if kernel == "roma" else couplb_amd.PESKIN4
     d = developed_device(kern)
     x, V = markers()
     n = len(x)
     dx, dV, fb = DeviceArray(x), DeviceArray(V), DeviceArray(np.zeros((n, 3)))
-    print("case MDF CG vs oracle", kernel, flush=True)
+    print("case MDF CG vs oracle", kernel, operator, flush=True)
     st = d.mdf_device(n, dx.addr, dV.addr, None, fb.addr, (0, 0, 0), 1.0, 1.0, 1.0,
                       tol=1e-10, max_iter=400, method="cg")
     F_dev = -fb.download()
@@ -147,3 +156,34 @@ def test_mdf_momentum_balance_periodic():
     m1 = d.diagnostics()["mom"]
     print("dmom", m1 - m0, "sum F", Ftot)
     assert np.max(np.abs((m1 - m0) - Ftot)) < 1e-12 * max(1.0, np.abs(Ftot).max()) * 1e3
+
+
+def test_mdf_matrix_reuse_over_steps_matches_node_path(monkeypatch):
+    # fixed markers over several LB steps: by default the second solve of a
+    # pair map builds the explicit operator and later solves reuse it with
+    # the new density; the result follows the node path to rounding
+    import couplb_amd
+    x, V = markers()
+    n = len(x)
+    runs = {}
+    for mode, env in (("node", "0"), ("auto", None)):
+        if env is None:
+            monkeypatch.delenv("COUPLB_MDF_MATRIX", raising=False)
+        else:
+            monkeypatch.setenv("COUPLB_MDF_MATRIX", env)
+        print("case MDF operator reuse", mode, flush=True)
+        d = developed_device(couplb_amd.PESKIN4)
+        dx, dV, fb = DeviceArray(x), DeviceArray(V), DeviceArray(np.zeros((n, 3)))
+        forces = []
+        for step in range(4):
+            st = d.mdf_device(n, dx.addr, dV.addr, None, fb.addr, (0, 0, 0), 1.0, 1.0, 1.0,
+                              tol=1e-30, relax=1.0, max_iter=30, method="richardson",
+                              positions_unchanged=step > 0)
+            assert st["iterations"] == 30
+            forces.append(fb.download())
+            d.step((0.0, 0.0, 0.0), 0)
+        d.sync()
+        runs[mode] = (forces, interior(d.get_state(), 3))
+    for a, b in zip(runs["node"][0], runs["auto"][0]):
+        assert rel_l2(a, b) < 1e-11
+    assert rel_l2(runs["node"][1], runs["auto"][1]) < 1e-13

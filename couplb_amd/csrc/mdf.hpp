@@ -39,18 +39,22 @@ struct MdfScalars {
   int iters;   // operator applications counted by the loop kernels
 };
 
-// block reduction of one value per thread: sum (op 0) or max (op 1), fixed order
+// block reduction of one value per thread: sum (op 0) or max (op 1), fixed
+// order.  A butterfly inside each 64-lane wave (the two lanes of a step add
+// the same two numbers, so every lane ends with the same bits), then the
+// wave results in wave order.  The block has MDF_THREADS threads; sh holds
+// one value per wave.
 __device__ inline double block_reduce(double v, int op, double* sh) {
-  sh[threadIdx.x] = v;
-  __syncthreads();
-  for (int h = MDF_THREADS / 2; h > 0; h >>= 1) {
-    if ((int)threadIdx.x < h) {
-      const double o = sh[threadIdx.x + h];
-      sh[threadIdx.x] = op ? (sh[threadIdx.x] > o ? sh[threadIdx.x] : o) : sh[threadIdx.x] + o;
-    }
-    __syncthreads();
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const double w = __shfl_xor(v, o, 64);
+    v = op ? (v > w ? v : w) : v + w;
   }
-  const double r = sh[0];
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double r = sh[0];
+#pragma unroll
+  for (int i = 1; i < MDF_THREADS / 64; i++) r = op ? (r > sh[i] ? r : sh[i]) : r + sh[i];
   __syncthreads();
   return r;
 }
@@ -154,7 +158,7 @@ __global__ void __launch_bounds__(MDF_THREADS) k_mdf_interp(int n, Geo g, Walls 
                                                             double* out,
                                                             double* part_a, double* part_b,
                                                             const MdfScalars* S) {
-  __shared__ double sh[MDF_THREADS];
+  __shared__ double sh[MDF_THREADS / 64];
   if (S->done) return;
   constexpr int LPM = (D == 3) ? 4 : 1;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -212,14 +216,33 @@ __global__ void __launch_bounds__(MDF_THREADS) k_mdf_interp(int n, Geo g, Walls 
 // Partials folded in a fixed order: thread t sums entries t, t + 256, ...,
 // then the block tree of block_reduce.  Every block that folds the same
 // partials finds the same value.
+// (eight loads in flight per thread)
 __device__ inline double fold_sum(const double* part, int nblk, double* sh) {
   double a = 0.0;
-  for (int i = threadIdx.x; i < nblk; i += MDF_THREADS) a += part[i];
+  for (int i0 = threadIdx.x; i0 < nblk; i0 += 8 * MDF_THREADS) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int i = i0 + u * MDF_THREADS;
+      v[u] = i < nblk ? part[i] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) a += v[u];
+  }
   return block_reduce(a, 0, sh);
 }
 __device__ inline double fold_max(const double* part, int nblk, double* sh) {
   double b = 0.0;
-  for (int i = threadIdx.x; i < nblk; i += MDF_THREADS) b = b > part[i] ? b : part[i];
+  for (int i0 = threadIdx.x; i0 < nblk; i0 += 8 * MDF_THREADS) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int i = i0 + u * MDF_THREADS;
+      v[u] = i < nblk ? part[i] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) b = b > v[u] ? b : v[u];
+  }
   return block_reduce(b, 1, sh);
 }
 
@@ -234,7 +257,7 @@ __global__ void __launch_bounds__(MDF_THREADS) k_mdf_scalar(const double* part_a
                                                             double tol, int first, int maxit,
                                                             int respect_done, MdfScalars* S,
                                                             MdfScalars* mirror) {
-  __shared__ double sh[MDF_THREADS];
+  __shared__ double sh[MDF_THREADS / 64];
   if (respect_done && S->done) {
     if (threadIdx.x == 0 && mirror) *mirror = *S;
     return;
@@ -266,7 +289,7 @@ __global__ void __launch_bounds__(MDF_THREADS) k_cg_update(int n, const double* 
                                                            double* Fm, double* R, const double* px,
                                                            int npx, double* part_a, double* part_b,
                                                            MdfScalars* Sk) {
-  __shared__ double sh[MDF_THREADS];
+  __shared__ double sh[MDF_THREADS / 64];
   if (Sk->done) return;
   const double rr = Sk->rr;
   const double pAp = fold_sum(px, npx, sh);
@@ -305,7 +328,7 @@ __global__ void __launch_bounds__(MDF_THREADS) k_cg_next(int n, const double* R,
                                                          int nblk, double tol, int maxit,
                                                          const MdfScalars* Sk, MdfScalars* Sn,
                                                          MdfScalars* mirror) {
-  __shared__ double sh[MDF_THREADS];
+  __shared__ double sh[MDF_THREADS / 64];
   MdfScalars s = *Sk;
   const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
   if (!s.done && !(s.pAp > 0.0)) s.done = 2;

@@ -105,19 +105,30 @@ __global__ void __launch_bounds__(MDF_THREADS) k_mm_spmv(int n, const int* rowpt
                                                          const double* B, double* y, int mode,
                                                          double* part_a, double* part_b,
                                                          const MdfScalars* S) {
-  __shared__ double sh[MDF_THREADS];
+  __shared__ double sh[MDF_THREADS / 64];
   if (S->done) return;
   const int k = (blockIdx.x * blockDim.x + threadIdx.x) / MM_LANES;
   const int l = threadIdx.x % MM_LANES;
   double s0 = 0.0, s1 = 0.0, s2 = 0.0;
   if (k < n) {
+    // lane l takes entries l, l + 8, ... of the row, four loads in flight
     const int e1 = rowptr[k + 1];
-    for (int e = rowptr[k] + l; e < e1; e += MM_LANES) {
-      const double v = aval[e];
-      const int c = col[e];
-      s0 += v * x[3 * (size_t)c + 0];
-      s1 += v * x[3 * (size_t)c + 1];
-      s2 += v * x[3 * (size_t)c + 2];
+    for (int e = rowptr[k] + l; e < e1; e += 4 * MM_LANES) {
+      double v[4];
+      int cc[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int ei = e + i * MM_LANES;
+        const bool in = ei < e1;
+        v[i] = in ? aval[ei] : 0.0;
+        cc[i] = in ? col[ei] : k;
+      }
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        s0 += v[i] * x[3 * (size_t)cc[i] + 0];
+        s1 += v[i] * x[3 * (size_t)cc[i] + 1];
+        s2 += v[i] * x[3 * (size_t)cc[i] + 2];
+      }
     }
   }
 #pragma unroll

@@ -76,6 +76,11 @@ class Atoms(C.Structure):
                 ("mass", _dp), ("type", C.POINTER(C.c_int))]
 
 
+class AtomExtras(C.Structure):
+    _fields_ = [("tag", C.c_void_p), ("tag_bytes", C.c_int), ("molecule", C.c_void_p),
+                ("radius", _dp), ("mu", _dp), ("omega", _dp), ("torque", _dp)]
+
+
 VAR_EVAL = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_char_p, _dp)
 
 _lib = None
@@ -109,6 +114,7 @@ def lib() -> C.CDLL:
         "couplb_ibm_substep_device": (ip, [vp, ip, vp, vp, vp, vp, C.POINTER(IbmArgs)]),
         "couplb_refresh_macro": (ip, [vp]),
         "couplb_get_macro": (ip, [vp, dp, dp, dp, dp]),
+        "couplb_get_node_forces": (ip, [vp, dp, ip]),
         "couplb_diagnostics": (ip, [vp, dp]),
         "couplb_profile_y": (ip, [vp, dp, dp, dp]),
         "couplb_synchronize": (ip, [vp]),
@@ -135,6 +141,7 @@ def lib() -> C.CDLL:
         "couplb_fix_setup": (ip, [vp, C.POINTER(Atoms)]),
         "couplb_fix_post_force": (ip, [vp, C.POINTER(Atoms)]),
         "couplb_fix_end_of_step": (ip, [vp, C.c_long]),
+        "couplb_fix_set_atom_extras": (ip, [vp, C.POINTER(AtomExtras)]),
         "couplb_fix_setmask": (ip, []),
         "couplb_fix_scalar": (C.c_double, [vp, C.c_char_p]),
         "couplb_fix_last_diag": (ip, [vp, dp]),
@@ -287,6 +294,14 @@ class Device:
         out = [np.empty(nx * ny * nz) for _ in range(4)]
         self._chk(lib().couplb_get_macro(self.h, *[_p(o) for o in out]), "get_macro")
         return [o.reshape(nz, ny, nx) for o in out]
+
+    def node_forces(self, fold=True):
+        """Node forces of the IBM records since the last LB step, [nz, ny, nx, 3]
+        (fold: periodic ghost images added into the interior)."""
+        nx, ny, nz = self.n
+        out = np.empty(3 * nx * ny * nz)
+        self._chk(lib().couplb_get_node_forces(self.h, _p(out), int(bool(fold))), "get_node_forces")
+        return out.reshape(nz, ny, nx, 3)
 
     def diagnostics(self):
         out = np.zeros(6)
@@ -454,7 +469,30 @@ class Fix:
         a.mask = atoms["mask"].ctypes.data_as(C.POINTER(C.c_int))
         a.groupbit = atoms.get("groupbit", 1)
         a.rmass = _p(atoms["mass"])
+        if "type" in atoms:
+            a.type = atoms["type"].ctypes.data_as(C.POINTER(C.c_int))
         return a
+
+    def set_atom_extras(self, tag=None, molecule=None, radius=None, mu=None, omega=None, torque=None):
+        """Per-atom arrays for vtk_solid (int32 tag / molecule; float64
+        radius [n], mu [n, 4], omega and torque [n, 3]); kept alive here."""
+        e = AtomExtras()
+        keep = []
+        for name in ("tag", "molecule"):
+            arr = locals()[name]
+            if arr is not None:
+                arr = np.ascontiguousarray(arr, dtype=np.int32)
+                keep.append(arr)
+                setattr(e, name, arr.ctypes.data)
+        e.tag_bytes = 4
+        for name in ("radius", "mu", "omega", "torque"):
+            arr = locals()[name]
+            if arr is not None:
+                arr = np.ascontiguousarray(arr, dtype=np.float64)
+                keep.append(arr)
+                setattr(e, name, _p(arr))
+        self._extras_keep = keep
+        self._chk(lib().couplb_fix_set_atom_extras(self.h, C.byref(e)))
 
     def setup(self, atoms=None):
         self._keep = atoms

@@ -39,10 +39,14 @@ def build(verbose: bool = True) -> str:
     obj_fix = os.path.join(bdir, "fix_couplb_core.o")
     _run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(HERE, "csrc", "couplb_hip.hip"), "-o", obj_hip])
     _run([HIPCC] + HIP_FLAGS + ["-c", os.path.join(HERE, "csrc", "mlp.hip"), "-o", obj_mlp])
+    obj_vtk = os.path.join(bdir, "vtk_io.o")
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-c",
           os.path.join(HERE, "host", "fix_couplb_core.cpp"), "-o", obj_fix])
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-c",
+          os.path.join(HERE, "host", "vtk_io.cpp"), "-o", obj_vtk])
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, obj_hip, obj_mlp, obj_fix])
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, obj_hip, obj_mlp, obj_fix,
+          obj_vtk])
     os.replace(tmp, LIB)
     return LIB
 

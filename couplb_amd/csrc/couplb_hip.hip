@@ -980,6 +980,44 @@ int mdf_impl(couplb_ctx* c, int n, const double* x, const double* V, const doubl
   return COUPLB_OK;
 }
 
+// k_spread of the pending record slots into F4 / flag (fold: add the
+// periodic ghost images into the interior)
+int tile_spread(couplb_ctx* c, double* F4, unsigned char* flag, int fold) {
+  if (!c->slot_table) HIPCHK(c, hipMalloc(&c->slot_table, sizeof(Slot) * MAX_SLOTS));
+  if (c->slot_table_n != (int)c->slots.size()) {
+    Slot h[MAX_SLOTS];
+    for (size_t s = 0; s < c->slots.size(); s++) {
+      h[s].rowstart = c->slots[s].rowstart;
+      h[s].rec = c->slots[s].rec;
+    }
+    HIPCHK(c, hipMemcpy(c->slot_table, h, sizeof(Slot) * c->slots.size(), hipMemcpyHostToDevice));
+    c->slot_table_n = (int)c->slots.size();
+  }
+  SpreadArgs sa;
+  sa.g = c->geo;
+  sa.w = c->walls;
+  sa.nslots = c->nslots;
+  sa.slots = c->slot_table;
+  sa.F4 = F4;
+  sa.flag = flag;
+  sa.fold = fold;
+  const TileShape ts = tile_shape(c->D, c->geo.nx);
+#define SPREAD_LAUNCH(DD, TJ, TK, XM) \
+  hipLaunchKernelGGL((k_spread<DD, TJ, TK, XM>), tile_grid(c->geo), dim3(TILE_THREADS), 0, c->stream, sa)
+  if (c->D == 3) {
+    if (ts.XM == 1) SPREAD_LAUNCH(3, 2, 2, 1);
+    else if (ts.XM == 2) SPREAD_LAUNCH(3, 2, 1, 2);
+    else SPREAD_LAUNCH(3, 1, 1, 4);
+  } else {
+    if (ts.XM == 1) SPREAD_LAUNCH(2, 4, 1, 1);
+    else if (ts.XM == 2) SPREAD_LAUNCH(2, 2, 1, 2);
+    else SPREAD_LAUNCH(2, 1, 1, 4);
+  }
+#undef SPREAD_LAUNCH
+  HIPCHK(c, hipGetLastError());
+  return COUPLB_OK;
+}
+
 int step_impl(couplb_ctx* c, const double g[3], int flags, bool timed) {
   StepArgs a;
   a.pc = make_pull(c);
@@ -1019,41 +1057,12 @@ int step_impl(couplb_ctx* c, const double g[3], int flags, bool timed) {
   } else if (c->nslots > 0) {
     // spread the pending marker records into the per-node force (k_spread)
     c->pair_epoch = -1;  // k_spread rewrites the node flags
-    if (!c->slot_table) HIPCHK(c, hipMalloc(&c->slot_table, sizeof(Slot) * MAX_SLOTS));
-    if (c->slot_table_n != (int)c->slots.size()) {
-      Slot h[MAX_SLOTS];
-      for (size_t s = 0; s < c->slots.size(); s++) {
-        h[s].rowstart = c->slots[s].rowstart;
-        h[s].rec = c->slots[s].rec;
-      }
-      HIPCHK(c, hipMemcpy(c->slot_table, h, sizeof(Slot) * c->slots.size(), hipMemcpyHostToDevice));
-      c->slot_table_n = (int)c->slots.size();
-    }
-    SpreadArgs sa;
-    sa.g = c->geo;
-    sa.w = c->walls;
-    sa.nslots = c->nslots;
-    sa.slots = c->slot_table;
-    sa.F4 = c->F4;
-    sa.flag = c->flag;
     if (timed) {
       const int rc = timer_begin(c, 2, &tspread);
       if (rc) return rc;
     }
-    const TileShape ts = tile_shape(c->D, c->geo.nx);
-#define SPREAD_LAUNCH(DD, TJ, TK, XM) \
-  hipLaunchKernelGGL((k_spread<DD, TJ, TK, XM>), tile_grid(c->geo), dim3(TILE_THREADS), 0, c->stream, sa)
-    if (c->D == 3) {
-      if (ts.XM == 1) SPREAD_LAUNCH(3, 2, 2, 1);
-      else if (ts.XM == 2) SPREAD_LAUNCH(3, 2, 1, 2);
-      else SPREAD_LAUNCH(3, 1, 1, 4);
-    } else {
-      if (ts.XM == 1) SPREAD_LAUNCH(2, 4, 1, 1);
-      else if (ts.XM == 2) SPREAD_LAUNCH(2, 2, 1, 2);
-      else SPREAD_LAUNCH(2, 1, 1, 4);
-    }
-#undef SPREAD_LAUNCH
-    HIPCHK(c, hipGetLastError());
+    const int rc = tile_spread(c, c->F4, c->flag, 1);
+    if (rc) return rc;
   }
   if (tspread >= 0) {
     const int rc = timer_end(c, tspread);
@@ -1342,6 +1351,32 @@ int couplb_get_macro(couplb_ctx* c, double* rho, double* ux, double* uy, double*
     if (dsts[f])
       HIPCHK(c, hipMemcpyAsync(dsts[f], c->mac + f * c->N, sizeof(double) * c->N,
                                hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return COUPLB_OK;
+}
+
+int couplb_get_node_forces(couplb_ctx* c, double* f, int fold) {
+  if (!c || !f) return COUPLB_EINVAL;
+  const size_t N = c->N;
+  if (c->nslots == 0) {
+    std::memset(f, 0, sizeof(double) * 3 * N);
+    return COUPLB_OK;
+  }
+  int rc = ensure_staging(c);
+  if (rc) return rc;
+  // staging (Q ghost-inclusive planes, at least 9 N doubles): 4 N force
+  // records, N flag bytes, then the 3 N dense output
+  double* F4x = c->staging;
+  unsigned char* flagx = reinterpret_cast<unsigned char*>(c->staging + 4 * N);
+  double* out = c->staging + 4 * N + (N + 7) / 8;
+  if ((size_t)(out + 3 * N - c->staging) > c->staging_len)
+    return fail(c, COUPLB_ESTATE, "staging buffer too small for the node forces");
+  rc = tile_spread(c, F4x, flagx, fold ? 1 : 0);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_force_export, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, c->stream, N,
+                     (const double*)F4x, (const unsigned char*)flagx, out);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(f, out, sizeof(double) * 3 * N, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return COUPLB_OK;
 }

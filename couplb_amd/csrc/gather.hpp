@@ -187,33 +187,38 @@ struct SpreadArgs {
   const Slot* slots;    // device table of nslots descriptors
   double* F4;  // node force (fx, fy, fz, unused), 32 B per node
   unsigned char* flag;  // per node: 1 where some record contributes
+  int fold;  // 1: periodic ghost images add into the interior (exchange_forces);
+             // 0: direct contributions only (before the interval's last sub-step)
 };
 
 // Per-thread view of its tile nodes: XM x-positions x TJ*TK rows, with the
-// ghost-plane images of every coordinate computed once.
+// ghost-plane images of every coordinate computed once (fold = false: the
+// node itself only).
 template <int D, int TJ, int TK, int XM>
 struct TileNodes {
   int x[XM], j[TJ], k[TK];
   Img xa[XM], ya[TJ], za[TK];
   bool xin[XM], yin[TJ], zin[TK];
-  __device__ TileNodes(const Geo& g, const Walls& w, int J0, int K0) {
+  __device__ TileNodes(const Geo& g, const Walls& w, int J0, int K0, bool fold = true) {
 #pragma unroll
     for (int m = 0; m < XM; m++) {
       x[m] = threadIdx.x + m * TILE_THREADS;
       xin[m] = x[m] < g.nx;
-      xa[m] = images(x[m] + 1, g.nx, w.type[0][0], w.type[0][1]);
+      xa[m] = fold ? images(x[m] + 1, g.nx, w.type[0][0], w.type[0][1]) : Img{1, x[m] + 1, -9, -9};
     }
 #pragma unroll
     for (int r = 0; r < TJ; r++) {
       j[r] = J0 - 1 + r;
       yin[r] = j[r] < g.ny;
-      ya[r] = images(j[r] + 1, g.ny, w.type[1][0], w.type[1][1]);
+      ya[r] = fold ? images(j[r] + 1, g.ny, w.type[1][0], w.type[1][1]) : Img{1, j[r] + 1, -9, -9};
     }
 #pragma unroll
     for (int r = 0; r < TK; r++) {
       k[r] = K0 - 1 + r;
       zin[r] = (D == 3) ? k[r] < g.nz : r == 0;
-      za[r] = (D == 3) ? images(k[r] + 1, g.nz, w.type[2][0], w.type[2][1]) : Img{1, 0, -9, -9};
+      za[r] = (D == 3) ? (fold ? images(k[r] + 1, g.nz, w.type[2][0], w.type[2][1])
+                               : Img{1, k[r] + 1, -9, -9})
+                       : Img{1, 0, -9, -9};
     }
   }
 };
@@ -234,7 +239,7 @@ __global__ void __launch_bounds__(TILE_THREADS) k_spread(SpreadArgs a) {
   const int J0 = 1 + blockIdx.y * TJ, K0 = 1 + blockIdx.z * TK;
   const int nbin = g.nx + 5;
   tile_lists<D>(g, w, J0, K0, ts, tl);
-  const TileNodes<D, TJ, TK, XM> tn(g, w, J0, K0);
+  const TileNodes<D, TJ, TK, XM> tn(g, w, J0, K0, a.fold != 0);
   double acc[XM][TJ][TK][3];
   bool hit[XM][TJ][TK];
 #pragma unroll
@@ -305,6 +310,16 @@ __global__ void __launch_bounds__(TILE_THREADS) k_spread(SpreadArgs a) {
               double4{acc[m][r][q][0], acc[m][r][q][1], acc[m][r][q][2], 0.0};
         }
       }
+}
+
+// dense interior node forces from k_spread's output: out = F where flagged, else 0
+__global__ void __launch_bounds__(256) k_force_export(size_t N, const double* F4,
+                                                     const unsigned char* flag, double* out) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const bool h = flag[i] != 0;
+#pragma unroll
+  for (int c = 0; c < 3; c++) out[3 * i + c] = h ? F4[4 * i + c] : 0.0;
 }
 
 struct BaseSlots {

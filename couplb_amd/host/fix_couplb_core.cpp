@@ -2,6 +2,12 @@
 // Behaviour follows CoupLB/fix_couplb.cpp; citations give the line ranges.
 #include "fix_couplb_core.hpp"
 
+#include <algorithm>
+#include <functional>
+#include <set>
+
+#include "vtk_io.hpp"
+
 #include <cmath>
 #include <cstdarg>
 #include <cstdint>
@@ -158,7 +164,9 @@ FixCoupLBCore::FixCoupLBCore(int narg, const char* const* arg, const couplb_doma
     } else if (!std::strcmp(k, "vtk")) {
       need(2, "fix couplb vtk: need N prefix");
       vtk_every = to_int(arg[iarg + 1]);
+      vtk_prefix = arg[iarg + 2];
       if (vtk_every < 0) throw fix_error("fix couplb: vtk frequency cannot be negative");
+      vtk_pvd_file = vtk_prefix + ".pvd";
       iarg += 3;
     } else if (!std::strcmp(k, "vtk_region")) {
       iarg++;
@@ -166,12 +174,20 @@ FixCoupLBCore::FixCoupLBCore(int narg, const char* const* arg, const couplb_doma
       while (iarg + nreg < narg && !is_keyword(arg[iarg + nreg])) nreg++;
       if (nreg != 4 && nreg != 6)
         throw fix_error("fix couplb vtk_region: need 4 values (2D: xlo xhi ylo yhi) or 6 values (3D: xlo xhi ylo yhi zlo zhi)");
+      for (int d = 0; d < 2; d++) {
+        vtk_rlo[d] = to_double(arg[iarg + 2 * d]);
+        vtk_rhi[d] = to_double(arg[iarg + 2 * d + 1]);
+      }
+      vtk_rlo[2] = (nreg == 6) ? to_double(arg[iarg + 4]) : -1e30;
+      vtk_rhi[2] = (nreg == 6) ? to_double(arg[iarg + 5]) : 1e30;
+      has_vtk_region = true;
       iarg += nreg;
     } else if (!std::strcmp(k, "vtk_solid")) {
+      if (vtk_solid_on) throw fix_error("fix couplb vtk_solid: keyword specified more than once");
+      vtk_solid_on = true;
       iarg++;
-      int na = 0;
-      while (iarg < narg && !is_keyword(arg[iarg])) { iarg++; na++; }
-      if (na == 0) throw fix_error("fix couplb vtk_solid: need at least one attribute");
+      while (iarg < narg && !is_keyword(arg[iarg])) vtk_solid_attrs.push_back(arg[iarg++]);
+      if (vtk_solid_attrs.empty()) throw fix_error("fix couplb vtk_solid: need at least one attribute");
     } else if (!std::strcmp(k, "checkpoint")) {
       need(2, "fix couplb checkpoint: need N prefix");
       checkpoint_every = to_int(arg[iarg + 1]);
@@ -267,6 +283,50 @@ void FixCoupLBCore::init() {
   lbm_step_count = 0;
   md_sub_count = 0;
 
+  // vtk_solid / vtk_region set-up, fix_couplb.cpp:378-450
+  if (vtk_solid_on) {
+    if (vtk_every <= 0) throw fix_error("fix couplb vtk_solid requires vtk keyword to set output frequency");
+    vtk_solid_pvd_file = vtk_prefix + "_solid.pvd";
+    say("CoupLB: vtk_solid with %d attributes\n", (int)vtk_solid_attrs.size());
+  }
+  if (has_vtk_region) {
+    const int Nn[3] = {Nx, Ny, is3d ? Nz : 1};
+    for (int d = 0; d < 3; d++) {
+      vtk_rlo[d] = std::max(vtk_rlo[d], dom.boxlo[d]);  // clamp to the box first
+      vtk_rhi[d] = std::min(vtk_rhi[d], dom.boxhi[d]);
+    }
+    for (int d = 0; d < 3; d++) {
+      if (d == 2 && !is3d) {
+        vtk_region[4] = 0;
+        vtk_region[5] = 1;
+        continue;
+      }
+      vtk_region[2 * d] = (int)std::floor((vtk_rlo[d] - dom.boxlo[d]) / dx);
+      vtk_region[2 * d + 1] = (int)std::ceil((vtk_rhi[d] - dom.boxlo[d]) / dx);
+      vtk_region[2 * d] = std::max(0, std::min(vtk_region[2 * d], Nn[d]));
+      vtk_region[2 * d + 1] = std::max(0, std::min(vtk_region[2 * d + 1], Nn[d]));
+    }
+    if (vtk_region[1] <= vtk_region[0] || vtk_region[3] <= vtk_region[2] || vtk_region[5] <= vtk_region[4]) {
+      char b[256];
+      std::snprintf(b, sizeof(b),
+                    "fix couplb vtk_region: empty or inverted region ([%d,%d]x[%d,%d]x[%d,%d]). Check that "
+                    "xlo<xhi, ylo<yhi%s",
+                    vtk_region[0], vtk_region[1], vtk_region[2], vtk_region[3], vtk_region[4], vtk_region[5],
+                    is3d ? ", zlo<zhi" : "");
+      throw fix_error(b);
+    }
+    for (int d = 0; d < 3; d++) {  // snap to node boundaries (the atom filter of vtk_solid)
+      vtk_rlo[d] = dom.boxlo[d] + vtk_region[2 * d] * dx;
+      vtk_rhi[d] = dom.boxlo[d] + vtk_region[2 * d + 1] * dx;
+    }
+    if (is3d)
+      say("CoupLB: vtk_region [%.4f,%.4f]x[%.4f,%.4f]x[%.4f,%.4f] -> nodes [%d,%d]x[%d,%d]x[%d,%d]\n",
+          vtk_rlo[0], vtk_rhi[0], vtk_rlo[1], vtk_rhi[1], vtk_rlo[2], vtk_rhi[2], vtk_region[0], vtk_region[1],
+          vtk_region[2], vtk_region[3], vtk_region[4], vtk_region[5]);
+    else
+      say("CoupLB: vtk_region [%.4f,%.4f]x[%.4f,%.4f] -> nodes [%d,%d]x[%d,%d]\n", vtk_rlo[0], vtk_rhi[0],
+          vtk_rlo[1], vtk_rhi[1], vtk_region[0], vtk_region[1], vtk_region[2], vtk_region[3]);
+  }
   // setup_grid + setup_boundaries (fix_couplb.cpp:484-539): one device
   // grid holds the whole domain.
   for (int d = 0; d < dim; d++)
@@ -301,7 +361,9 @@ void FixCoupLBCore::init() {
     read_checkpoint();
     do_restart = false;
   }
-  if (vtk_every > 0) say("CoupLB WARNING: VTK output is not part of this build; vtk steps only refresh fields\n");
+  if (vtk_every > 0)
+    say("CoupLB: VTK output every %d -> %s_*.vti (series: %s)\n", vtk_every, vtk_prefix.c_str(),
+        vtk_pvd_file.c_str());
   say("CoupLB: %dD grid %dx%dx%d tau=%.6f nu=%.6f rho=%.4f\n", dim, Nx, Ny, Nz, tau, nu, rho);
   say("CoupLB: nu=%.4e -> nu_lb=%.6f\n", nu, nu_lb);
   say("CoupLB: dx=%.4e dt_LBM=%.4e vel_scale=%.4e force_scale=%.4e\n", dx, dt_LBM, vel_scale, force_scale);
@@ -344,6 +406,8 @@ void FixCoupLBCore::lbm_step(int ng) {
 
 // FixCoupLB::post_force, fix_couplb.cpp:640-651
 void FixCoupLBCore::post_force(const couplb_atoms& a) {
+  atoms_now = a;
+  have_atoms = true;
   const bool first = (md_sub_count == 0);
   if (first) {
     lbm_step(count_group(a));
@@ -396,6 +460,8 @@ void FixCoupLBCore::ibm_sub_coupling(const couplb_atoms& a, bool first) {
 
 // FixCoupLB::setup, fix_couplb.cpp:551-638
 void FixCoupLBCore::setup(const couplb_atoms& a) {
+  atoms_now = a;
+  have_atoms = true;
   md_sub_count = 0;
   ibm_sub_coupling(a, true);
   md_sub_count = 0;
@@ -412,6 +478,7 @@ void FixCoupLBCore::setup(const couplb_atoms& a) {
     }
     write_profile(dom.ntimestep);
   }
+  if (vtk_every > 0) write_vtk(dom.ntimestep);
 }
 
 // check_stability_precomputed, fix_couplb.cpp:818-844
@@ -513,8 +580,147 @@ int FixCoupLBCore::end_of_step(long step) {
   if (chk || out || vtk) check(couplb_refresh_macro(ctx), "refresh");
   if (chk) check_stability_precomputed(step);
   if (out) write_profile(step);
+  if (vtk) write_vtk(step);
   if (ckpt) write_checkpoint(step);
   return 0;
+}
+
+// The vtk block of end_of_step / setup, fix_couplb.cpp:862-876, 588-603:
+// fluid fields, the group's atoms, and both collections.
+void FixCoupLBCore::write_vtk(long step) {
+  const size_t N = (size_t)Nx * Ny * (is3d ? Nz : 1);
+  std::vector<double> r(N), ux(N), uy(N), uz(N), F(3 * N);
+  check(couplb_get_macro(ctx, r.data(), ux.data(), uy.data(), uz.data()), "macro fields");
+  // Grid::fx holds the reactions spread since the LB step; their periodic
+  // ghost images reach the interior at the interval's last sub-step
+  // (exchange_forces, fix_couplb.cpp:795-797; setup, 558-563)
+  check(couplb_get_node_forces(ctx, F.data(), md_sub_count == 0 ? 1 : 0), "node forces");
+  VtiFields vf;
+  vf.nx = Nx; vf.ny = Ny; vf.nz = is3d ? Nz : 1;
+  vf.rho = r.data(); vf.ux = ux.data(); vf.uy = uy.data(); vf.uz = uz.data(); vf.force = F.data();
+  std::string fname, msg;
+  if (write_vti(vtk_prefix, step, vf, has_vtk_region ? vtk_region : nullptr, dom.boxlo, dx, vel_scale,
+                force_scale, rho, &fname, &msg))
+    say("CoupLB: VTK written at step %ld -> %s%s\n", step, fname.c_str(), msg.c_str());
+  else
+    say("%s\n", msg.c_str());
+  if (vtk_solid_on) write_solid_vtk(step);
+  vtk_steps.push_back(step);
+  if (!write_pvd(vtk_pvd_file, vtk_prefix, vtk_steps, dom.dt, "vti"))
+    say("CoupLB IO: cannot open %s\n", vtk_pvd_file.c_str());
+  if (vtk_solid_on && !write_pvd(vtk_solid_pvd_file, vtk_prefix + "_solid", vtk_steps, dom.dt, "vtp"))
+    say("CoupLB IO: cannot open %s\n", vtk_solid_pvd_file.c_str());
+}
+
+// write_solid_vtk, couplb_io.h:304-609: the group's atoms (inside the
+// snapped vtk_region) as points, with the requested attributes; an x/y/z
+// style triple is one 3-component array when all three are requested.
+void FixCoupLBCore::write_solid_vtk(long step) {
+  const couplb_atoms& a = atoms_now;
+  if (!have_atoms) throw fix_error("fix couplb vtk_solid: no atom arrays (setup / post_force not called)");
+  const couplb_atom_extras& e = extras;
+  const bool ex = have_extras;
+  auto tagval = [&](const void* p, int i) -> double {
+    return e.tag_bytes == 8 ? (double)static_cast<const int64_t*>(p)[i] : (double)static_cast<const int*>(p)[i];
+  };
+  struct Attr {
+    const char* name;
+    const char* group;  // "" for a scalar
+    int comp;
+    std::function<double(int)> get;
+  };
+  std::vector<Attr> reg;
+  if (ex && e.tag) reg.push_back({"id", "", 0, [&](int i) { return tagval(e.tag, i); }});
+  if (a.type) reg.push_back({"type", "", 0, [&](int i) { return (double)a.type[i]; }});
+  if (ex && e.molecule) reg.push_back({"mol", "", 0, [&](int i) { return tagval(e.molecule, i); }});
+  if (a.rmass) reg.push_back({"mass", "", 0, [&](int i) { return a.rmass[i]; }});
+  else if (a.mass && a.type) reg.push_back({"mass", "", 0, [&](int i) { return a.mass[a.type[i]]; }});
+  if (ex && e.radius) reg.push_back({"diameter", "", 0, [&](int i) { return 2.0 * e.radius[i]; }});
+  const char* xyz[3] = {"x", "y", "z"};
+  const char* vxyz[3] = {"vx", "vy", "vz"};
+  const char* fxyz[3] = {"fx", "fy", "fz"};
+  for (int c = 0; c < 3; c++) reg.push_back({xyz[c], "position", c, [&, c](int i) { return a.x[3 * i + c]; }});
+  for (int c = 0; c < 3; c++) reg.push_back({vxyz[c], "velocity", c, [&, c](int i) { return a.v[3 * i + c]; }});
+  for (int c = 0; c < 3; c++) reg.push_back({fxyz[c], "force", c, [&, c](int i) { return a.f[3 * i + c]; }});
+  const char* mu[3] = {"mux", "muy", "muz"};
+  const char* om[3] = {"omegax", "omegay", "omegaz"};
+  const char* tq[3] = {"tqx", "tqy", "tqz"};
+  if (ex && e.mu)
+    for (int c = 0; c < 3; c++) reg.push_back({mu[c], "dipole", c, [&, c](int i) { return e.mu[4 * i + c]; }});
+  if (ex && e.omega)
+    for (int c = 0; c < 3; c++) reg.push_back({om[c], "omega", c, [&, c](int i) { return e.omega[3 * i + c]; }});
+  if (ex && e.torque)
+    for (int c = 0; c < 3; c++) reg.push_back({tq[c], "torque", c, [&, c](int i) { return e.torque[3 * i + c]; }});
+  auto is_one = [](const std::string& s, const char* const* names) {
+    return s == names[0] || s == names[1] || s == names[2];
+  };
+  std::vector<const Attr*> req;
+  for (const std::string& s : vtk_solid_attrs) {
+    if (s == "mol" && !(ex && e.molecule))
+      throw fix_error("fix couplb vtk_solid: attribute 'mol' requires a molecular atom style");
+    if (s == "diameter" && !(ex && e.radius))
+      throw fix_error("fix couplb vtk_solid: attribute 'diameter' requires atom style sphere (radius)");
+    if (is_one(s, mu) && !(ex && e.mu))
+      throw fix_error("fix couplb vtk_solid: dipole attributes require dipole atom style (mu)");
+    if (is_one(s, om) && !(ex && e.omega))
+      throw fix_error("fix couplb vtk_solid: omega attributes require atom style sphere (omega)");
+    if (is_one(s, tq) && !(ex && e.torque))
+      throw fix_error("fix couplb vtk_solid: torque attributes require atom style sphere (torque)");
+    if (s == "mass" && !(a.rmass || a.mass))
+      throw fix_error("fix couplb vtk_solid: attribute 'mass' requires per-atom mass (rmass) or type mass table");
+    if (s == "id" && !(ex && e.tag)) throw fix_error("fix couplb vtk_solid: attribute 'id' needs the atom IDs");
+    if (s == "type" && !a.type) throw fix_error("fix couplb vtk_solid: attribute 'type' needs the atom types");
+    const Attr* hit = nullptr;
+    for (const Attr& r : reg)
+      if (s == r.name) { hit = &r; break; }
+    if (!hit) throw fix_error("fix couplb vtk_solid: unknown attribute '" + s + "'");
+    req.push_back(hit);
+  }
+  // output arrays in request order; a group becomes one array at its first
+  // member when all three components are requested
+  struct Out {
+    std::string name;
+    int ncomp;
+    const Attr* comp[3];
+  };
+  std::vector<Out> outs;
+  std::set<std::string> done;
+  for (const Attr* r : req) {
+    if (r->group[0]) {
+      if (done.count(r->group)) continue;
+      const Attr* comp[3] = {nullptr, nullptr, nullptr};
+      for (const Attr* q : req)
+        if (!std::strcmp(q->group, r->group)) comp[q->comp] = q;
+      if (comp[0] && comp[1] && comp[2]) {
+        outs.push_back({r->group, 3, {comp[0], comp[1], comp[2]}});
+        done.insert(r->group);
+        continue;
+      }
+    }
+    outs.push_back({r->name, 1, {r, nullptr, nullptr}});
+  }
+  std::vector<double> pts;
+  std::vector<VtpArray> arrays(outs.size());
+  for (size_t o = 0; o < outs.size(); o++) {
+    arrays[o].name = outs[o].name;
+    arrays[o].ncomp = outs[o].ncomp;
+  }
+  for (int i = 0; i < a.nlocal; i++) {
+    if (!(a.mask[i] & a.groupbit)) continue;
+    if (has_vtk_region) {
+      bool in = true;
+      for (int d = 0; d < 3; d++)
+        if (a.x[3 * i + d] < vtk_rlo[d] || a.x[3 * i + d] >= vtk_rhi[d]) in = false;
+      if (!in) continue;
+    }
+    for (int d = 0; d < 3; d++) pts.push_back(a.x[3 * i + d]);
+    for (size_t o = 0; o < outs.size(); o++)
+      for (int d = 0; d < outs[o].ncomp; d++) arrays[o].v.push_back(outs[o].comp[d]->get(i));
+  }
+  std::string fname;
+  if (!write_vtp(vtk_prefix, step, pts, arrays, &fname)) throw fix_error("CoupLB IO: cannot open " + fname);
+  say("CoupLB: solid VTK written at step %ld -> %s (%d atoms, %d fields)\n", step, fname.c_str(),
+      (int)(pts.size() / 3), (int)outs.size());
 }
 
 double FixCoupLBCore::scalar(const std::string& n) const {
@@ -605,6 +811,15 @@ int couplb_fix_post_force(couplb_fix* fx, const couplb_atoms* a) {
 int couplb_fix_end_of_step(couplb_fix* fx, long step) {
   if (!fx || !fx->core) return COUPLB_EINVAL;
   return guard(fx, [&] { fx->core->end_of_step(step); });
+}
+int couplb_fix_set_atom_extras(couplb_fix* fx, const couplb_atom_extras* e) {
+  if (!fx || !fx->core || !e) return COUPLB_EINVAL;
+  if (e->tag_bytes != 4 && e->tag_bytes != 8 && (e->tag || e->molecule)) {
+    fx->err = "couplb_atom_extras: tag_bytes must be 4 or 8";
+    return COUPLB_EINVAL;
+  }
+  fx->core->set_atom_extras(*e);
+  return COUPLB_OK;
 }
 int couplb_fix_setmask(void) { return couplb_host::FixCoupLBCore::setmask(); }
 

@@ -31,6 +31,7 @@ class FixCoupLBCore {
   int end_of_step(long step);
 
   void set_variable_callback(couplb_var_eval fn, void* user) { var_fn = fn; var_user = user; }
+  void set_atom_extras(const couplb_atom_extras& e) { extras = e; have_extras = true; }
   void set_domain(const couplb_domain& d) { dom = d; }
   double scalar(const std::string& name) const;
   const double* last_diag() const { return diag; }
@@ -56,6 +57,14 @@ class FixCoupLBCore {
   std::string output_file = "couplb_profile.dat";
   int check_every = 0;
   int vtk_every = 0;
+  std::string vtk_prefix = "couplb_vtk", vtk_pvd_file;
+  bool has_vtk_region = false;
+  double vtk_rlo[3] = {0, 0, 0}, vtk_rhi[3] = {0, 0, 0};
+  int vtk_region[6] = {0, 0, 0, 0, 0, 0};
+  bool vtk_solid_on = false;
+  std::vector<std::string> vtk_solid_attrs;
+  std::string vtk_solid_pvd_file;
+  std::vector<long> vtk_steps;
   int checkpoint_every = 0;
   std::string checkpoint_prefix = "couplb_ckpt";
   std::string restart_prefix;
@@ -80,6 +89,10 @@ class FixCoupLBCore {
   std::vector<double> px, pv, pm, pf;
   std::vector<double> px_prev;  // positions of the previous sub-step (reuse test)
   std::vector<int> pidx;
+  couplb_atoms atoms_now{};  // the arrays of the current step (setup / post_force)
+  bool have_atoms = false;
+  couplb_atom_extras extras{};
+  bool have_extras = false;
 
   void say(const char* fmt, ...);
   void check(int rc, const char* what);
@@ -90,6 +103,8 @@ class FixCoupLBCore {
   void check_stability_precomputed(long step);
   void write_profile(long step);
   void write_checkpoint(long step);
+  void write_vtk(long step);
+  void write_solid_vtk(long step);
   void read_checkpoint();
   bool refresh_follows_next_step() const;
 };

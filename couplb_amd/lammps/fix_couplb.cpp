@@ -89,15 +89,32 @@ void FixCoupLB::init()
   if (comm->me == 0 && screen) fputs(couplb_fix_log(core), screen);
 }
 
+// per-atom arrays only vtk_solid reads (couplb_io.h:335-377)
+void FixCoupLB::send_extras()
+{
+  couplb_atom_extras e;
+  const int n = atom->nlocal;
+  e.tag = atom->tag;
+  e.tag_bytes = (int) sizeof(tagint);
+  e.molecule = atom->molecule;
+  e.radius = atom->radius;
+  e.mu = (atom->mu && n) ? &atom->mu[0][0] : nullptr;
+  e.omega = (atom->omega && n) ? &atom->omega[0][0] : nullptr;
+  e.torque = (atom->torque && n) ? &atom->torque[0][0] : nullptr;
+  check(couplb_fix_set_atom_extras(core, &e));
+}
+
 void FixCoupLB::setup(int)
 {
   couplb_atoms a = atom_view();
+  send_extras();
   check(couplb_fix_setup(core, &a));
 }
 
 void FixCoupLB::post_force(int)
 {
   couplb_atoms a = atom_view();
+  send_extras();
   check(couplb_fix_post_force(core, &a));
 }
 

@@ -32,6 +32,7 @@ class FixCoupLB : public Fix {
   couplb_fix *core;
   couplb_domain current_domain() const;
   couplb_atoms atom_view() const;
+  void send_extras();
   void check(int rc);
   static int eval_variable(void *self, const char *name, double *value);
 };

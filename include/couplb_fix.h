@@ -46,6 +46,18 @@ typedef struct couplb_atoms {
   const int* type;
 } couplb_atoms;
 
+/* Per-atom arrays that only `vtk_solid` reads (write_solid_vtk,
+ * couplb_io.h:335-377); NULL where the atom style lacks them. */
+typedef struct couplb_atom_extras {
+  const void* tag;        /* atom->tag */
+  int tag_bytes;          /* sizeof(tagint): 4 or 8 */
+  const void* molecule;   /* atom->molecule (tagint, tag_bytes each), or NULL */
+  const double* radius;   /* atom->radius, or NULL */
+  const double* mu;       /* &atom->mu[0][0]: 4 values per atom, or NULL */
+  const double* omega;    /* &atom->omega[0][0], or NULL */
+  const double* torque;   /* &atom->torque[0][0], or NULL */
+} couplb_atom_extras;
+
 /* Equal-style variable evaluation for `gravity v_name ...`
  * (input->variable->compute_equal, fix_couplb.cpp:671-678).  Return 0 on
  * success; a nonzero return is the reference's "does not exist" error. */
@@ -63,7 +75,12 @@ int couplb_fix_set_domain(couplb_fix* fx, const couplb_domain* dom);
 int couplb_fix_init(couplb_fix* fx);                     /* FixCoupLB::init */
 int couplb_fix_setup(couplb_fix* fx, const couplb_atoms* a);      /* ::setup */
 int couplb_fix_post_force(couplb_fix* fx, const couplb_atoms* a); /* ::post_force */
-int couplb_fix_end_of_step(couplb_fix* fx, long step);   /* ::end_of_step */
+/* ::end_of_step; VTK output reads the atom arrays handed to the last
+ * setup / post_force call (the same timestep's arrays in LAMMPS) */
+int couplb_fix_end_of_step(couplb_fix* fx, long step);
+/* the extra per-atom arrays of vtk_solid; refresh them with the atom
+ * arrays before setup / post_force (they move when LAMMPS reallocates) */
+int couplb_fix_set_atom_extras(couplb_fix* fx, const couplb_atom_extras* e);
 int couplb_fix_setmask(void);                            /* POST_FORCE|END_OF_STEP */
 
 /* Introspection used by tests and the bench. */

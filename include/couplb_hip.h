@@ -196,6 +196,15 @@ int couplb_refresh_macro(couplb_ctx* ctx);
  * nx*ny*nz array with x fastest.  Also a refresh. */
 int couplb_get_macro(couplb_ctx* ctx, double* rho, double* ux, double* uy, double* uz);
 
+/* Node forces of the IBM reactions recorded since the last LB step, as the
+ * reference's Grid::fx/fy/fz hold them at end_of_step (lattice units): the
+ * records spread by IBM::spread (couplb_ibm.h:92-133); fold = 1 adds the
+ * periodic ghost images into the interior as exchange_forces does after the
+ * interval's last sub-step (couplb_streaming.h:217-225), fold = 0 keeps the
+ * direct contributions only.  f: 3*nx*ny*nz doubles, (fx, fy, fz) per node,
+ * x fastest.  Feeds the VTK force field (couplb_io.h:255-258). */
+int couplb_get_node_forces(couplb_ctx* ctx, double* f, int fold);
+
 /* FP64 global diagnostics: replaces Grid::max_velocity and
  * Grid::compute_diagnostics (couplb_lattice.h:203-242) used by
  * check_stability_precomputed (fix_couplb.cpp:818-844).

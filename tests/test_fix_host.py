@@ -22,6 +22,9 @@ from couplb_amd import CoupLBError, Fix
     ("f all couplb 8 8 8 0.1 1.0 check_every -1", "check_every cannot be negative"),
     ("f all couplb 8 8 8 0.1 1.0 vtk_region 0 1 2", "vtk_region: need 4 values"),
     ("f all couplb 8 8 8 0.1 1.0 storage fp8", "use fp64, fp32, fp16 or bf16"),
+    ("f all couplb 8 8 8 0.1 1.0 vtk 10 v vtk_solid id vtk_solid x", "vtk_solid: keyword specified more than once"),
+    ("f all couplb 8 8 8 0.1 1.0 vtk_solid", "vtk_solid: need at least one attribute"),
+    ("f all couplb 8 8 8 0.1 1.0 vtk -1 v", "vtk frequency cannot be negative"),
 ])
 def test_constructor_errors(cmd, msg):
     with pytest.raises(CoupLBError, match=msg.replace("[", r"\[").replace("]", r"\]")):
@@ -39,6 +42,12 @@ def test_2d_requires_nz_1():
     ("f all couplb 8 8 8 0.1 1.0 wall_vel 0.4 0 0", {}, "wall Ma > 0.5"),
     ("f all couplb 8 32 8 0.1 1.0 wall_y 1 1 gravity 1e-3 0 0", {}, "Poiseuille Ma > 0.5"),
     ("f all couplb 8 8 8 0.1 1.0 gravity v_gx 0 0", {}, "Variable gx for gravity x"),
+    # vtk_solid / vtk_region set-up, fix_couplb.cpp:378-425
+    ("f all couplb 8 8 8 0.1 1.0 vtk_solid id", {}, "vtk_solid requires vtk keyword"),
+    ("f all couplb 8 8 8 0.1 1.0 vtk 10 v vtk_region 5 3 0 8 0 8", {}, "empty or inverted region"),
+    # clamped to the box first: a region beyond the box is empty
+    ("f all couplb 8 8 8 0.1 1.0 vtk 10 v vtk_region 9 12 0 8 0 8", {},
+     r"empty or inverted region \(\[8,8\]x\[0,8\]x\[0,8\]\)"),
 ])
 def test_init_errors_before_device(cmd, kw, msg):
     fx = Fix(cmd, **kw)

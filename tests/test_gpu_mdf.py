@@ -91,6 +91,40 @@ def test_mdf_richardson_matches_oracle_and_next_step(kernel, operator, monkeypat
 
 
 @pytest.mark.parametrize("operator", list(OPERATORS))
+def test_mdf_near_moving_wall_matches_oracle(operator, monkeypatch):
+    # stencils that reach the moving y wall: the interpolation counts the
+    # wall nodes (their velocity in b, zero in A) while spreading skips
+    # them, so A_kj carries the ratio of fluid to counted weight
+    import couplb_amd
+    monkeypatch.setenv("COUPLB_MDF_MATRIX", OPERATORS[operator])
+    d = developed_device(couplb_amd.PESKIN4)
+    x = sphere((9.6, 2.7, 7.9), 2.2, 90)          # y from 0.5 to 4.9; the wall ghost is y = 0
+    V = np.tile([1e-3, 0.0, 0.0], (len(x), 1))
+    n = len(x)
+    dx, dV, fb = DeviceArray(x), DeviceArray(V), DeviceArray(np.zeros((n, 3)))
+    print("case MDF near the moving wall", operator, flush=True)
+    st = d.mdf_device(n, dx.addr, dV.addr, None, fb.addr, (0, 0, 0), 1.0, 1.0, 1.0,
+                      tol=1e-30, relax=0.5, max_iter=30, method="richardson")
+    F_dev = -fb.download()
+    g = _oracle_grid(d)
+    F_ref, it, s0, s1 = oracle.mdf_solve(g, x, V, (0, 0, 0), 1.0, couplb_amd.PESKIN4, 1e-30,
+                                         relax=0.5, max_iter=30, method="richardson")
+    print("device", st, "oracle", it, s0, s1, "rel F", rel_l2(F_dev, F_ref))
+    assert st["iterations"] == it == 30
+    assert st["slip_last"] == pytest.approx(s1, rel=1e-9)
+    assert rel_l2(F_dev, F_ref) < 1e-12
+    d.step((0.0, 0.0, 0.0), 0)
+    o = g.ops
+    o.compute_macroscopic(g.g, 1)
+    o.collide(g.g)
+    o.exchange(g.g)
+    o.stream(g.g)
+    fd = interior(d.get_state(), 3)
+    fo = interior(g.field(oracle.F_F).reshape(19, N3[2] + 2, N3[1] + 2, N3[0] + 2), 3)
+    assert rel_l2(fd, fo) < 1e-12
+
+
+@pytest.mark.parametrize("operator", list(OPERATORS))
 @pytest.mark.parametrize("kernel", ["roma", "peskin4"])
 def test_mdf_cg_converges_like_oracle(kernel, operator, monkeypatch):
     import couplb_amd

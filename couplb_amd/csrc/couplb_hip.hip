@@ -657,6 +657,11 @@ int mm_build(couplb_ctx* c, int n) {
   HIPCHK(c, hipMemcpyAsync(&T, off + nh, sizeof(T), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (T == 0 || T > (unsigned long long)MM_MAX_TRIPLES) return COUPLB_OK;
+  // the build holds about 40 bytes per triple (keys, values, sorted copies,
+  // flags, entry starts, sort scratch); leave half of the free memory alone
+  size_t mem_free = 0, mem_total = 0;
+  HIPCHK(c, hipMemGetInfo(&mem_free, &mem_total));
+  if (40ull * T > mem_free / 2) return COUPLB_OK;
   unsigned long long *key = nullptr, *key_s = nullptr, *val = nullptr;
   unsigned char* flag = nullptr;
   int *ent = nullptr, *cnt_d = nullptr;

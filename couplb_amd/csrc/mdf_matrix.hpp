@@ -162,9 +162,4 @@ __global__ void __launch_bounds__(MDF_THREADS) k_mm_spmv(int n, const int* rowpt
   }
 }
 
-__global__ void __launch_bounds__(256) k_copy3(int n, const double* src, double* dst) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < 3 * n) dst[i] = src[i];
-}
-
 }  // namespace couplb

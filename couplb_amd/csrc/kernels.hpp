@@ -1,8 +1,10 @@
 // kernels.hpp -- HIP kernels of the LB/IBM step for CDNA4 (gfx950).
 //
 // Device layout (interior nodes only, no ghost planes):
-//   populations  f[q*pitch + node], node = i + nx*(j + ny*k), x fastest;
-//                two buffers, ping-ponged by the fused step.
+//   populations  f[q*pq + row*pr + i], row = j + ny*k, x fastest (Layout):
+//                q planes (pq >= nx*ny*nz, pr = nx) or x-rows holding all
+//                q together (pq >= nx, pr >= Q*pq); two buffers,
+//                ping-ponged by the fused step.
 //   The stored state is POST-COLLISION (f* of the last step) unless the
 //   context is in "local" mode (right after create/import), when the buffer
 //   holds the POST-STREAM populations the reference keeps between steps.
@@ -97,9 +99,17 @@ __device__ inline Img images(int I, int n, int tlo, int thi) {
 //              rho_w = max(rho_coll[n], rho_floor);
 //   3 free-slip f[reflect_d q][n]; 4 open f[q][n].
 // ---------------------------------------------------------------------------
+// Element (q, x, row) of a population buffer: q*pq + row*pr + x.
+struct Layout {
+  size_t pq, pr;
+  __host__ __device__ size_t at(int q, unsigned row, unsigned x) const {
+    return (size_t)q * pq + (size_t)row * pr + x;
+  }
+};
+
 struct PullCtx {
   const void* src;
-  size_t pitch;
+  Layout lay;
   Geo g;
   Walls w;
   const double* rho_coll;  // density of the collision that made src
@@ -121,11 +131,12 @@ __device__ inline void pull_node_with(const PullCtx& pc, int i, int j, int k,
   constexpr int Q = L::Q;
   const S* src = static_cast<const S*>(pc.src);
   const int nx = pc.g.nx, ny = pc.g.ny, nz = pc.g.nz;
-  const unsigned node = (unsigned)i + (unsigned)nx * ((unsigned)j + (unsigned)ny * (unsigned)k);
+  const unsigned row = (unsigned)j + (unsigned)ny * (unsigned)k;
+  const unsigned node = (unsigned)i + (unsigned)nx * row;
   if (pc.local) {
 #pragma unroll
     for (int q = 0; q < Q; q++) {
-      f[q] = Store<S>::load(src + q * pc.pitch + node);
+      f[q] = Store<S>::load(src + pc.lay.at(q, row, i));
       if (Store<S>::deviation) f[q] *= (T)pc.inv_scale;
     }
     return;
@@ -177,7 +188,7 @@ __device__ inline void pull_node_with(const PullCtx& pc, int i, int j, int k,
       int qs = q;
       if (wt == 1 || wt == 2) qs = L::opp(q);
       else if (wt == 3) qs = (wd == 0) ? L::reflect(0, q) : (wd == 1 ? L::reflect(1, q) : L::reflect(2, q));
-      f[q] = Store<S>::load(src + qs * pc.pitch + node);
+      f[q] = Store<S>::load(src + pc.lay.at(qs, row, i));
       if (wt == 2) {
         const double* uw = pc.w.vel[wd][wsd];
         if (Store<S>::deviation) {
@@ -203,10 +214,9 @@ template <int D, typename S>
 __device__ inline void pull_node(const PullCtx& pc, int i, int j, int k,
                                  typename Store<S>::T (&f)[Lat<D>::Q]) {
   const S* src = static_cast<const S*>(pc.src);
-  const int nx = pc.g.nx, ny = pc.g.ny;
+  const int ny = pc.g.ny;
   pull_node_with<D, S>(pc, i, j, k, f, [&](int q, int si, int sj, int sk) {
-    const unsigned addr = (unsigned)si + (unsigned)nx * ((unsigned)sj + (unsigned)ny * (unsigned)sk);
-    return Store<S>::load(src + q * pc.pitch + addr);
+    return Store<S>::load(src + pc.lay.at(q, (unsigned)sj + (unsigned)ny * (unsigned)sk, (unsigned)si));
   });
 }
 
@@ -249,7 +259,7 @@ __device__ inline double edot(int q, double v0, double v1, double v2) {
 template <int D>
 __device__ inline void collide_exact(const double (&f)[Lat<D>::Q], double r, double su, double sv,
                                      double sw, double F0, double F1, double F2, double om,
-                                     double* dst, size_t pitch, unsigned node) {
+                                     double* dst, size_t pq, size_t at0) {
 #pragma clang fp contract(off)
   using L = Lat<D>;
   constexpr int Q = L::Q;
@@ -269,7 +279,7 @@ __device__ inline void collide_exact(const double (&f)[Lat<D>::Q], double r, dou
     const double feq = w * r * (1.0 + div_cs2(eu) + div_cs4(0.5 * eu * eu) - uu_term);
     const double Si = w * half_om * (div_cs2(ef - uf) + div_cs4(eu * ef));
     const double fn = f[q] + (-om * (f[q] - feq) + Si);
-    Store<double>::store(dst + q * pitch + node, fn);
+    Store<double>::store(dst + q * pq + at0, fn);
   }
 }
 
@@ -337,7 +347,8 @@ __device__ inline void step_node(const StepArgs& a, int i, int j, int k, unsigne
     if (wall_adj) a.rho_out[node] = (double)rho;
   }
   if constexpr (!Store<S>::deviation) {
-    collide_exact<D>(f, rho, jx, jy, jz, Fx, Fy, Fz, a.omega, static_cast<S*>(a.dst), a.pc.pitch, node);
+    collide_exact<D>(f, rho, jx, jy, jz, Fx, Fy, Fz, a.omega, static_cast<S*>(a.dst), a.pc.lay.pq,
+                     a.pc.lay.at(0, (unsigned)j + (unsigned)g.ny * (unsigned)k, (unsigned)i));
     (void)out;
   } else {
     const T inv_r = (T)1 / rho;
@@ -378,17 +389,18 @@ __global__ void __launch_bounds__(256) k_step(StepArgs a) {
   T f[Q];
   pull_node<D, S>(a.pc, i, j, k, f);
   S* dst = static_cast<S*>(a.dst);
+  const size_t at0 = a.pc.lay.at(0, (unsigned)j + (unsigned)g.ny * (unsigned)k, (unsigned)i);
   step_node<D, S>(a, i, j, k, node, f,
-                  [&](int q, T v) { Store<S>::store(dst + q * a.pc.pitch + node, v); });
+                  [&](int q, T v) { Store<S>::store(dst + q * a.pc.lay.pq + at0, v); });
 }
 
 // Reduced formats: the same step with the 19 source rows of a 256-node row
 // chunk staged through LDS in 16-byte groups, and the results written back
 // the same way (through the same LDS buffer), so that each
 // global access moves 16 bytes per lane instead of one 2- or 4-byte
-// population.  Needs nx and the plane pitch to be multiples of the group
-// (8 or 4 elements) and a state in pull form (not the post-import local
-// form).
+// population.  Needs nx and both layout strides to be multiples of the
+// group (8 or 4 elements) and a state in pull form (not the post-import
+// local form).
 constexpr int LDS_GB = 16;  // bytes per staged group
 // nodes per block
 template <typename S> constexpr int lds_chunk() { return 256; }
@@ -412,7 +424,7 @@ __global__ void __launch_bounds__(lds_chunk<S>()) k_step_lds(StepArgs a) {
   // copies, not references: the lambdas below must not take the address of
   // the kernel argument (that would move it to scratch)
   const Geo g = a.pc.g;
-  const size_t pitch = a.pc.pitch;
+  const Layout lay = a.pc.lay;
   const int x0 = blockIdx.x * LDS_XC, j = blockIdx.y, k = blockIdx.z;
   const int xc = (g.nx - x0 < LDS_XC) ? g.nx - x0 : LDS_XC;
   const S* src = static_cast<const S*>(a.pc.src);
@@ -434,8 +446,7 @@ __global__ void __launch_bounds__(lds_chunk<S>()) k_step_lds(StepArgs a) {
     const int ez = (int)((ZP >> q) & 1u) - (int)((ZN >> q) & 1u);
     const int sj = ey == 1 ? jm : (ey == -1 ? jp : j);
     const int sk = (D == 3) ? (ez == 1 ? km : (ez == -1 ? kp : k)) : 0;
-    const size_t addr = (size_t)q * pitch + (unsigned)x +
-                        (unsigned)g.nx * ((unsigned)sj + (unsigned)g.ny * (unsigned)sk);
+    const size_t addr = lay.at(q, (unsigned)sj + (unsigned)g.ny * (unsigned)sk, (unsigned)x);
     const V v = __builtin_nontemporal_load(reinterpret_cast<const V*>(src + addr));
     *reinterpret_cast<V*>(&win[q][G * gi]) = v;
   }
@@ -453,13 +464,13 @@ __global__ void __launch_bounds__(lds_chunk<S>()) k_step_lds(StepArgs a) {
   if (live) step_node<D, S>(a, i, j, k, node, f, [&](int q, T v) { res[q][t] = Store<S>::encode(v); });
   __syncthreads();
   S* dst = static_cast<S*>(a.dst);
-  const size_t row = (size_t)(unsigned)g.nx * ((unsigned)j + (unsigned)g.ny * (unsigned)k);
+  const unsigned row = (unsigned)j + (unsigned)g.ny * (unsigned)k;
   const int ngo = (xc == LDS_XC) ? NGO : xc / G;
   for (int e = threadIdx.x; e < Q * ngo; e += LDS_XC) {
     const int q = (xc == LDS_XC) ? e / NGO : e / ngo;
     const int gi = e - q * ngo;
     const V v = *reinterpret_cast<const V*>(&res[q][G * gi]);
-    __builtin_nontemporal_store(v, reinterpret_cast<V*>(dst + (size_t)q * pitch + row + x0 + G * gi));
+    __builtin_nontemporal_store(v, reinterpret_cast<V*>(dst + lay.at(q, row, (unsigned)(x0 + G * gi))));
   }
 }
 
@@ -575,7 +586,7 @@ __global__ void __launch_bounds__(256) k_macro_reduce(const double* partial, int
 // State import/export in the reference's ghost-inclusive layout.
 // ---------------------------------------------------------------------------
 template <int D, typename S>
-__global__ void k_import(const double* fin, void* dstv, size_t pitch, Geo g, double base_w_rho0,
+__global__ void k_import(const double* fin, void* dstv, Layout lay, Geo g, double base_w_rho0,
                          float scale) {
   using L = Lat<D>;
   constexpr int Q = L::Q;
@@ -585,26 +596,26 @@ __global__ void k_import(const double* fin, void* dstv, size_t pitch, Geo g, dou
   const int gx = g.nx + 2, gy = g.ny + 2, gz = (D == 3) ? g.nz + 2 : 1;
   const size_t nt = (size_t)gx * gy * gz;
   const size_t gn = (size_t)(i + 1) + (size_t)gx * ((size_t)(j + 1) + (size_t)gy * (D == 3 ? (size_t)(k + 1) : 0));
-  const unsigned node = (unsigned)i + (unsigned)g.nx * ((unsigned)j + (unsigned)g.ny * (unsigned)k);
+  const unsigned row = (unsigned)j + (unsigned)g.ny * (unsigned)k;
   S* dst = static_cast<S*>(dstv);
 #pragma unroll
   for (int q = 0; q < Q; q++) {
     double v = fin[q * nt + gn];
     if (Store<S>::deviation) v = (v - L::w(q) * base_w_rho0) * scale;
-    Store<S>::store(dst + q * pitch + node, (typename Store<S>::T)v);
+    Store<S>::store(dst + lay.at(q, row, (unsigned)i), (typename Store<S>::T)v);
   }
 }
 
 // Grid::init_equilibrium(rho0, ux0, uy0, uz0), couplb_lattice.h:137-154
 template <int D, typename S>
-__global__ void k_init_eq(void* dstv, size_t pitch, Geo g, double r, double u0, double u1,
+__global__ void k_init_eq(void* dstv, Layout lay, Geo g, double r, double u0, double u1,
                           double u2, double base, float scale) {
   using L = Lat<D>;
   constexpr int Q = L::Q;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y, k = blockIdx.z;
   if (i >= g.nx) return;
-  const unsigned node = (unsigned)i + (unsigned)g.nx * ((unsigned)j + (unsigned)g.ny * (unsigned)k);
+  const unsigned row = (unsigned)j + (unsigned)g.ny * (unsigned)k;
   S* dst = static_cast<S*>(dstv);
   const double cs2 = 1.0 / 3.0, cs4 = cs2 * cs2;
   const double uu = u0 * u0 + u1 * u1 + u2 * u2;
@@ -613,7 +624,7 @@ __global__ void k_init_eq(void* dstv, size_t pitch, Geo g, double r, double u0, 
     const double eu = L::ex(q) * u0 + L::ey(q) * u1 + L::ez(q) * u2;
     double v = L::w(q) * r * (1.0 + eu / cs2 + 0.5 * eu * eu / cs4 - 0.5 * uu / cs2);
     if (Store<S>::deviation) v = (v - L::w(q) * base) * scale;
-    Store<S>::store(dst + q * pitch + node, (typename Store<S>::T)v);
+    Store<S>::store(dst + lay.at(q, row, (unsigned)i), (typename Store<S>::T)v);
   }
 }
 

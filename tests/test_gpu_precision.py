@@ -75,29 +75,45 @@ def test_reduced_storage_error_vs_fp64(fp64, storage):
     assert err < BOUNDS[storage]
 
 
-@pytest.mark.parametrize("storage", ["fp32", "fp16", "bf16"])
-def test_staged_step_equals_per_node_step(storage, monkeypatch):
-    """The reduced formats run the LDS-staged step (8-byte row groups); a
-    plane pitch that is not a multiple of 4 elements selects the per-node
-    step instead.  Both must produce identical bits, walls and IBM included."""
+def _walled_ibm_state(storage, env, monkeypatch):
+    """30 LB steps with moving / free-slip walls, gravity and fixed IBM
+    markers under the given environment (layout knobs); the final state."""
     from tests.scenarios import DeviceArray
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
     walls = ((0, 0), (2, 1), (3, 3))
     face = [[[0, 0, 0], [0, 0, 0]], [[0.01, 0, 0], [0, 0, 0]], [[0, 0, 0], [0, 0, 0]]]
     rng = np.random.default_rng(6)
     x = rng.uniform(2.0, 14.0, (40, 3))
-    states = []
-    for pad in ("256", "257"):
-        monkeypatch.setenv("COUPLB_PLANE_PAD", pad)
-        d = couplb_amd.Device(3, 32, 16, 16, tau=0.7, walls=walls, face_vel=face,
-                              kernel=couplb_amd.PESKIN4, storage=storage)
-        d.init_equilibrium(1.0, (0.01, 0.002, 0.0))
-        dx, dv = DeviceArray(x), DeviceArray(np.zeros((40, 3)))
-        dm, df = DeviceArray(np.ones(40)), DeviceArray(np.zeros((40, 3)))
-        for s in range(30):
-            d.step((1e-5, 0.0, 0.0), 0)
-            d.ibm_device(40, dx.addr, dv.addr, dm.addr, df.addr, (0, 0, 0), 1.0, 1.0, 1.0, 1.0, 1.0,
-                         positions_unchanged=s > 0)
+    d = couplb_amd.Device(3, 32, 16, 16, tau=0.7, walls=walls, face_vel=face,
+                          kernel=couplb_amd.PESKIN4, storage=storage)
+    d.init_equilibrium(1.0, (0.01, 0.002, 0.0))
+    dx, dv = DeviceArray(x), DeviceArray(np.zeros((40, 3)))
+    dm, df = DeviceArray(np.ones(40)), DeviceArray(np.zeros((40, 3)))
+    for s in range(30):
         d.step((1e-5, 0.0, 0.0), 0)
-        d.sync()
-        states.append(d.get_state())
+        d.ibm_device(40, dx.addr, dv.addr, dm.addr, df.addr, (0, 0, 0), 1.0, 1.0, 1.0, 1.0, 1.0,
+                     positions_unchanged=s > 0)
+    d.step((1e-5, 0.0, 0.0), 0)
+    d.sync()
+    return d.get_state()
+
+
+@pytest.mark.parametrize("layout,padvar", [("planes", "COUPLB_PLANE_PAD"), ("rows", "COUPLB_ROW_PAD")])
+@pytest.mark.parametrize("storage", ["fp32", "fp16", "bf16"])
+def test_staged_step_equals_per_node_step(storage, layout, padvar, monkeypatch):
+    """The reduced formats run the LDS-staged step (16-byte row groups); a
+    layout stride that is not a multiple of 4 elements selects the per-node
+    step instead.  Both must produce identical bits, walls and IBM included."""
+    states = [_walled_ibm_state(storage, {"COUPLB_LAYOUT": layout, padvar: pad}, monkeypatch)
+              for pad in ("256", "257")]
     assert np.array_equal(states[0], states[1])
+
+
+@pytest.mark.parametrize("storage", ["fp64", "fp16"])
+def test_layouts_give_identical_states(storage, monkeypatch):
+    """q planes and x-rows of all q are two address maps of the same
+    populations: every kernel must give the same bits in both."""
+    a = _walled_ibm_state(storage, {"COUPLB_LAYOUT": "planes"}, monkeypatch)
+    b = _walled_ibm_state(storage, {"COUPLB_LAYOUT": "rows"}, monkeypatch)
+    assert np.array_equal(a, b)

@@ -311,7 +311,16 @@ template <int D, typename S> struct LaunchUgather {
     // map, when it belongs to this ordering); measured faster for 16-bit
     // storage and slower for FP64
     const bool dense = sizeof(S) < 8 && c->pair_epoch == c->pos_epoch && 4 * (size_t)c->pnh > c->N;
-    if (dense)
+    constexpr int G = LDS_GB / (int)sizeof(S), XC = lds_chunk<S>();
+    const bool staged = !pc.local && c->geo.nx % G == 0 && c->lay.pq % G == 0 && c->lay.pr % G == 0;
+    if (dense && staged) {
+      if constexpr (Store<S>::deviation) {
+        const dim3 grid((unsigned)((c->geo.nx + XC - 1) / XC), (unsigned)c->geo.ny,
+                        (unsigned)(D == 3 ? c->geo.nz : 1));
+        hipLaunchKernelGGL((k_upull_lds<D, S>), grid, dim3(XC), 0, c->stream, pc, c->cover, c->u4, c->stamp,
+                           c->cycle);
+      }
+    } else if (dense)
       hipLaunchKernelGGL((k_upull<D, S, true>), row_grid(c->geo, b.x), b, 0, c->stream, pc, c->cover, c->u4,
                          c->stamp, c->cycle);
     else

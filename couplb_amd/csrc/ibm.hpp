@@ -186,21 +186,11 @@ __global__ void __launch_bounds__(TILE_THREADS) k_cover(Geo g, Walls w, BaseSlot
 // nodes are written; for coverage high enough that nearly every cache line
 // of the populations is needed anyway.  Otherwise only covered nodes not yet
 // computed for this state (stamp) pull.
-template <int D, typename S, bool DENSE>
-__global__ void __launch_bounds__(256) k_upull(PullCtx pc, const unsigned char* cover, double* u4,
-                                               int* stamp, int cycle) {
+// u = j/rho and rho from the pulled populations (compute_macroscopic(false))
+template <int D, typename S>
+__device__ inline double4 upull_moments(const typename Store<S>::T (&f)[Lat<D>::Q], double base) {
   using L = Lat<D>;
-  using T = typename Store<S>::T;
   constexpr int Q = L::Q;
-  const Geo& g = pc.g;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y, k = blockIdx.z;
-  if (i >= g.nx) return;
-  const unsigned node = (unsigned)i + (unsigned)g.nx * ((unsigned)j + (unsigned)g.ny * (unsigned)k);
-  const bool want = cover[node] && (DENSE || stamp[node] != cycle);
-  if (!DENSE && !want) return;
-  T f[Q];
-  pull_node<D, S>(pc, i, j, k, f);
   double A = 0, jx = 0, jy = 0, jz = 0;
 #pragma unroll
   for (int q = 0; q < Q; q++) {
@@ -210,7 +200,7 @@ __global__ void __launch_bounds__(256) k_upull(PullCtx pc, const unsigned char* 
     jy += L::ey(q) * fv;
     jz += L::ez(q) * fv;
   }
-  double r = A + pc.base;
+  double r = A + base;
   if (r <= RHO_MIN) r = RHO_MIN;
   if (!isfinite(r)) r = RHO_MIN;
   const double inv_r = 1.0 / r;
@@ -219,8 +209,59 @@ __global__ void __launch_bounds__(256) k_upull(PullCtx pc, const unsigned char* 
   out.y = jy * inv_r;
   out.z = jz * inv_r;
   out.w = r;
+  return out;
+}
+
+template <int D, typename S, bool DENSE>
+__global__ void __launch_bounds__(256) k_upull(PullCtx pc, const unsigned char* cover, double* u4,
+                                               int* stamp, int cycle) {
+  using T = typename Store<S>::T;
+  const Geo& g = pc.g;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y, k = blockIdx.z;
+  if (i >= g.nx) return;
+  const unsigned node = (unsigned)i + (unsigned)g.nx * ((unsigned)j + (unsigned)g.ny * (unsigned)k);
+  const bool want = cover[node] && (DENSE || stamp[node] != cycle);
+  if (!DENSE && !want) return;
+  T f[Lat<D>::Q];
+  pull_node<D, S>(pc, i, j, k, f);
+  const double4 out = upull_moments<D, S>(f, pc.base);
   if (!want) return;
   *reinterpret_cast<double4*>(u4 + 4 * (size_t)node) = out;
+  stamp[node] = cycle;
+}
+
+// The dense gather of the reduced formats with the source rows of a 256-node
+// chunk staged through LDS in 16-byte groups (stage_rows, as k_step_lds):
+// the same values at every covered node, with 16-byte instead of 2- or
+// 4-byte global loads.  Needs what k_step_lds needs (strides multiple of
+// the group, state in pull form).
+template <int D, typename S>
+__global__ void __launch_bounds__(lds_chunk<S>()) k_upull_lds(PullCtx pc, const unsigned char* cover,
+                                                              double* u4, int* stamp, int cycle) {
+  constexpr int XC = lds_chunk<S>();
+  using L = Lat<D>;
+  using T = typename Store<S>::T;
+  constexpr int Q = L::Q;
+  constexpr int G = LDS_GB / (int)sizeof(S);
+  constexpr int W = XC + 2 * G;
+  __shared__ __attribute__((aligned(16))) S lds[Q * W];
+  S (*win)[W] = reinterpret_cast<S (*)[W]>(lds);
+  const Geo g = pc.g;
+  const int x0 = blockIdx.x * XC, j = blockIdx.y, k = blockIdx.z;
+  const int xc = (g.nx - x0 < XC) ? g.nx - x0 : XC;
+  stage_rows<D, S, XC>(g, pc.lay, static_cast<const S*>(pc.src), lds, x0, xc, j, k);
+  __syncthreads();
+  const int t = threadIdx.x;
+  const int i = x0 + t;
+  if (t >= xc) return;
+  const unsigned node = (unsigned)i + (unsigned)g.nx * ((unsigned)j + (unsigned)g.ny * (unsigned)k);
+  if (!cover[node]) return;
+  T f[Q];
+  pull_node_with<D, S>(pc, i, j, k, f, [&](int q, int, int, int) {
+    return Store<S>::decode(win[q][t + G - L::ex(q)]);
+  });
+  *reinterpret_cast<double4*>(u4 + 4 * (size_t)node) = upull_moments<D, S>(f, pc.base);
   stamp[node] = cycle;
 }
 

@@ -404,30 +404,19 @@ __global__ void __launch_bounds__(256) k_step(StepArgs a) {
 constexpr int LDS_GB = 16;  // bytes per staged group
 // nodes per block
 template <typename S> constexpr int lds_chunk() { return 256; }
-template <int D, typename S>
-__global__ void __launch_bounds__(lds_chunk<S>()) k_step_lds(StepArgs a) {
-  constexpr int LDS_XC = lds_chunk<S>();
+// Stage the Q source rows of the chunk [x0, x0 + xc) of row (j, k) -- row
+// (j - e_y, k - e_z) of plane q over the window [x0 - G, x0 + XC + G), x
+// wrapped periodically -- into win (W = XC + 2G elements per q), one
+// 16-byte group per lane and load.  Arguments by value: a kernel argument
+// whose address is taken moves to scratch.
+template <int D, typename S, int XC>
+__device__ inline void stage_rows(Geo g, Layout lay, const S* src, S* win, int x0, int xc, int j, int k) {
   using L = Lat<D>;
-  using T = typename Store<S>::T;
   typedef unsigned int V __attribute__((ext_vector_type(4)));  // one 16-byte group
   constexpr int Q = L::Q;
-  constexpr int G = LDS_GB / (int)sizeof(S);  // elements per group
-  constexpr int W = LDS_XC + 2 * G;          // window [x0 - G, x0 + XC + G)
-  constexpr int NGR = W / G;                 // groups per staged row (full chunk)
-  constexpr int NGO = LDS_XC / G;            // groups per result row (full chunk)
-  // one LDS buffer: the staged windows, then (after every lane has pulled
-  // its populations into registers) the results
-  constexpr int LDS_ELEMS = (Q * W > Q * LDS_XC) ? Q * W : Q * LDS_XC;
-  __shared__ __attribute__((aligned(16))) S lds[LDS_ELEMS];
-  S (*win)[W] = reinterpret_cast<S (*)[W]>(lds);
-  S (*res)[LDS_XC] = reinterpret_cast<S (*)[LDS_XC]>(lds);
-  // copies, not references: the lambdas below must not take the address of
-  // the kernel argument (that would move it to scratch)
-  const Geo g = a.pc.g;
-  const Layout lay = a.pc.lay;
-  const int x0 = blockIdx.x * LDS_XC, j = blockIdx.y, k = blockIdx.z;
-  const int xc = (g.nx - x0 < LDS_XC) ? g.nx - x0 : LDS_XC;
-  const S* src = static_cast<const S*>(a.pc.src);
+  constexpr int G = LDS_GB / (int)sizeof(S);
+  constexpr int W = XC + 2 * G;
+  constexpr int NGR = W / G;  // groups per staged row (full chunk)
   const int jm = (j == 0) ? g.ny - 1 : j - 1, jp = (j == g.ny - 1) ? 0 : j + 1;
   int km = 0, kp = 0;
   if (D == 3) { km = (k == 0) ? g.nz - 1 : k - 1; kp = (k == g.nz - 1) ? 0 : k + 1; }
@@ -436,9 +425,9 @@ __global__ void __launch_bounds__(lds_chunk<S>()) k_step_lds(StepArgs a) {
   constexpr unsigned YN = [] { unsigned m = 0; for (int q = 0; q < Q; q++) m |= (L::ey(q) == -1) << q; return m; }();
   constexpr unsigned ZP = [] { unsigned m = 0; for (int q = 0; q < Q; q++) m |= (L::ez(q) == 1) << q; return m; }();
   constexpr unsigned ZN = [] { unsigned m = 0; for (int q = 0; q < Q; q++) m |= (L::ez(q) == -1) << q; return m; }();
-  const int ngr = (xc == LDS_XC) ? NGR : (xc + 2 * G) / G;
-  for (int e = threadIdx.x; e < Q * ngr; e += LDS_XC) {
-    const int q = (xc == LDS_XC) ? e / NGR : e / ngr;
+  const int ngr = (xc == XC) ? NGR : (xc + 2 * G) / G;
+  for (int e = threadIdx.x; e < Q * ngr; e += XC) {
+    const int q = (xc == XC) ? e / NGR : e / ngr;
     const int gi = e - q * ngr;
     int x = x0 - G + G * gi;
     x = (x < 0) ? x + g.nx : (x >= g.nx ? x - g.nx : x);
@@ -448,8 +437,31 @@ __global__ void __launch_bounds__(lds_chunk<S>()) k_step_lds(StepArgs a) {
     const int sk = (D == 3) ? (ez == 1 ? km : (ez == -1 ? kp : k)) : 0;
     const size_t addr = lay.at(q, (unsigned)sj + (unsigned)g.ny * (unsigned)sk, (unsigned)x);
     const V v = __builtin_nontemporal_load(reinterpret_cast<const V*>(src + addr));
-    *reinterpret_cast<V*>(&win[q][G * gi]) = v;
+    *reinterpret_cast<V*>(&win[q * W + G * gi]) = v;
   }
+}
+
+template <int D, typename S>
+__global__ void __launch_bounds__(lds_chunk<S>()) k_step_lds(StepArgs a) {
+  constexpr int LDS_XC = lds_chunk<S>();
+  using L = Lat<D>;
+  using T = typename Store<S>::T;
+  typedef unsigned int V __attribute__((ext_vector_type(4)));  // one 16-byte group
+  constexpr int Q = L::Q;
+  constexpr int G = LDS_GB / (int)sizeof(S);  // elements per group
+  constexpr int W = LDS_XC + 2 * G;          // window [x0 - G, x0 + XC + G)
+  constexpr int NGO = LDS_XC / G;            // groups per result row (full chunk)
+  // one LDS buffer: the staged windows, then (after every lane has pulled
+  // its populations into registers) the results
+  constexpr int LDS_ELEMS = (Q * W > Q * LDS_XC) ? Q * W : Q * LDS_XC;
+  __shared__ __attribute__((aligned(16))) S lds[LDS_ELEMS];
+  S (*win)[W] = reinterpret_cast<S (*)[W]>(lds);
+  S (*res)[LDS_XC] = reinterpret_cast<S (*)[LDS_XC]>(lds);
+  const Geo g = a.pc.g;
+  const Layout lay = a.pc.lay;
+  const int x0 = blockIdx.x * LDS_XC, j = blockIdx.y, k = blockIdx.z;
+  const int xc = (g.nx - x0 < LDS_XC) ? g.nx - x0 : LDS_XC;
+  stage_rows<D, S, LDS_XC>(g, lay, static_cast<const S*>(a.pc.src), lds, x0, xc, j, k);
   __syncthreads();
   const int t = threadIdx.x;
   const bool live = t < xc;

@@ -161,6 +161,20 @@ __device__ inline void pull_node_with(const PullCtx& pc, int i, int j, int k,
   const bool any_moving = (xlo && pc.w.type[0][0] == 2) || (xhi && pc.w.type[0][1] == 2) ||
                           (ylo && pc.w.type[1][0] == 2) || (yhi && pc.w.type[1][1] == 2) ||
                           (zlo && pc.w.type[2][0] == 2) || (zhi && pc.w.type[2][1] == 2);
+  if (!(xlo || xhi || ylo || yhi || zlo || zhi)) {
+    // no wall face next to this node (most of the grid): every population
+    // comes from its fluid or periodic source, as in the loop below
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+      const int ex = L::ex(q), ey = L::ey(q), ez = L::ez(q);
+      const int si = ex == 1 ? im : (ex == -1 ? ip : i);
+      const int sj = ey == 1 ? jm : (ey == -1 ? jp : j);
+      const int sk = (D == 3) ? (ez == 1 ? km : (ez == -1 ? kp : k)) : 0;
+      f[q] = fluid(q, si, sj, sk);
+      if (Store<S>::deviation) f[q] *= (T)pc.inv_scale;
+    }
+    return;
+  }
   T rw = 0;
   if (any_moving) {
     const double r = pc.rho_coll[node];

@@ -281,37 +281,54 @@ __device__ inline void interp_plane(const Geo& g, const Walls& w, bool fin, int 
   s[0] = s[1] = s[2] = s[3] = s[4] = 0.0;
   const int c = (D == 3) ? k0 - 1 + vv : 0;
   if (!fin || (D == 3 && (c < 0 || c > g.nz + 1))) return;
+  // two rows of the plane at a time: first the eight velocity loads (all in
+  // flight together), then the sums in the reference's node order
 #pragma unroll
-  for (int u = 0; u < 4; u++) {
-    const int b = j0 - 1 + u;
-    if (b < 0 || b > g.ny + 1) continue;
+  for (int u0 = 0; u0 < 4; u0 += 2) {
+    int nt[2][4], wd[2][4], wsd[2][4];
+    double un[2][4][3];
 #pragma unroll
-    for (int t = 0; t < 4; t++) {
-      const int a = i0 - 1 + t;
-      if (a < 0 || a > g.nx + 1) continue;
-      int wd, wsd;
-      const int nt = node_type(w, g, a, b, c, wd, wsd);
-      const double d = (D == 3) ? (wx[t] * wy[u]) * wzv : wx[t] * wy[u];
-      if (nt == 0) {
-        const unsigned node = (unsigned)wrap_ghost(a, g.nx) +
-                              (unsigned)g.nx * ((unsigned)wrap_ghost(b, g.ny) +
-                                                (unsigned)g.ny * (unsigned)((D == 3) ? wrap_ghost(c, g.nz) : 0));
-        double un[3];
-        vel(node, un);
-        s[0] += un[0] * d;
-        s[1] += un[1] * d;
-        s[2] += un[2] * d;
-        s[3] += d;
-        s[4] += d;
-      } else if (nt == 1 || nt == 3) {
-        s[3] += d;
-      } else if (nt == 2) {
-        if (wall_vel) {
-          s[0] += w.vel[wd][wsd][0] * d;
-          s[1] += w.vel[wd][wsd][1] * d;
-          s[2] += ((D == 3) ? w.vel[wd][wsd][2] : 0.0) * d;
+    for (int h = 0; h < 2; h++) {
+      const int b = j0 - 1 + u0 + h;
+#pragma unroll
+      for (int t = 0; t < 4; t++) {
+        const int a = i0 - 1 + t;
+        const bool in = b >= 0 && b <= g.ny + 1 && a >= 0 && a <= g.nx + 1;
+        nt[h][t] = in ? node_type(w, g, a, b, c, wd[h][t], wsd[h][t]) : -1;
+        un[h][t][0] = un[h][t][1] = un[h][t][2] = 0.0;
+        if (nt[h][t] == 0) {
+          const unsigned node = (unsigned)wrap_ghost(a, g.nx) +
+                                (unsigned)g.nx * ((unsigned)wrap_ghost(b, g.ny) +
+                                                  (unsigned)g.ny * (unsigned)((D == 3) ? wrap_ghost(c, g.nz) : 0));
+          vel(node, un[h][t]);
         }
-        s[3] += d;
+      }
+    }
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int u = u0 + h;
+#pragma unroll
+      for (int t = 0; t < 4; t++) {
+        const int tp = nt[h][t];
+        if (tp < 0) continue;
+        const double d = (D == 3) ? (wx[t] * wy[u]) * wzv : wx[t] * wy[u];
+        if (tp == 0) {
+          s[0] += un[h][t][0] * d;
+          s[1] += un[h][t][1] * d;
+          s[2] += un[h][t][2] * d;
+          s[3] += d;
+          s[4] += d;
+        } else if (tp == 1 || tp == 3) {
+          s[3] += d;
+        } else if (tp == 2) {
+          if (wall_vel) {
+            const double* wvel = w.vel[wd[h][t]][wsd[h][t]];
+            s[0] += wvel[0] * d;
+            s[1] += wvel[1] * d;
+            s[2] += ((D == 3) ? wvel[2] : 0.0) * d;
+          }
+          s[3] += d;
+        }
       }
     }
   }

@@ -262,8 +262,7 @@ __global__ void __launch_bounds__(TILE_THREADS) k_spread(SpreadArgs a) {
         for (int m = 0; m < XM; m++) {
 #pragma clang fp contract(off)
           if (!tn.xin[m]) continue;
-          for (int ia = 0; ia < tn.xa[m].n; ia++) {
-            const int aa = tn.xa[m].at(ia);
+          each_image(tn.xa[m], [&](int aa) {
             for (int kb = aa; kb <= aa + 3 && kb < nbin; kb++) {  // base cells aa-2 .. aa+1
               const int t = aa - kb + 3;
               for (int e = start[kb]; e < start[kb + 1]; e++) {
@@ -275,25 +274,25 @@ __global__ void __launch_bounds__(TILE_THREADS) k_spread(SpreadArgs a) {
 #pragma unroll
                   for (int q = 0; q < TK; q++) {
                     if (!tn.zin[q]) continue;
-                    for (int ib = 0; ib < tn.ya[r].n; ib++) {
-                      const int u = tn.ya[r].at(ib) - c.j0 + 1;
-                      if (u < 0 || u > 3) continue;
-                      for (int ic = 0; ic < tn.za[q].n; ic++) {
-                        const int v = (D == 3) ? tn.za[q].at(ic) - c.k0 + 1 : 0;
-                        if (v < 0 || v > 3) continue;
+                    each_image(tn.ya[r], [&](int bb) {
+                      const int u = bb - c.j0 + 1;
+                      if (u < 0 || u > 3) return;
+                      each_image(tn.za[q], [&](int cc) {
+                        const int v = (D == 3) ? cc - c.k0 + 1 : 0;
+                        if (v < 0 || v > 3) return;
                         const double d = (D == 3) ? (wxt * c.wy[u]) * c.wz[v] : wxt * c.wy[u];
                         const double wgt = d * c.sc;
                         acc[m][r][q][0] += c.fx * wgt;
                         acc[m][r][q][1] += c.fy * wgt;
                         acc[m][r][q][2] += c.fz * wgt;
                         hit[m][r][q] = true;
-                      }
-                    }
+                      });
+                    });
                   }
                 }
               }
             }
-          }
+          });
         }
       });
 #pragma unroll

@@ -147,8 +147,7 @@ __global__ void __launch_bounds__(TILE_THREADS) k_cover(Geo g, Walls w, BaseSlot
 #pragma unroll
         for (int m = 0; m < XM; m++) {
           if (!tn.xin[m]) continue;
-          for (int ia = 0; ia < tn.xa[m].n; ia++) {
-            const int aa = tn.xa[m].at(ia);
+          each_image(tn.xa[m], [&](int aa) {
             for (int kb = aa; kb <= aa + 3 && kb < nbin; kb++)
               for (int e = start[kb]; e < start[kb + 1]; e++) {
                 const Base& c = win[list[e]];
@@ -156,14 +155,15 @@ __global__ void __launch_bounds__(TILE_THREADS) k_cover(Geo g, Walls w, BaseSlot
                 for (int r = 0; r < TJ; r++)
 #pragma unroll
                   for (int q = 0; q < TK; q++)
-                    for (int ib = 0; ib < tn.ya[r].n; ib++)
-                      for (int ic = 0; ic < tn.za[q].n; ic++) {
-                        const int u = tn.ya[r].at(ib) - c.j0 + 1;
-                        const int v = (D == 3) ? tn.za[q].at(ic) - c.k0 + 1 : 0;
+                    each_image(tn.ya[r], [&](int bb) {
+                      each_image(tn.za[q], [&](int cc) {
+                        const int u = bb - c.j0 + 1;
+                        const int v = (D == 3) ? cc - c.k0 + 1 : 0;
                         if (u >= 0 && u <= 3 && v >= 0 && v <= 3) cov[m][r][q] = true;
-                      }
+                      });
+                    });
               }
-          }
+          });
         }
       });
 #pragma unroll

@@ -80,8 +80,15 @@ __device__ inline int wrap_ghost(int a, int n) {
 // maps onto it.  Faces carrying a wall never receive spread force.
 struct Img {
   int n, a0, a1, a2;
-  __device__ int at(int k) const { return k == 0 ? a0 : (k == 1 ? a1 : a2); }
 };
+// f(a) for each image in order a0, a1, a2 (straight-line code: an indexed
+// pick of a0..a2 would put the images in scratch memory)
+template <typename F>
+__device__ inline void each_image(const Img& im, F&& f) {
+  f(im.a0);
+  if (im.n > 1) f(im.a1);
+  if (im.n > 2) f(im.a2);
+}
 __device__ inline Img images(int I, int n, int tlo, int thi) {
   Img r{1, I, -9, -9};
   const bool lo = (I == n && tlo == 0), hi = (I == 1 && thi == 0);

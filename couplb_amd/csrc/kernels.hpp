@@ -274,6 +274,21 @@ __device__ inline double edot(int q, double v0, double v1, double v2) {
   return s;
 }
 
+// e_q . v for the reduced-precision collision: the nonzero terms only, so no
+// multiplications by 0 or +-1 are issued (they cannot be dropped by the
+// compiler under IEEE rules)
+template <int D, typename T>
+__device__ inline T esum(int q, T v0, T v1, T v2) {
+  using L = Lat<D>;
+  const int ex = L::ex(q), ey = L::ey(q), ez = L::ez(q);
+  T s = (T)0;
+  bool any = false;
+  if (ex) { s = ex > 0 ? v0 : -v0; any = true; }
+  if (ey) { s = any ? (ey > 0 ? s + v1 : s - v1) : (ey > 0 ? v1 : -v1); any = true; }
+  if (D == 3 && ez) { s = any ? (ez > 0 ? s + v2 : s - v2) : (ez > 0 ? v2 : -v2); }
+  return s;
+}
+
 // BGK + Guo in FP64 with the reference's expression order and rounding
 // (couplb_lattice.h:196-200 velocity, couplb_collision.h:36-58 collision):
 // the result equals the reference's to the last bit.
@@ -381,18 +396,32 @@ __device__ inline void step_node(const StepArgs& a, int i, int j, int k, unsigne
     const T uu = ux * ux + uy * uy + uz * uz;
     const T uf = ux * Fx + uy * Fy + uz * Fz;
     const T scale = (T)a.pc.scale;
+    // BGK + Guo per direction (couplb_collision.h:36-58), in deviation form:
+    //   feq - w*base = w*(A + rho*(3eu + 4.5eu^2 - 1.5uu))   (couplb_lattice.h:137-144)
+    //   S = w*(1-om/2)*(3(e.F - u.F) + 9(e.u)(e.F))
+    //   f' = f - om*(f - feq) + S = (1-om)*f + w*(om*sym + Ssym +- (om*asym + Sasym))
+    // with the parts even in e (sym = A + rho*(4.5eu^2 - 1.5uu), Ssym =
+    // 9 hom eu ef - 3 hom uf) shared by q and opp(q), and the odd parts
+    // (asym = 3 rho eu, Sasym = 3 hom ef) changing sign: one evaluation per
+    // pair of opposite directions.  The storage scale is folded in.
+    const T omc_s = ((T)1 - om) * scale;
+    const T uu15 = (T)1.5 * uu, rho3 = (T)3 * rho;
+    const T hom3 = (T)3 * hom, hom9 = (T)9 * hom, huf3 = hom3 * uf;
 #pragma unroll
     for (int q = 0; q < Q; q++) {
-      const T cx = (T)L::ex(q), cy = (T)L::ey(q), cz = (T)L::ez(q);
-      const T wq = (T)L::w(q);
-      const T eu = cx * ux + cy * uy + cz * uz;
-      const T ef = cx * Fx + cy * Fy + cz * Fz;
-      // feq - w*base = w*(A + rho*(3eu + 4.5eu^2 - 1.5uu))   (couplb_lattice.h:137-144)
-      const T feq = wq * (A + rho * ((T)3 * eu + (T)4.5 * eu * eu - (T)1.5 * uu));
-      // Guo source w(1-om/2)[(e.F - u.F)/cs^2 + (e.u)(e.F)/cs^4]  (couplb_collision.h:51-52)
-      const T Si = wq * hom * ((T)3 * (ef - uf) + (T)9 * eu * ef);
-      T fn = f[q] - om * (f[q] - feq) + Si;
-      out(q, fn * scale);
+      const int o = L::opp(q);
+      if (o < q) continue;
+      const T eu = esum<D, T>(q, ux, uy, uz);
+      const T ef = esum<D, T>(q, Fx, Fy, Fz);
+      const T ws = (T)L::w(q) * scale;
+      const T P = om * (A + rho * ((T)4.5 * eu * eu - uu15)) + (hom9 * eu * ef - huf3);
+      if (o == q) {
+        out(q, omc_s * f[q] + ws * P);
+        continue;
+      }
+      const T M = om * (rho3 * eu) + hom3 * ef;
+      out(q, omc_s * f[q] + ws * (P + M));
+      out(o, omc_s * f[o] + ws * (P - M));
     }
   }
 }

@@ -399,12 +399,12 @@ __device__ inline void step_node(const StepArgs& a, int i, int j, int k, unsigne
     // BGK + Guo per direction (couplb_collision.h:36-58), in deviation form:
     //   feq - w*base = w*(A + rho*(3eu + 4.5eu^2 - 1.5uu))   (couplb_lattice.h:137-144)
     //   S = w*(1-om/2)*(3(e.F - u.F) + 9(e.u)(e.F))
-    //   f' = f - om*(f - feq) + S = (1-om)*f + w*(om*sym + Ssym +- (om*asym + Sasym))
-    // with the parts even in e (sym = A + rho*(4.5eu^2 - 1.5uu), Ssym =
-    // 9 hom eu ef - 3 hom uf) shared by q and opp(q), and the odd parts
-    // (asym = 3 rho eu, Sasym = 3 hom ef) changing sign: one evaluation per
-    // pair of opposite directions.  The storage scale is folded in.
-    const T omc_s = ((T)1 - om) * scale;
+    //   f' = f - om*(f - feq) + S
+    // The parts even in e (sym = A + rho*(4.5eu^2 - 1.5uu), Ssym =
+    // 9 hom eu ef - 3 hom uf) are shared by q and opp(q); the odd parts
+    // (asym = 3 rho eu, Sasym = 3 hom ef) change sign: one evaluation per
+    // pair of opposite directions.  f - feq stays one difference, as in the
+    // reference's form, so the non-equilibrium part keeps its precision.
     const T uu15 = (T)1.5 * uu, rho3 = (T)3 * rho;
     const T hom3 = (T)3 * hom, hom9 = (T)9 * hom, huf3 = hom3 * uf;
 #pragma unroll
@@ -413,15 +413,16 @@ __device__ inline void step_node(const StepArgs& a, int i, int j, int k, unsigne
       if (o < q) continue;
       const T eu = esum<D, T>(q, ux, uy, uz);
       const T ef = esum<D, T>(q, Fx, Fy, Fz);
-      const T ws = (T)L::w(q) * scale;
-      const T P = om * (A + rho * ((T)4.5 * eu * eu - uu15)) + (hom9 * eu * ef - huf3);
+      const T wq = (T)L::w(q);
+      const T sym = A + rho * ((T)4.5 * eu * eu - uu15);
+      const T Ssym = hom9 * eu * ef - huf3;
       if (o == q) {
-        out(q, omc_s * f[q] + ws * P);
+        out(q, (f[q] - om * (f[q] - wq * sym) + wq * Ssym) * scale);
         continue;
       }
-      const T M = om * (rho3 * eu) + hom3 * ef;
-      out(q, omc_s * f[q] + ws * (P + M));
-      out(o, omc_s * f[o] + ws * (P - M));
+      const T asym = rho3 * eu, Sasym = hom3 * ef;
+      out(q, (f[q] - om * (f[q] - wq * (sym + asym)) + wq * (Ssym + Sasym)) * scale);
+      out(o, (f[o] - om * (f[o] - wq * (sym - asym)) + wq * (Ssym - Sasym)) * scale);
     }
   }
 }

@@ -322,10 +322,10 @@ template <int D, typename S> struct LaunchUgather {
       }
     } else if (dense)
       hipLaunchKernelGGL((k_upull<D, S, true>), row_grid(c->geo, b.x), b, 0, c->stream, pc, c->cover, c->u4,
-                         c->stamp, c->cycle);
+                         c->stamp, c->cycle, 1);
     else
       hipLaunchKernelGGL((k_upull<D, S, false>), row_grid(c->geo, b.x), b, 0, c->stream, pc, c->cover,
-                         c->u4, c->stamp, c->cycle);
+                         c->u4, c->stamp, c->cycle, c->ugather_cycle != c->cycle ? 1 : 0);
   }
 };
 

@@ -212,16 +212,17 @@ __device__ inline double4 upull_moments(const typename Store<S>::T (&f)[Lat<D>::
   return out;
 }
 
+// fresh: first gather of this state, every stamp is older (no stamp reads)
 template <int D, typename S, bool DENSE>
 __global__ void __launch_bounds__(256) k_upull(PullCtx pc, const unsigned char* cover, double* u4,
-                                               int* stamp, int cycle) {
+                                               int* stamp, int cycle, int fresh) {
   using T = typename Store<S>::T;
   const Geo& g = pc.g;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y, k = blockIdx.z;
   if (i >= g.nx) return;
   const unsigned node = (unsigned)i + (unsigned)g.nx * ((unsigned)j + (unsigned)g.ny * (unsigned)k);
-  const bool want = cover[node] && (DENSE || stamp[node] != cycle);
+  const bool want = cover[node] && (DENSE || fresh || stamp[node] != cycle);
   if (!DENSE && !want) return;
   T f[Lat<D>::Q];
   pull_node<D, S>(pc, i, j, k, f);

@@ -195,10 +195,10 @@ __device__ inline double4 upull_moments(const typename Store<S>::T (&f)[Lat<D>::
 #pragma unroll
   for (int q = 0; q < Q; q++) {
     const double fv = (double)f[q];
-    A += fv;
-    jx += L::ex(q) * fv;
-    jy += L::ey(q) * fv;
-    jz += L::ez(q) * fv;
+    A += fv;  // c.f with the nonzero terms only: the same sums, fewer FP64 ops
+    if (L::ex(q) > 0) jx += fv; else if (L::ex(q) < 0) jx -= fv;
+    if (L::ey(q) > 0) jy += fv; else if (L::ey(q) < 0) jy -= fv;
+    if (L::ez(q) > 0) jz += fv; else if (L::ez(q) < 0) jz -= fv;
   }
   double r = A + base;
   if (r <= RHO_MIN) r = RHO_MIN;

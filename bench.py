@@ -30,6 +30,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+# algorithmic bytes of one D3Q19 Peskin4 FP64 marker sub-step: interpolate
+# 64 nodes x 3 x 8 B, spread 64 x 3 x 8 B, marker I/O x 24 + v 24 + m 8 + f RMW 48
+IBM_BYTES_PER_MARKER = 64 * 3 * 8 * 2 + 104
 Q = 19
 
 
@@ -458,6 +461,7 @@ def main():
     lb_tot, lb_n = dev.kernel_ms_total(0)
     ibm_tot, ibm_n = dev.kernel_ms_total(1)
     sp_tot, sp_n = dev.kernel_ms_total(2)
+    ga_tot, ga_n = dev.kernel_ms_total(4)
     nodes = N ** 3
     mlups = nodes * args.steps / wall / 1e6
     s_bytes = {"fp64": 8, "fp32": 4, "fp16": 2, "bf16": 2}[args.storage]
@@ -469,6 +473,11 @@ def main():
     # IBM time per marker sub-step: the sub-step kernels plus this sub-step's
     # share of the force spreading that precedes the next LB kernel
     markers_per_s = n / ((ibm_avg + spread_avg / md) * 1e-3) if n else 0.0
+    # the same without the fluid-velocity gather of the first sub-step of each
+    # LB step, which is reported on its own (a grid operation, not per marker)
+    gather_avg = ga_tot / max(args.steps, 1)
+    ibm_wo = ibm_avg - ga_tot / max(ibm_n, 1)
+    markers_per_s_wo = n / ((ibm_wo + spread_avg / md) * 1e-3) if n else 0.0
     diag = dev.diagnostics()
 
     # LB-only throughput of the same grid (no markers), for reference
@@ -521,7 +530,12 @@ def main():
                      "kernel": f"k_step<3,{args.storage}> (fused pull + IBM force gather)",
                      "bytes_per_lup": b_lup, "avg_kernel_ms": round(lb_avg, 4)},
         "ibm": {"markers_per_s": round(markers_per_s, 1), "avg_substep_ms": round(ibm_avg, 4),
-                "avg_spread_ms_per_lb_step": round(spread_avg, 4), "markers": n},
+                "avg_spread_ms_per_lb_step": round(spread_avg, 4), "markers": n,
+                "gather_ms_per_lb_step": round(gather_avg, 4),
+                "markers_per_s_without_gather": round(markers_per_s_wo, 1),
+                "bytes_per_marker_substep": IBM_BYTES_PER_MARKER,
+                "hbm_frac_without_gather": round(markers_per_s_wo * IBM_BYTES_PER_MARKER
+                                                 / (HBM_PEAK_GBS * 1e9), 4)},
         "mlups_markers_moving": round(mlups_moving, 1),
         "lb_only_mlups": round(lb_only, 1),
         "lb_only_hbm_frac": round(lb_only * 1e6 * b_lup / (HBM_PEAK_GBS * 1e9), 4),

@@ -322,7 +322,8 @@ class Device:
 
     def kernel_ms_total(self, which=0):
         """(total milliseconds, count) of the timed regions of one kind since
-        the last reset: 0 LB kernel, 1 IBM sub-step, 2 spreading, 3 MDF."""
+        the last reset: 0 LB kernel, 1 IBM sub-step, 2 spreading, 3 MDF, 4 fluid
+        velocity gather at covered nodes (inside 1 or 3)."""
         t, n = C.c_double(), C.c_long()
         self._chk(lib().couplb_kernel_ms_total(self.h, which, C.byref(t), C.byref(n)),
                   "kernel_ms_total")

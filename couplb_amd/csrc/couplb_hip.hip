@@ -31,13 +31,14 @@ namespace {
 // when its ring entry is reused or when the caller asks for the times.
 struct TimerRing {
   static constexpr int R = 256;
-  static constexpr int W = 4;  // 0 LB kernel, 1 IBM sub-step, 2 spreading, 3 MDF solve
+  static constexpr int W = 5;  // 0 LB kernel, 1 IBM sub-step, 2 spreading, 3 MDF solve,
+                               // 4 velocity gather (inside 1 or 3)
   hipEvent_t a[R] = {}, b[R] = {};
   int which[R] = {};
   bool pending[R] = {};
   int head = 0;
-  double total[W] = {0, 0, 0, 0}, last[W] = {0, 0, 0, 0};
-  long count[W] = {0, 0, 0, 0};
+  double total[W] = {0, 0, 0, 0, 0}, last[W] = {0, 0, 0, 0, 0};
+  long count[W] = {0, 0, 0, 0, 0};
 };
 
 struct SlotHost {
@@ -464,8 +465,13 @@ int prepare_markers(couplb_ctx* c, int n, const double* x, const IbmParams& ip,
     // coverage depends on the marker cells only, the velocities on the state
     BaseSlot bs{c->rowstart_cur, c->bases};
     PullCtx pc = make_pull(c);
+    int tg = -1;
+    int rc = timer_begin(c, 4, &tg);
+    if (rc) return rc;
     dispatch<LaunchUgather>(c, pc, bs, c->cover_epoch != c->pos_epoch);
     HIPCHK(c, hipGetLastError());
+    rc = timer_end(c, tg);
+    if (rc) return rc;
     c->cover_epoch = c->pos_epoch;
     c->ugather_cycle = c->cycle;
   }

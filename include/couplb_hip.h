@@ -221,7 +221,9 @@ void* couplb_stream(couplb_ctx* ctx);
 
 /* Kernel timing from HIP events on the context stream, read back without
  * stalling the stream: which = 0 fused LB kernel, 1 IBM sub-step (all its
- * kernels), 2 force spreading before an LB kernel, 3 MDF solve.
+ * kernels), 2 force spreading before an LB kernel, 3 MDF solve, 4 the fluid
+ * velocity gather at covered nodes (part of 1 or 3; once per LB state and
+ * marker ordering).
  * couplb_last_kernel_ms waits for the newest region of that kind and returns
  * its milliseconds; couplb_kernel_ms_total returns the sum and count of all
  * regions since the last reset. */

@@ -131,7 +131,7 @@ def pmc_traffic(storage, n):
 
 
 # ---------------------------------------------------------------------------
-# Further workloads (not the headline line): python bench.py --workload c2|c4|c5
+# Further workloads (not the headline line): python bench.py --workload c1|c2|c4|c5
 # ---------------------------------------------------------------------------
 D3Q19_E = np.array([[0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0],
                     [0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1],
@@ -152,6 +152,68 @@ def feq_field(rho, u):
 def ghost_coords(n):
     """cell-centre coordinate of ghost-inclusive index 0..n+1 (x = i - 1 + 1/2)"""
     return np.arange(-1, n + 1) + 0.5
+
+
+def workload_c1():
+    """D2Q9 8x32 Poiseuille channel (the reference's own CPU-runnable deck:
+    walls in y, g_x = 5e-6, nu = 0.1, rest start, 50,000 steps).  A step of
+    256 nodes is far shorter than a kernel launch: the run of steps goes
+    through couplb_lbm_steps_async, which keeps the state in LDS and runs the
+    steps inside one launch; one launch per step and the fix driven step by
+    step from Python are timed beside it, and the reference's pass
+    structure on host cores as the baseline."""
+    import couplb_amd
+    import oracle
+    steps, g = 50000, (5e-6, 0.0, 0.0)
+    walls = ((0, 0), (1, 1), (0, 0))
+    exact = 5e-6 / (2 * 0.1) * (np.arange(32) + 0.5) * (32 - 0.5 - np.arange(32))
+    out = {}
+    for name, bound in (("fused", None), ("launch_per_step", "0")):
+        if bound is not None:
+            os.environ["COUPLB_FUSED_NODES"] = bound
+        d = couplb_amd.Device(2, 8, 32, 1, tau=0.8, walls=walls)
+        os.environ.pop("COUPLB_FUSED_NODES", None)
+        d.init_equilibrium(1.0)
+        d.sync()
+        t0 = time.perf_counter()
+        d.steps_async(steps, g, 0)
+        d.sync()
+        dt = time.perf_counter() - t0
+        rho, ux, uy, uz = d.macro()
+        prof = ux.reshape(32, 8).mean(axis=1)
+        out[name] = dict(mlups=round(256 * steps / dt / 1e6, 1), us_per_step=round(dt / steps * 1e6, 3),
+                         fuses=d.fuses_steps(),
+                         profile_max_rel_err=float(np.max(np.abs(prof - exact)) / exact.max()))
+        d.close()
+    fx = couplb_amd.Fix("flow all couplb 8 32 1 0.1 1.0 wall_y 1 1 gravity 5e-6 0.0 0.0 check_every 2000",
+                        dimension=2, dt=1.0)
+    fx.init()
+    fx.setup(None)
+    t0 = time.perf_counter()
+    fx.run(steps)
+    couplb_amd.lib().couplb_synchronize(couplb_amd.lib().couplb_fix_context(fx.h))
+    dt = time.perf_counter() - t0
+    out["fix_python_driver"] = dict(mlups=round(256 * steps / dt / 1e6, 1),
+                                    us_per_step=round(dt / steps * 1e6, 3))
+    fx.close()
+    cpu = None
+    try:
+        fo = oracle.Fix(8, 32, 1, 0.1, 1.0, dim=2, walls=walls, gravity=g, numerics="ref")
+        kind = "reference"
+    except OSError:
+        fo = oracle.Fix(8, 32, 1, 0.1, 1.0, dim=2, walls=walls, gravity=g)
+        kind = "port"
+    fo.setup(None)
+    dt = fo.time_lbm_steps(steps, g)
+    fo.close()
+    cpu = dict(value=round(256 * steps / dt / 1e6, 2), unit="MLUPS", cores=oracle.omp_threads(), kind=kind,
+               sample=f"the whole workload: 50,000 steps, reference pass structure, {dt:.2f} s")
+    return dict(metric="D2Q9 8x32 channel MLUPS (steps after the first in one launch)", unit="MLUPS",
+                value=out["fused"]["mlups"], steps=steps, ms_per_step=round(1e3 / (out["fused"]["mlups"] * 1e6 / 256), 6),
+                data="synthetic (rest start)",
+                config={"workload": "D2Q9 8x32 channel, no-slip walls in y, g_x=5e-6, nu=0.1, rest start, "
+                                    "50,000 steps"},
+                results=out, cpu_baseline=cpu)
 
 
 def workload_c2(steps):
@@ -385,9 +447,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="time a plain device copy of MB megabytes (PMC byte-counter scale)")
-    ap.add_argument("--workload", default="c3", choices=["c3", "c2", "c4", "c5"],
-                    help="c3 (default): the headline line; c2 channel, c4 precision study, "
-                         "c5 learned MDF warm start")
+    ap.add_argument("--workload", default="c3", choices=["c3", "c1", "c2", "c4", "c5"],
+                    help="c3 (default): the headline line; c1 small 2D channel, c2 channel, "
+                         "c4 precision study, c5 learned MDF warm start")
     ap.add_argument("--grid", type=int, default=0, help="grid edge for c4 (default 384)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -403,7 +465,9 @@ def main():
         copy_calibration(args.copy_calibration, args.steps)
         return
     if args.workload != "c3":
-        if args.workload == "c2":
+        if args.workload == "c1":
+            out = workload_c1()
+        elif args.workload == "c2":
             out = workload_c2(args.steps)
         elif args.workload == "c4":
             out = workload_c4(args.steps, args.grid or 384)

@@ -110,6 +110,7 @@ def lib() -> C.CDLL:
         "couplb_init_equilibrium": (ip, [vp, C.c_double, C.c_double, C.c_double, C.c_double]),
         "couplb_lbm_step": (ip, [vp, dp, ip]),
         "couplb_lbm_steps_async": (ip, [vp, dp, ip, ip]),
+        "couplb_lbm_fuses_steps": (ip, [vp]),
         "couplb_ibm_substep": (ip, [vp, ip, dp, dp, dp, dp, C.POINTER(IbmArgs)]),
         "couplb_ibm_substep_device": (ip, [vp, ip, vp, vp, vp, vp, C.POINTER(IbmArgs)]),
         "couplb_refresh_macro": (ip, [vp]),
@@ -230,6 +231,10 @@ class Device:
     def steps_async(self, n, g=(0.0, 0.0, 0.0), flags=0):
         ga = np.asarray(g, dtype=np.float64)
         self._chk(lib().couplb_lbm_steps_async(self.h, _p(ga), flags, n), "lbm_steps_async")
+
+    def fuses_steps(self):
+        """True when steps_async runs the steps after the first in one launch."""
+        return bool(lib().couplb_lbm_fuses_steps(self.h))
 
     def sync(self):
         self._chk(lib().couplb_synchronize(self.h), "synchronize")

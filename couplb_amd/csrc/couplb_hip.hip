@@ -67,6 +67,8 @@ struct couplb_ctx {
   double base = 0.0;
   float scale = 1.0f;
   int cycle = 0;         // state counter for the u stamps
+  size_t fused_nodes = 0;  // grids up to this many nodes: k_steps_fused
+  size_t fused_lds = 0;    // its LDS bytes; 0 when steps do not fuse
   hipStream_t stream = nullptr;
   TimerRing tr;
   std::string err;
@@ -1040,11 +1042,11 @@ int tile_spread(couplb_ctx* c, double* F4, unsigned char* flag, int fold) {
   return COUPLB_OK;
 }
 
-int step_impl(couplb_ctx* c, const double g[3], int flags, bool timed) {
+// Arguments of the next LB step of the current state.
+StepArgs step_args(couplb_ctx* c, const double g[3], int flags) {
   StepArgs a;
   a.pc = make_pull(c);
-  const int dst = c->cur ^ 1;
-  a.dst = c->f[dst];
+  a.dst = c->f[c->cur ^ 1];
   a.omega = 1.0 / c->p.tau;
   const double ZERO_TOL = 1e-15;  // fix_couplb.cpp:693-695
   const bool grav = g && !(std::fabs(g[0]) < ZERO_TOL && std::fabs(g[1]) < ZERO_TOL &&
@@ -1057,6 +1059,55 @@ int step_impl(couplb_ctx* c, const double g[3], int flags, bool timed) {
   a.has_ibm = c->nslots > 0;
   a.flag = c->flag;
   a.F4 = c->F4;
+  return a;
+}
+
+template <int D, typename S> struct LaunchFused {
+  static void run(couplb_ctx* c, const StepArgs& a, int n) {
+    const unsigned t = (unsigned)std::min<size_t>(FUSED_THREADS, round_up(c->N, 64));
+    hipLaunchKernelGGL((k_steps_fused<D, S>), dim3(1), dim3(t), c->fused_lds, c->stream, a, n);
+  }
+};
+
+// LDS bytes of k_steps_fused for this grid, 0 when the grid is larger than
+// the node bound or the state does not fit the LDS of one workgroup
+template <int D, typename S> struct FusedSetup {
+  static void run(couplb_ctx* c, size_t* bytes) {
+    *bytes = 0;
+    if (c->N > c->fused_nodes || c->N > FUSED_THREADS * 4) return;
+    const size_t b = fused_lds_bytes<D, S>(c->N);
+    int dev = 0, lim = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) {
+      (void)hipGetLastError();
+      return;
+    }
+    if (b > (size_t)lim) return;
+    if (b > 65536 && hipFuncSetAttribute(reinterpret_cast<const void*>(&k_steps_fused<D, S>),
+                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)b) != hipSuccess) {
+      (void)hipGetLastError();
+      return;
+    }
+    *bytes = b;
+  }
+};
+
+// n steps after a first one (no pending IBM force, state in pull form) as
+// one k_steps_fused launch; the context ends where n step_impl calls leave it.
+int fused_steps(couplb_ctx* c, const double g[3], int flags, int n) {
+  if (c->nslots > 0 || c->local) return COUPLB_EINVAL;
+  const StepArgs a = step_args(c, g, flags);
+  dispatch<LaunchFused>(c, a, n);
+  HIPCHK(c, hipGetLastError());
+  c->cur ^= (n & 1);
+  c->rho_fresh = 0;
+  c->cycle += n;
+  return COUPLB_OK;
+}
+
+int step_impl(couplb_ctx* c, const double g[3], int flags, bool timed) {
+  StepArgs a = step_args(c, g, flags);
+  const int dst = c->cur ^ 1;
   const int S = (c->D == 3) ? stencil_size<3>() : stencil_size<2>();
   // markers at the positions of the previous LB interval: constant map
   const bool fixed = c->nslots == 1 && c->slots[0].epoch == c->prev_step_epoch &&
@@ -1174,6 +1225,10 @@ int couplb_create(const couplb_params* p, couplb_ctx** out) {
       c->falloc = c->lay.pq * (size_t)c->Q;
     }
   }
+  // several steps of a grid this small run as one launch (k_steps_fused);
+  // COUPLB_FUSED_NODES overrides the bound (0 switches the fusion off)
+  c->fused_nodes = 1024;
+  if (const char* v = std::getenv("COUPLB_FUSED_NODES")) c->fused_nodes = (size_t)std::strtoull(v, nullptr, 10);
   c->elem = storage_bytes(p->storage);
   c->scale = (p->storage == COUPLB_FP16) ? 1024.0f : 1.0f;
   c->base = (p->storage == COUPLB_FP64) ? 0.0 : p->rho0;
@@ -1184,6 +1239,7 @@ int couplb_create(const couplb_params* p, couplb_ctx** out) {
   for (int e = 0; e < TimerRing::R; e++)
     if (hipEventCreate(&c->tr.a[e]) != hipSuccess || hipEventCreate(&c->tr.b[e]) != hipSuccess)
       return bad("hipEventCreate failed");
+  dispatch<FusedSetup>(c, &c->fused_lds);
   const size_t fbytes = c->elem * c->falloc;
   for (int b = 0; b < 2; b++) {
     if (hipMalloc(&c->f[b], fbytes) != hipSuccess) {
@@ -1338,11 +1394,16 @@ int couplb_lbm_step(couplb_ctx* c, const double g[3], int flags) {
 int couplb_lbm_steps_async(couplb_ctx* c, const double g[3], int flags, int nsteps) {
   if (!c) return COUPLB_EINVAL;
   for (int s = 0; s < nsteps; s++) {
-    int rc = step_impl(c, g, flags, false);
+    // after the first step (which takes any spread IBM force, the import
+    // form and a refreshed density), the rest of a small grid's run fuses
+    if (s == 1 && nsteps > 2 && c->fused_lds > 0) return fused_steps(c, g, flags, nsteps - 1);
+    const int rc = step_impl(c, g, flags, false);
     if (rc) return rc;
   }
   return COUPLB_OK;
 }
+
+int couplb_lbm_fuses_steps(const couplb_ctx* c) { return (c && c->fused_lds > 0) ? 1 : 0; }
 
 int couplb_ibm_substep_device(couplb_ctx* c, int n, const double* x, const double* v,
                               const double* m, double* f, const couplb_ibm_args* a) {

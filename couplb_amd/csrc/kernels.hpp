@@ -130,20 +130,23 @@ struct PullCtx {
 
 // fluid(q, si, sj, sk) returns the stored value of plane q at fluid source
 // node (si, sj, sk); the other rules read the node itself from global memory
+// lay, src, rho_coll: the state the wall rules read and the density of the
+// collision that made it (pc.lay, pc.src, pc.rho_coll in global memory for
+// the single-step kernels; LDS copies in k_steps_fused)
 template <int D, typename S, typename Fluid>
-__device__ inline void pull_node_with(const PullCtx& pc, int i, int j, int k,
+__device__ inline void pull_node_with(const PullCtx& pc, const Layout& lay, const S* src,
+                                      const double* rho_coll, int i, int j, int k,
                                       typename Store<S>::T (&f)[Lat<D>::Q], Fluid fluid) {
   using L = Lat<D>;
   using T = typename Store<S>::T;
   constexpr int Q = L::Q;
-  const S* src = static_cast<const S*>(pc.src);
   const int nx = pc.g.nx, ny = pc.g.ny, nz = pc.g.nz;
   const unsigned row = (unsigned)j + (unsigned)ny * (unsigned)k;
   const unsigned node = (unsigned)i + (unsigned)nx * row;
   if (pc.local) {
 #pragma unroll
     for (int q = 0; q < Q; q++) {
-      f[q] = Store<S>::load(src + pc.lay.at(q, row, i));
+      f[q] = Store<S>::load(src + lay.at(q, row, i));
       if (Store<S>::deviation) f[q] *= (T)pc.inv_scale;
     }
     return;
@@ -184,7 +187,7 @@ __device__ inline void pull_node_with(const PullCtx& pc, int i, int j, int k,
   }
   T rw = 0;
   if (any_moving) {
-    const double r = pc.rho_coll[node];
+    const double r = rho_coll[node];
     rw = (T)((r < pc.rho_floor) ? pc.rho_floor : r);
   }
 #pragma unroll
@@ -209,7 +212,7 @@ __device__ inline void pull_node_with(const PullCtx& pc, int i, int j, int k,
       int qs = q;
       if (wt == 1 || wt == 2) qs = L::opp(q);
       else if (wt == 3) qs = (wd == 0) ? L::reflect(0, q) : (wd == 1 ? L::reflect(1, q) : L::reflect(2, q));
-      f[q] = Store<S>::load(src + pc.lay.at(qs, row, i));
+      f[q] = Store<S>::load(src + lay.at(qs, row, i));
       if (wt == 2) {
         const double* uw = pc.w.vel[wd][wsd];
         if (Store<S>::deviation) {
@@ -229,6 +232,12 @@ __device__ inline void pull_node_with(const PullCtx& pc, int i, int j, int k,
     }
     if (Store<S>::deviation) f[q] *= (T)pc.inv_scale;
   }
+}
+
+template <int D, typename S, typename Fluid>
+__device__ inline void pull_node_with(const PullCtx& pc, int i, int j, int k,
+                                      typename Store<S>::T (&f)[Lat<D>::Q], Fluid fluid) {
+  pull_node_with<D, S>(pc, pc.lay, static_cast<const S*>(pc.src), pc.rho_coll, i, j, k, f, fluid);
 }
 
 template <int D, typename S>
@@ -337,8 +346,13 @@ struct StepArgs {
 // with the Guo half-force, density bookkeeping, collision.  out(q, v) stores
 // the post-collision value of plane q (already scaled for the reduced
 // formats).
+// lay, dst: the buffer collide_exact (FP64) writes (the reduced formats
+// store through out); rho_in, rho_out: the density array read for gravity
+// and the one written (a.pc.lay, a.dst, a.pc.rho_coll, a.rho_out in the
+// single-step kernels)
 template <int D, typename S, typename Out>
-__device__ inline void step_node(const StepArgs& a, int i, int j, int k, unsigned node,
+__device__ inline void step_node(const StepArgs& a, const Layout& lay, S* dst, const double* rho_in,
+                                 double* rho_out, int i, int j, int k, unsigned node,
                                  typename Store<S>::T (&f)[Lat<D>::Q], Out out) {
   using L = Lat<D>;
   using T = typename Store<S>::T;
@@ -369,22 +383,22 @@ __device__ inline void step_node(const StepArgs& a, int i, int j, int k, unsigne
   }
   // gravity: F += g * rho_array (fix_couplb.cpp:706)
   if (a.grav_mode) {
-    const T rg = (a.grav_mode == 1) ? rho : (T)a.pc.rho_coll[node];
+    const T rg = (a.grav_mode == 1) ? rho : (T)rho_in[node];
     Fx = add_prod(Fx, (T)a.g[0], rg);
     Fy = add_prod(Fy, (T)a.g[1], rg);
     Fz = add_prod(Fz, (T)a.g[2], rg);
   }
-  if (a.write_rho) a.rho_out[node] = (double)rho;
+  if (a.write_rho) rho_out[node] = (double)rho;
   else if (a.has_moving) {
     const bool wall_adj =
         (i == 0 && a.pc.w.type[0][0] == 2) || (i == g.nx - 1 && a.pc.w.type[0][1] == 2) ||
         (j == 0 && a.pc.w.type[1][0] == 2) || (j == g.ny - 1 && a.pc.w.type[1][1] == 2) ||
         (D == 3 && ((k == 0 && a.pc.w.type[2][0] == 2) || (k == g.nz - 1 && a.pc.w.type[2][1] == 2)));
-    if (wall_adj) a.rho_out[node] = (double)rho;
+    if (wall_adj) rho_out[node] = (double)rho;
   }
   if constexpr (!Store<S>::deviation) {
-    collide_exact<D>(f, rho, jx, jy, jz, Fx, Fy, Fz, a.omega, static_cast<S*>(a.dst), a.pc.lay.pq,
-                     a.pc.lay.at(0, (unsigned)j + (unsigned)g.ny * (unsigned)k, (unsigned)i));
+    collide_exact<D>(f, rho, jx, jy, jz, Fx, Fy, Fz, a.omega, dst, lay.pq,
+                     lay.at(0, (unsigned)j + (unsigned)g.ny * (unsigned)k, (unsigned)i));
     (void)out;
   } else {
     const T inv_r = (T)1 / rho;
@@ -427,6 +441,12 @@ __device__ inline void step_node(const StepArgs& a, int i, int j, int k, unsigne
   }
 }
 
+template <int D, typename S, typename Out>
+__device__ inline void step_node(const StepArgs& a, int i, int j, int k, unsigned node,
+                                 typename Store<S>::T (&f)[Lat<D>::Q], Out out) {
+  step_node<D, S>(a, a.pc.lay, static_cast<S*>(a.dst), a.pc.rho_coll, a.rho_out, i, j, k, node, f, out);
+}
+
 template <int D, typename S>
 __global__ void __launch_bounds__(256) k_step(StepArgs a) {
   using L = Lat<D>;
@@ -443,6 +463,73 @@ __global__ void __launch_bounds__(256) k_step(StepArgs a) {
   const size_t at0 = a.pc.lay.at(0, (unsigned)j + (unsigned)g.ny * (unsigned)k, (unsigned)i);
   step_node<D, S>(a, i, j, k, node, f,
                   [&](int q, T v) { Store<S>::store(dst + q * a.pc.lay.pq + at0, v); });
+}
+
+// nsteps LB steps of a small grid in one launch of one workgroup, with the
+// state and the density array held in LDS: a step of a grid this small
+// takes less time than a kernel launch, so a loop over launches would be
+// bound by launch latency.  Two LDS state buffers in the row layout without
+// padding (pq = nx, pr = Q*nx) and the density array, filled from a.pc.src
+// and a.pc.rho_coll, then one barrier per step, and the last state and the
+// density written back to the buffer the same steps launched one by one
+// would have ended in (a.pc.src for an even count, a.dst for an odd one)
+// and to a.rho_out (= a.pc.rho_coll).  The steps are those after the first
+// of a run (state in pull form; gravity, if any, from the density the step
+// before wrote), the same for every step but the buffers; the per-node
+// arithmetic is k_step's, so the bits are k_step's.
+constexpr int FUSED_THREADS = 1024;
+template <int D, typename S>
+__host__ __device__ constexpr size_t fused_state_elems(size_t nodes) { return nodes * (size_t)Lat<D>::Q; }
+template <int D, typename S>
+__host__ __device__ constexpr size_t fused_lds_bytes(size_t nodes) {
+  return (2 * fused_state_elems<D, S>(nodes) * sizeof(S) + 7) / 8 * 8 + nodes * sizeof(double);
+}
+
+template <int D, typename S>
+__global__ void __launch_bounds__(FUSED_THREADS) k_steps_fused(StepArgs a, int nsteps) {
+  using L = Lat<D>;
+  using T = typename Store<S>::T;
+  constexpr int Q = L::Q;
+  extern __shared__ __attribute__((aligned(16))) unsigned char fused_lds[];
+  const unsigned nx = (unsigned)a.pc.g.nx, ny = (unsigned)a.pc.g.ny;
+  const unsigned N = nx * ny * (unsigned)(D == 3 ? a.pc.g.nz : 1);
+  const Layout ll{nx, (size_t)Q * nx};
+  const size_t ne = fused_state_elems<D, S>(N);
+  S* A = reinterpret_cast<S*>(fused_lds);
+  S* B = A + ne;
+  double* R = reinterpret_cast<double*>(fused_lds + (2 * ne * sizeof(S) + 7) / 8 * 8);
+  const S* gsrc = static_cast<const S*>(a.pc.src);
+  for (unsigned node = threadIdx.x; node < N; node += blockDim.x) {
+    const unsigned row = node / nx, i = node - row * nx;
+#pragma unroll
+    for (int q = 0; q < Q; q++) A[ll.at(q, row, i)] = gsrc[a.pc.lay.at(q, row, i)];
+    R[node] = a.pc.rho_coll[node];
+  }
+  for (int s = 0; s < nsteps; s++) {
+    __syncthreads();
+    const S* src = (s & 1) ? B : A;
+    S* dst = (s & 1) ? A : B;
+    for (unsigned node = threadIdx.x; node < N; node += blockDim.x) {
+      const unsigned row = node / nx, i = node - row * nx;
+      const unsigned j = row % ny, k = row / ny;
+      T f[Q];
+      pull_node_with<D, S>(a.pc, ll, src, R, (int)i, (int)j, (int)k, f, [&](int q, int si, int sj, int sk) {
+        return Store<S>::load(src + ll.at(q, (unsigned)sj + ny * (unsigned)sk, (unsigned)si));
+      });
+      const size_t at0 = ll.at(0, row, i);
+      step_node<D, S>(a, ll, dst, R, R, (int)i, (int)j, (int)k, node, f,
+                      [&](int q, T v) { Store<S>::store(dst + q * ll.pq + at0, v); });
+    }
+  }
+  __syncthreads();
+  const S* fin = (nsteps & 1) ? B : A;
+  S* gdst = static_cast<S*>((nsteps & 1) ? a.dst : const_cast<void*>(a.pc.src));
+  for (unsigned node = threadIdx.x; node < N; node += blockDim.x) {
+    const unsigned row = node / nx, i = node - row * nx;
+#pragma unroll
+    for (int q = 0; q < Q; q++) gdst[a.pc.lay.at(q, row, i)] = fin[ll.at(q, row, i)];
+    a.rho_out[node] = R[node];
+  }
 }
 
 // Reduced formats: the same step with the 19 source rows of a 256-node row

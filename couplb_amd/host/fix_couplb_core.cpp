@@ -21,6 +21,8 @@ namespace {
 constexpr double ZERO_TOL = 1e-15;  // couplb_lattice.h:16
 constexpr double MA_WARN = 0.3;     // couplb_lattice.h:18
 constexpr double MA_ERROR = 0.5;    // couplb_lattice.h:19
+// held-back LB steps issued as one run at the latest after this many
+constexpr int MAX_HELD_STEPS = 512;
 constexpr uint32_t CLBK_MAGIC = 0x434C424Bu;  // couplb_io.h:19-23
 constexpr int32_t CLBK_VERSION = 1;
 
@@ -355,6 +357,8 @@ void FixCoupLBCore::init() {
     if (ctx) { couplb_destroy(ctx); ctx = nullptr; }
     throw fix_error("fix couplb: device context: " + msg);
   }
+  defer_steps = couplb_lbm_fuses_steps(ctx) != 0;
+  pend_n = 0;
   prof_r.assign(Ny, 0.0); prof_u.assign(Ny, 0.0); prof_v.assign(Ny, 0.0);
 
   if (do_restart) {
@@ -401,7 +405,32 @@ void FixCoupLBCore::lbm_step(int ng) {
   // With group atoms present the IBM sub-step that follows recomputes the
   // density (fix_couplb.cpp:757-760), so the step need not keep it.
   const int flags = (ng > 0) ? COUPLB_STEP_RHO_REFRESHED_NEXT : 0;
+  if (ng == 0 && defer_steps) {
+    // the step changes nothing the host reads until the next output, IBM
+    // sub-step or context access, each of which issues the held-back run
+    if (pend_n > 0 && (std::memcmp(g, pend_g, sizeof(g)) != 0 || flags != pend_flags))
+      check(flush_steps(), "LB steps");
+    if (pend_n == 0) {
+      std::memcpy(pend_g, g, sizeof(g));
+      pend_flags = flags;
+    }
+    if (++pend_n >= MAX_HELD_STEPS) check(flush_steps(), "LB steps");
+    return;
+  }
+  check(flush_steps(), "LB steps");
   check(couplb_lbm_step(ctx, g, flags), "LB step");
+}
+
+int FixCoupLBCore::flush_steps() {
+  if (pend_n == 0) return COUPLB_OK;
+  const int n = pend_n;
+  pend_n = 0;
+  return couplb_lbm_steps_async(ctx, pend_g, pend_flags, n);
+}
+
+couplb_ctx* FixCoupLBCore::context() {
+  if (ctx && flush_steps() != COUPLB_OK) say("CoupLB WARNING: %s\n", couplb_last_error(ctx));
+  return ctx;
 }
 
 // FixCoupLB::post_force, fix_couplb.cpp:640-651
@@ -425,6 +454,7 @@ void FixCoupLBCore::ibm_sub_coupling(const couplb_atoms& a, bool first) {
     ibm_has_particles = false;
     return;
   }
+  check(flush_steps(), "LB steps");
   ibm_has_particles = true;
   px.resize(3 * (size_t)ng); pv.resize(3 * (size_t)ng); pm.resize(ng); pf.assign(3 * (size_t)ng, 0.0);
   pidx.resize(ng);
@@ -460,6 +490,7 @@ void FixCoupLBCore::ibm_sub_coupling(const couplb_atoms& a, bool first) {
 
 // FixCoupLB::setup, fix_couplb.cpp:551-638
 void FixCoupLBCore::setup(const couplb_atoms& a) {
+  check(flush_steps(), "LB steps");
   atoms_now = a;
   have_atoms = true;
   md_sub_count = 0;
@@ -577,6 +608,7 @@ int FixCoupLBCore::end_of_step(long step) {
   if (vtk_every > 0 && step >= next_vtk) { vtk = true; next_vtk = step + vtk_every; }
   if (checkpoint_every > 0 && step >= next_checkpoint) { ckpt = true; next_checkpoint = step + checkpoint_every; }
   if (!chk && !out && !vtk && !ckpt) return 0;
+  check(flush_steps(), "LB steps");
   if (chk || out || vtk) check(couplb_refresh_macro(ctx), "refresh");
   if (chk) check_stability_precomputed(step);
   if (out) write_profile(step);

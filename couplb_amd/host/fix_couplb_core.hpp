@@ -36,7 +36,8 @@ class FixCoupLBCore {
   double scalar(const std::string& name) const;
   const double* last_diag() const { return diag; }
   bool last_profile(double* r, double* u, double* v) const;
-  couplb_ctx* context() { return ctx; }
+  // the device context with every held-back LB step issued
+  couplb_ctx* context();
   const std::string& log() const { return screen; }
 
  private:
@@ -78,6 +79,11 @@ class FixCoupLBCore {
   long lbm_step_count = 0;
   bool ibm_has_particles = false;
   couplb_ctx* ctx = nullptr;
+  // LB steps of a small grid with no atoms in the group are held back and
+  // issued as one run (couplb_lbm_steps_async fuses them into one launch)
+  bool defer_steps = false;
+  int pend_n = 0, pend_flags = 0;
+  double pend_g[3] = {0, 0, 0};
   couplb_var_eval var_fn = nullptr;
   void* var_user = nullptr;
 
@@ -98,6 +104,7 @@ class FixCoupLBCore {
   void check(int rc, const char* what);
   void update_gravity_variables();
   void lbm_step(int ng);
+  int flush_steps();
   int count_group(const couplb_atoms& a) const;
   void ibm_sub_coupling(const couplb_atoms& a, bool first);
   void check_stability_precomputed(long step);

@@ -89,7 +89,8 @@ double couplb_fix_scalar(const couplb_fix* fx, const char* name);
 int couplb_fix_last_diag(const couplb_fix* fx, double out[6]);
 /* last y-profile in physical units: Ny rows each of rho, ux, uy */
 int couplb_fix_last_profile(const couplb_fix* fx, double* rho, double* ux, double* uy);
-/* device context, for state import/export */
+/* device context, for state import/export; issues the LB steps the fix
+ * still holds back first (see couplb_fix_end_of_step) */
 void* couplb_fix_context(couplb_fix* fx);
 /* captured screen output (the reference prints to LAMMPS' screen) */
 const char* couplb_fix_log(const couplb_fix* fx);

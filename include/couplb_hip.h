@@ -100,7 +100,14 @@ int couplb_init_equilibrium(couplb_ctx* ctx, double rho, double ux, double uy, d
  * (get_state_f, get_macro, diagnostics, profile_y, the host-array IBM
  * sub-step) wait for it. */
 int couplb_lbm_step(couplb_ctx* ctx, const double g_lb[3], int flags);
+/* nsteps steps with the same gravity and flags.  On a grid of at most
+ * 1024 nodes (COUPLB_FUSED_NODES overrides the bound) whose two states fit
+ * the LDS of one workgroup, every step after the first runs inside one
+ * launch of one workgroup, with the same results. */
 int couplb_lbm_steps_async(couplb_ctx* ctx, const double g_lb[3], int flags, int nsteps);
+/* 1 when couplb_lbm_steps_async runs the steps of this grid in one launch.
+ * No counterpart in the reference (its steps are host loops). */
+int couplb_lbm_fuses_steps(const couplb_ctx* ctx);
 
 /* IBM sub-step: replaces the loop body of do_ibm_sub_coupling
  * (fix_couplb.cpp:757-797): fluid velocity at the marker (IBM::interpolate,

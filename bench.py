@@ -564,6 +564,10 @@ def main():
     dev.sync()
     mlups_moving = nodes * mv_steps / (time.perf_counter() - t2) / 1e6
 
+    # achievable-bandwidth reference: a plain copy of one population buffer's
+    # worth (k_step reads one state and writes the other)
+    copy = dev.copy_bandwidth(nodes * Q * s_bytes, 10)
+
     cpu = None
     if not args.no_cpu_baseline:
         log("bench: cpu baseline")
@@ -592,7 +596,8 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": pmc_traffic(args.storage, N),
                      "kernel": f"k_step<3,{args.storage}> (fused pull + IBM force gather)",
-                     "bytes_per_lup": b_lup, "avg_kernel_ms": round(lb_avg, 4)},
+                     "bytes_per_lup": b_lup, "avg_kernel_ms": round(lb_avg, 4),
+                     "device_copy_gbs": round(copy, 1), "frac_of_device_copy": round(achieved / copy, 4)},
         "ibm": {"markers_per_s": round(markers_per_s, 1), "avg_substep_ms": round(ibm_avg, 4),
                 "avg_spread_ms_per_lb_step": round(spread_avg, 4), "markers": n,
                 "gather_ms_per_lb_step": round(gather_avg, 4),

@@ -111,6 +111,7 @@ def lib() -> C.CDLL:
         "couplb_lbm_step": (ip, [vp, dp, ip]),
         "couplb_lbm_steps_async": (ip, [vp, dp, ip, ip]),
         "couplb_lbm_fuses_steps": (ip, [vp]),
+        "couplb_copy_bandwidth": (ip, [vp, C.c_size_t, ip, dp]),
         "couplb_ibm_substep": (ip, [vp, ip, dp, dp, dp, dp, C.POINTER(IbmArgs)]),
         "couplb_ibm_substep_device": (ip, [vp, ip, vp, vp, vp, vp, C.POINTER(IbmArgs)]),
         "couplb_refresh_macro": (ip, [vp]),
@@ -238,6 +239,12 @@ class Device:
 
     def sync(self):
         self._chk(lib().couplb_synchronize(self.h), "synchronize")
+
+    def copy_bandwidth(self, nbytes=2 << 30, reps=10):
+        """Read + write GB/s of a plain device copy of nbytes (measurement helper)."""
+        out = np.zeros(1)
+        self._chk(lib().couplb_copy_bandwidth(self.h, nbytes, reps, _p(out)), "copy_bandwidth")
+        return float(out[0])
 
     def ibm(self, x, v, m, f, lo, inv_dx, xi_over_dtLB, vel_scale, inv_N, force_scale,
             positions_unchanged=False):

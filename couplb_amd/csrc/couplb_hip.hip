@@ -1062,6 +1062,22 @@ StepArgs step_args(couplb_ctx* c, const double g[3], int flags) {
   return a;
 }
 
+// Plain copy in 16-byte groups, four groups in flight per lane, one pass
+// over the grid: the achievable-bandwidth reference for the kernels'
+// roofline fractions.
+typedef unsigned int CopyV __attribute__((ext_vector_type(4)));
+constexpr int COPY_UNROLL = 4;
+__global__ void __launch_bounds__(256) k_copy16(const CopyV* __restrict__ src, CopyV* __restrict__ dst, size_t n) {
+  const size_t base = (size_t)blockIdx.x * (256 * COPY_UNROLL) + threadIdx.x;
+  CopyV v[COPY_UNROLL];
+#pragma unroll
+  for (int u = 0; u < COPY_UNROLL; u++)
+    if (base + (size_t)u * 256 < n) v[u] = __builtin_nontemporal_load(src + base + (size_t)u * 256);
+#pragma unroll
+  for (int u = 0; u < COPY_UNROLL; u++)
+    if (base + (size_t)u * 256 < n) __builtin_nontemporal_store(v[u], dst + base + (size_t)u * 256);
+}
+
 template <int D, typename S> struct LaunchFused {
   static void run(couplb_ctx* c, const StepArgs& a, int n) {
     const unsigned t = (unsigned)std::min<size_t>(FUSED_THREADS, round_up(c->N, 64));
@@ -1514,6 +1530,41 @@ int couplb_synchronize(couplb_ctx* c) {
   if (!c) return COUPLB_EINVAL;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return COUPLB_OK;
+}
+
+int couplb_copy_bandwidth(couplb_ctx* c, size_t bytes, int reps, double* gbs) {
+  if (!c || !gbs || reps < 1 || bytes < 16) return COUPLB_EINVAL;
+  const size_t n = bytes / 16;
+  void *a = nullptr, *b = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = COUPLB_OK;
+  if (hipMalloc(&a, n * 16) != hipSuccess || hipMalloc(&b, n * 16) != hipSuccess) {
+    c->err = "hipMalloc of the copy buffers failed";
+    rc = COUPLB_ENOMEM;
+  } else if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    c->err = "hipEventCreate failed";
+    rc = COUPLB_EDEVICE;
+  } else {
+    const unsigned grid = (unsigned)((n + 256 * COPY_UNROLL - 1) / (256 * COPY_UNROLL));
+    hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(256), 0, c->stream, (const CopyV*)a, (CopyV*)b, n);
+    hipEventRecord(e0, c->stream);
+    for (int r = 0; r < reps; r++)
+      hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(256), 0, c->stream, (const CopyV*)a, (CopyV*)b, n);
+    hipEventRecord(e1, c->stream);
+    float ms = 0.0f;
+    if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&ms, e0, e1) != hipSuccess || !(ms > 0.0f)) {
+      c->err = "copy kernel failed";
+      rc = COUPLB_EDEVICE;
+    } else {
+      *gbs = 2.0 * (double)(n * 16) * reps / (ms * 1e-3) / 1e9;
+    }
+  }
+  if (e0) hipEventDestroy(e0);
+  if (e1) hipEventDestroy(e1);
+  if (a) hipFree(a);
+  if (b) hipFree(b);
+  return rc;
 }
 
 void* couplb_stream(couplb_ctx* c) { return c ? (void*)c->stream : nullptr; }

@@ -24,7 +24,7 @@ extern "C" {
 #define COUPLB_ENOMEM -3   /* device allocation failed */
 #define COUPLB_ESTATE -4   /* call not valid in the current state */
 
-/* population storage (north-star precision variants) */
+/* population storage (precision variants) */
 #define COUPLB_FP64 0 /* f_i, the reference's representation */
 #define COUPLB_FP32 1 /* f_i - w_i*rho0 in float, float arithmetic */
 #define COUPLB_FP16 2 /* f_i - w_i*rho0 in IEEE half, float arithmetic */
@@ -223,6 +223,11 @@ int couplb_diagnostics(couplb_ctx* ctx, double out[6]);
 int couplb_profile_y(couplb_ctx* ctx, double* rho_sum, double* ux_sum, double* uy_sum);
 
 int couplb_synchronize(couplb_ctx* ctx);
+/* Measurement helper, no counterpart in the reference: read + write GB/s of
+ * a plain 16-byte-per-lane copy of `bytes` between two fresh device buffers
+ * (allocated and freed here), timed over `reps` launches on the context
+ * stream after one untimed launch. */
+int couplb_copy_bandwidth(couplb_ctx* ctx, size_t bytes, int reps, double* gbs);
 /* the context's hipStream_t, for callers that time with HIP events */
 void* couplb_stream(couplb_ctx* ctx);
 

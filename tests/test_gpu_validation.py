@@ -13,7 +13,9 @@ post_force of fix addforce and of the fix, final_integrate, end_of_step):
   integrator);
 - tests/drag_forces_sphere/in.drag_sphere:46-50: terminal velocity of the
   162-marker sphere (fix rigid/nve single: translation only here) against
-  Hasimoto's drag for a periodic array of spheres.
+  Hasimoto's drag for a periodic array of spheres;
+- tests/poiseuille3d/in.poiseuille3d with validate.py's rule on the written
+  profile file.
 """
 import os
 
@@ -235,3 +237,78 @@ def test_sphere_drag_hasimoto():
     assert d["clamps"] == 0
     assert abs(vx[-1] - vx[-6]) <= 0.01 * abs(vx[-1])   # terminal
     assert abs(err) < 0.10
+
+
+def read_profile(path):
+    """validate.py:29-72 restated: header parameters and the last block."""
+    import re
+    params, steps, cur = {}, {}, None
+    with open(path) as fh:
+        lines = fh.read().splitlines()
+    for s in (ln.strip() for ln in lines):
+        if not s.startswith("#"):
+            break
+        m = re.search(r"H=Ny=(\d+)", s)
+        if m:
+            params["Ny"] = int(m.group(1))
+        for key in ("tau", "nu", "nu_lb", "rho"):
+            m = re.search(key + r"=([\d.eE+\-]+)", s)
+            if m:
+                params[key] = float(m.group(1))
+    for s in (ln.strip() for ln in lines):
+        if not s:
+            continue
+        if s.startswith("# step ="):
+            cur = int(s.split("=")[1])
+            steps[cur] = []
+            continue
+        if s.startswith("#") or cur is None:
+            continue
+        parts = s.split()
+        if len(parts) >= 6:
+            steps[cur].append([float(x) for x in parts])
+    last = max(steps)
+    return params, last, np.array(steps[last])
+
+
+def test_poiseuille3d_deck(tmp_path):
+    # tests/poiseuille3d/in.poiseuille3d: D3Q19 8x32x8, walls in y, g_x = 5e-6,
+    # 50000 steps, profile every 5000; validate.py's rule: max |u - u_exact| <
+    # 1 % of u_max.  The first 2000 steps also match the oracle bit for bit.
+    out = tmp_path / "couplb_poiseuille3d.dat"
+    cmd = (f"flow noatoms couplb 8 32 8 0.1 1.0 wall_y 1 1 gravity 5e-6 0.0 0.0 "
+           f"output 5000 {out} check_every 2000")
+    box = dict(boxlo=(0.0, 0.0, 0.0), boxhi=(8.0, 32.0, 8.0), periodicity=(1, 0, 1))
+    print("case in.poiseuille3d: D3Q19 8x32x8, 50000 steps", flush=True)
+    fd = couplb_amd.Fix(cmd, dt=1.0, **box)
+    fd.init()
+    fd.setup(None)
+    kw = parse_cmd(cmd)
+    kw.pop("output_every", None)
+    fo = oracle.Fix(dim=3, dt=1.0, boxlo=box["boxlo"], boxhi=box["boxhi"], **kw)
+    fo.setup(None)
+    for step in range(1, 50001):
+        fd.step = step
+        fd.post_force(None)
+        fd.end_of_step(step)
+        if step <= 2000:
+            fo.step = step
+            fo.post_force(None)
+            fo.end_of_step(step)
+        if step == 2000:
+            a = fd.device_state().reshape(19, 10, 34, 10)[:, 1:-1, 1:-1, 1:-1]
+            b = fo.f().reshape(19, 10, 34, 10)[:, 1:-1, 1:-1, 1:-1]
+            assert np.array_equal(a, b)
+            md, mo = fd.last_diag()["mass"], fo.last_diag()["mass"]
+            assert abs(md - mo) <= 1e-12 * mo
+            fo.close()
+    fd.close()
+    params, last, data = read_profile(out)
+    assert params["Ny"] == 32 and last == 50000 and len(data) == 32
+    y, ux = data[:, 2], data[:, 4]
+    dx = y[1] - y[0]
+    H = params["Ny"] * dx
+    exact = 5e-6 / (2 * params["nu"]) * (y + 0.5 * dx) * (H - 0.5 * dx - y)
+    rel = np.max(np.abs(ux - exact)) / exact.max()
+    print(f"  step {last}: max |u - u_exact| / u_max = {100 * rel:.4f} %", flush=True)
+    assert rel < 0.01

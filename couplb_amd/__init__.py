@@ -111,6 +111,7 @@ def lib() -> C.CDLL:
         "couplb_lbm_step": (ip, [vp, dp, ip]),
         "couplb_lbm_steps_async": (ip, [vp, dp, ip, ip]),
         "couplb_lbm_fuses_steps": (ip, [vp]),
+        "couplb_set_timer_mask": (ip, [vp, C.c_uint]),
         "couplb_copy_bandwidth": (ip, [vp, C.c_size_t, ip, dp]),
         "couplb_ibm_substep": (ip, [vp, ip, dp, dp, dp, dp, C.POINTER(IbmArgs)]),
         "couplb_ibm_substep_device": (ip, [vp, ip, vp, vp, vp, vp, C.POINTER(IbmArgs)]),
@@ -340,6 +341,10 @@ class Device:
         self._chk(lib().couplb_kernel_ms_total(self.h, which, C.byref(t), C.byref(n)),
                   "kernel_ms_total")
         return t.value, n.value
+
+    def set_timer_mask(self, mask):
+        """Kinds of kernel_ms_total that record events (bit k for kind k)."""
+        self._chk(lib().couplb_set_timer_mask(self.h, mask), "set_timer_mask")
 
     def kernel_ms_reset(self):
         self._chk(lib().couplb_kernel_ms_reset(self.h), "kernel_ms_reset")

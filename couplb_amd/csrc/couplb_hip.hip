@@ -68,6 +68,7 @@ struct couplb_ctx {
   float scale = 1.0f;
   int cycle = 0;         // state counter for the u stamps
   size_t fused_nodes = 0;  // grids up to this many nodes: k_steps_fused
+  unsigned timer_mask = ~0u;  // timed region kinds (bit per kind)
   size_t fused_lds = 0;    // its LDS bytes; 0 when steps do not fuse
   hipStream_t stream = nullptr;
   TimerRing tr;
@@ -180,6 +181,10 @@ int timer_harvest_all(couplb_ctx* c) {
 }
 // opens a timed region of kind `which`; returns its ring entry in *slot
 int timer_begin(couplb_ctx* c, int which, int* slot) {
+  if (!((c->timer_mask >> which) & 1u)) {  // kind not timed: no events
+    *slot = -1;
+    return COUPLB_OK;
+  }
   TimerRing& t = c->tr;
   const int e = t.head;
   int rc = timer_harvest(c, e);
@@ -191,6 +196,7 @@ int timer_begin(couplb_ctx* c, int which, int* slot) {
   return COUPLB_OK;
 }
 int timer_end(couplb_ctx* c, int slot) {
+  if (slot < 0) return COUPLB_OK;
   HIPCHK(c, hipEventRecord(c->tr.b[slot], c->stream));
   c->tr.pending[slot] = true;
   return COUPLB_OK;
@@ -1416,6 +1422,12 @@ int couplb_lbm_steps_async(couplb_ctx* c, const double g[3], int flags, int nste
     const int rc = step_impl(c, g, flags, false);
     if (rc) return rc;
   }
+  return COUPLB_OK;
+}
+
+int couplb_set_timer_mask(couplb_ctx* c, unsigned mask) {
+  if (!c) return COUPLB_EINVAL;
+  c->timer_mask = mask;
   return COUPLB_OK;
 }
 

@@ -108,6 +108,11 @@ int couplb_lbm_steps_async(couplb_ctx* ctx, const double g_lb[3], int flags, int
 /* 1 when couplb_lbm_steps_async runs the steps of this grid in one launch.
  * No counterpart in the reference (its steps are host loops). */
 int couplb_lbm_fuses_steps(const couplb_ctx* ctx);
+/* Timed kinds (bit k for kind k of couplb_kernel_ms_total; all by default).
+ * Each timed region records two HIP events on the stream, which costs a few
+ * microseconds between kernels; untimed kinds record none.  Measurement
+ * helper, no counterpart in the reference. */
+int couplb_set_timer_mask(couplb_ctx* ctx, unsigned mask);
 
 /* IBM sub-step: replaces the loop body of do_ibm_sub_coupling
  * (fix_couplb.cpp:757-797): fluid velocity at the marker (IBM::interpolate,

@@ -1115,15 +1115,21 @@ template <int D, typename S> struct FusedSetup {
 };
 
 // n steps after a first one (no pending IBM force, state in pull form) as
-// one k_steps_fused launch; the context ends where n step_impl calls leave it.
+// k_steps_fused launches of at most FUSED_MAX_RUN steps (a bound on the
+// duration of one launch); the context ends where n step_impl calls leave it.
+constexpr int FUSED_MAX_RUN = 1 << 16;
 int fused_steps(couplb_ctx* c, const double g[3], int flags, int n) {
   if (c->nslots > 0 || c->local) return COUPLB_EINVAL;
-  const StepArgs a = step_args(c, g, flags);
-  dispatch<LaunchFused>(c, a, n);
-  HIPCHK(c, hipGetLastError());
-  c->cur ^= (n & 1);
-  c->rho_fresh = 0;
-  c->cycle += n;
+  while (n > 0) {
+    const int m = std::min(n, FUSED_MAX_RUN);
+    const StepArgs a = step_args(c, g, flags);
+    dispatch<LaunchFused>(c, a, m);
+    HIPCHK(c, hipGetLastError());
+    c->cur ^= (m & 1);
+    c->rho_fresh = 0;
+    c->cycle += m;
+    n -= m;
+  }
   return COUPLB_OK;
 }
 

@@ -113,6 +113,37 @@ def copy_calibration(mb, reps):
                       "copy_gbs": 2 * n * 8 / (ms * 1e-3) / 1e9}), flush=True)
 
 
+def storage_error(N, storage, md, tx, tv, tm, ibm_kw, g, steps=20):
+    """Velocity error of the workload run with `storage` against the FP64
+    device path (tied to the oracle by the parity tests), both from the same
+    state and markers after `steps` LB steps: rel-L2 and rel-Linf of u, and
+    rel-L2 of the disturbance u - u0."""
+    import torch
+    import couplb_amd
+    n = tm.numel()
+    fields = {}
+    for st in (storage, "fp64"):
+        d = couplb_amd.Device(3, N, N, N, tau=0.8, kernel=couplb_amd.PESKIN4, storage=st, md_per_lb=md)
+        d.init_equilibrium(1.0, (0.02, 0.0, 0.0))
+        tf = torch.zeros((n, 3), dtype=torch.float64, device="cuda:0")
+        torch.cuda.synchronize()
+        for s in range(1 + steps * md):
+            if s and (s - 1) % md == 0:
+                d.step(g, couplb_amd.RHO_REFRESHED_NEXT)
+            d.ibm_device(n, tx.data_ptr(), tv.data_ptr(), tm.data_ptr(), tf.data_ptr(),
+                         positions_unchanged=s > 0, **ibm_kw)
+        d.sync()
+        _, ux, uy, uz = d.macro()
+        fields[st] = np.stack([ux, uy, uz])
+        d.close()
+        del tf
+    a, b = fields[storage], fields["fp64"]
+    u0 = np.array([0.02, 0.0, 0.0]).reshape(3, 1, 1, 1)
+    return dict(steps=steps, rel_l2_u=float(np.linalg.norm(a - b) / np.linalg.norm(b)),
+                rel_linf_u=float(np.max(np.abs(a - b)) / np.max(np.abs(b))),
+                rel_l2_disturbance=float(np.linalg.norm(a - b) / np.linalg.norm(b - u0)))
+
+
 def pmc_traffic(storage, n):
     """HBM bytes per k_step launch measured by rocprofv3 --pmc FETCH_SIZE and
     WRITE_SIZE (scaled by a plain copy of known size), read from the
@@ -580,6 +611,12 @@ def main():
     # worth (k_step reads one state and writes the other)
     copy = dev.copy_bandwidth(nodes * Q * s_bytes, 10)
 
+    # reduced storage: the error against the FP64 path on the same workload
+    err = None
+    if args.storage != "fp64":
+        log("bench: error against the FP64 path")
+        err = storage_error(N, args.storage, md, tx, tv, tm, ibm_kw, g)
+
     cpu = None
     if not args.no_cpu_baseline:
         log("bench: cpu baseline")
@@ -621,6 +658,7 @@ def main():
         "lb_only_mlups": round(lb_only, 1),
         "lb_only_hbm_frac": round(lb_only * 1e6 * b_lup / (HBM_PEAK_GBS * 1e9), 4),
         "mass": diag["mass"],
+        "err_vs_fp64": err,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

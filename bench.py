@@ -92,6 +92,20 @@ def cpu_baseline(N, nz_sample, steps, md_per_lb):
                         f"OMP_NUM_THREADS={threads}, {dt:.1f} s"))
 
 
+def host_cpu():
+    """CPU model and logical CPU count of this host."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count()}
+
+
 def copy_calibration(mb, reps):
     """Plain device-to-device copy of known size: the reference point for
     rocprofv3 FETCH_SIZE / WRITE_SIZE and for achievable HBM bandwidth."""
@@ -475,6 +489,9 @@ def main():
     ap.add_argument("--storage", default="fp64", choices=["fp64", "fp32", "fp16", "bf16"])
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--cpu-nz", type=int, default=64)
+    ap.add_argument("--cpu-only", default="", metavar="NZ,STEPS",
+                    help="run only the CPU baseline on an NZ-plane slab for STEPS LB steps "
+                         "and print it (no GPU; used for the one-thread figure)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="time a plain device copy of MB megabytes (PMC byte-counter scale)")
@@ -483,6 +500,10 @@ def main():
                          "c4 precision study, c5 learned MDF warm start")
     ap.add_argument("--grid", type=int, default=0, help="grid edge for c4 (default 384)")
     args = ap.parse_args()
+    if args.cpu_only:
+        nz, steps = (int(t) for t in args.cpu_only.split(","))
+        print(json.dumps(cpu_baseline(args.n, nz, steps, 4)), flush=True)
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
@@ -621,6 +642,16 @@ def main():
     if not args.no_cpu_baseline:
         log("bench: cpu baseline")
         cpu = cpu_baseline(N, args.cpu_nz, args.cpu_steps, md)
+        cpu["host"] = host_cpu()
+        # the same pass structure on one thread, on a thinner slab (its own
+        # process: the OpenMP thread count is fixed when the library loads)
+        import subprocess
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-only", "8,10"], env=env,
+                           capture_output=True, text=True, timeout=600)
+        if r.returncode == 0 and r.stdout.strip():
+            one = json.loads(r.stdout.strip().splitlines()[-1])
+            cpu["one_thread"] = {"value": one["value"], "cores": one["cores"], "sample": one["sample"]}
 
     dtype = {"fp64": "f64", "fp32": "f32 storage/f32 math", "fp16": "f16 storage/f32 math",
              "bf16": "bf16 storage/f32 math"}[args.storage]

@@ -1565,12 +1565,13 @@ int couplb_copy_bandwidth(couplb_ctx* c, size_t bytes, int reps, double* gbs) {
   } else {
     const unsigned grid = (unsigned)((n + 256 * COPY_UNROLL - 1) / (256 * COPY_UNROLL));
     hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(256), 0, c->stream, (const CopyV*)a, (CopyV*)b, n);
-    hipEventRecord(e0, c->stream);
+    const hipError_t r0 = hipEventRecord(e0, c->stream);
     for (int r = 0; r < reps; r++)
       hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(256), 0, c->stream, (const CopyV*)a, (CopyV*)b, n);
-    hipEventRecord(e1, c->stream);
+    const hipError_t r1 = hipEventRecord(e1, c->stream);
     float ms = 0.0f;
-    if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+    if (r0 != hipSuccess || r1 != hipSuccess || hipGetLastError() != hipSuccess ||
+        hipEventSynchronize(e1) != hipSuccess ||
         hipEventElapsedTime(&ms, e0, e1) != hipSuccess || !(ms > 0.0f)) {
       c->err = "copy kernel failed";
       rc = COUPLB_EDEVICE;
@@ -1578,10 +1579,10 @@ int couplb_copy_bandwidth(couplb_ctx* c, size_t bytes, int reps, double* gbs) {
       *gbs = 2.0 * (double)(n * 16) * reps / (ms * 1e-3) / 1e9;
     }
   }
-  if (e0) hipEventDestroy(e0);
-  if (e1) hipEventDestroy(e1);
-  if (a) hipFree(a);
-  if (b) hipFree(b);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
   return rc;
 }
 

@@ -15,7 +15,7 @@ import oracle
 pytestmark = pytest.mark.gpu
 
 # relative L2 velocity error vs the FP64 device path after STEPS steps;
-# bounds measured on MI355X (precision study) with margin
+# bounds: the errors measured on MI355X for this case, with margin
 BOUNDS = {"fp32": 1e-5, "fp16": 2e-2, "bf16": 1e-1}
 N, NZ, STEPS, U0, NU = 64, 4, 400, 0.03, 0.02
 

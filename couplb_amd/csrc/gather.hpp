@@ -100,10 +100,15 @@ __device__ inline void tile_lists(const Geo& g, const Walls& w, int J0, int K0, 
 
 // Walk every candidate of the tile window by window.  load(slot, idx, dst)
 // copies record idx of the slot into LDS entry dst; use(count) consumes the
-// staged window.  All threads of the block call this together.
+// staged window.  All threads of the block call this together.  Candidate
+// q of the tile (rows in slot, z, y order, each row's records in order)
+// lands in window entry q - w0; one thread per candidate loads it, finding
+// its row by binary search over the rows' offsets, so the loads of a window
+// are all in flight together.
 template <int D, typename Slots, typename Cand, int WIN, typename Load, typename Use>
 __device__ inline void stage_tile(const Slots& slots, int nslots, const Geo& g, const TileLists& tl,
                                   Cand* win, int* sh, Load&& load, Use&& use) {
+  __shared__ int row_off[TILE_THREADS], row_r0[TILE_THREADS], row_slot[TILE_THREADS];
   const int per = tl.nY * tl.nZ;
   const int R = nslots * per;
   for (int g0 = 0; g0 < R; g0 += blockDim.x) {
@@ -119,14 +124,30 @@ __device__ inline void stage_tile(const Slots& slots, int nslots, const Geo& g, 
     }
     int total;
     const int off = block_exscan(cnt, sh, total);
+    row_off[threadIdx.x] = off;
+    row_r0[threadIdx.x] = r0;
+    row_slot[threadIdx.x] = s;
+    __syncthreads();
+    const int nrow = min((int)blockDim.x, R - g0);
     for (int w0 = 0; w0 < total; w0 += WIN) {
-      const int lo = off > w0 ? off : w0;
-      const int hi = (off + cnt) < (w0 + WIN) ? (off + cnt) : (w0 + WIN);
-      for (int q = lo; q < hi; q++) load(s, r0 + (q - off), win[q - w0]);
+      const int nw = min(WIN, total - w0);
+      for (int q = (int)threadIdx.x; q < nw; q += blockDim.x) {
+        const int gq = w0 + q;
+        // the last row starting at or before gq holds it (rows without
+        // records share their offset with the next row)
+        int lo = 0, hi = nrow - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (row_off[mid] <= gq) lo = mid;
+          else hi = mid - 1;
+        }
+        load(row_slot[lo], row_r0[lo] + (gq - row_off[lo]), win[q]);
+      }
       __syncthreads();
-      use(total - w0 < WIN ? total - w0 : WIN);
+      use(nw);
       __syncthreads();
     }
+    __syncthreads();  // row_* are rewritten by the next group of rows
   }
 }
 

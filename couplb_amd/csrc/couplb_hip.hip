@@ -301,19 +301,15 @@ template <int D, typename S> struct LaunchExport {
 };
 template <int D, typename S> struct LaunchUgather {
   static void run(couplb_ctx* c, const PullCtx& pc, const BaseSlot& bs, bool cover) {
-    const TileShape ts = tile_shape(D, c->geo.nx);
-#define COVER_LAUNCH(TJ, TK, XM)                                                                  \
-  hipLaunchKernelGGL((k_cover<D, TJ, TK, XM>), tile_grid(c->geo), dim3(TILE_THREADS), 0, c->stream, \
-                     c->geo, c->walls, bs, c->cover)
-    if (!cover) {
-    } else if (ts.XM == 1) {
-      if (D == 3) COVER_LAUNCH(2, 2, 1); else COVER_LAUNCH(4, 1, 1);
-    } else if (ts.XM == 2) {
-      COVER_LAUNCH(2, 1, 2);
-    } else {
-      COVER_LAUNCH(1, 1, 4);
+    if (cover) {
+      const int tb = 256;
+      hipLaunchKernelGGL(k_fill_u8, dim3((unsigned)((c->N + tb - 1) / tb)), dim3(tb), 0, c->stream, c->cover,
+                         c->N, (unsigned char)0);
+      const long ne = (long)c->order_n * stencil_size<D>();
+      if (ne > 0)
+        hipLaunchKernelGGL(k_cover_scatter<D>, dim3((unsigned)((ne + tb - 1) / tb)), dim3(tb), 0, c->stream,
+                           c->order_n, c->geo, c->walls, bs.rec, c->cover);
     }
-#undef COVER_LAUNCH
     const dim3 b = row_block(c->geo.nx);
     // reduced formats: dense pull when the current marker cells cover more
     // than a quarter of the nodes (the covered-node count of the fixed-marker

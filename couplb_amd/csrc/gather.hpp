@@ -194,7 +194,6 @@ __device__ inline void build_bins(int n, int nbin, int* cnt, int* start, int* li
 }
 
 constexpr int SPREAD_WIN = 160;  // MarkerRec per LDS window (22.5 KiB)
-constexpr int COVER_WIN = 512;   // base cells per LDS window (8 KiB)
 
 struct RecSlots {
   const Slot* sl;
@@ -341,10 +340,5 @@ __global__ void __launch_bounds__(256) k_force_export(size_t N, const double* F4
 #pragma unroll
   for (int c = 0; c < 3; c++) out[3 * i + c] = h ? F4[4 * i + c] : 0.0;
 }
-
-struct BaseSlots {
-  const int* rs;
-  __device__ const int* rowstart(int) const { return rs; }
-};
 
 }  // namespace couplb

@@ -118,64 +118,28 @@ struct BaseSlot {
   const Base* rec;
 };
 
-// Which interior nodes some marker stencil (or its ghost-plane image)
-// covers: cover[node] = 1/0 for every node of the tile.
-template <int D, int TJ, int TK, int XM>
-__global__ void __launch_bounds__(TILE_THREADS) k_cover(Geo g, Walls w, BaseSlot bs,
-                                                        unsigned char* cover) {
-  __shared__ Base win[COVER_WIN];
-  __shared__ int cnt[MAX_BIN], start[MAX_BIN + 1], list[COVER_WIN], sh[TILE_THREADS / 64];
-  __shared__ TileLists tl;
-  const TileShape ts{TJ, TK, XM};
-  const int J0 = 1 + blockIdx.y * TJ, K0 = 1 + blockIdx.z * TK;
-  const int nbin = g.nx + 5;
-  tile_lists<D>(g, w, J0, K0, ts, tl);
-  const TileNodes<D, TJ, TK, XM> tn(g, w, J0, K0);
-  bool cov[XM][TJ][TK];
-#pragma unroll
-  for (int m = 0; m < XM; m++)
-#pragma unroll
-    for (int r = 0; r < TJ; r++)
-#pragma unroll
-      for (int q = 0; q < TK; q++) cov[m][r][q] = false;
-  BaseSlots slots{bs.rowstart};
-  stage_tile<D, BaseSlots, Base, COVER_WIN>(
-      slots, 1, g, tl, win, sh, [&](int, int idx, Base& c) { c = bs.rec[idx]; },
-      [&](int n) {
-        build_bins(n, nbin, cnt, start, list, sh,
-                   [&](int c) { return win[c].valid ? win[c].i0 + 2 : -1; });
-#pragma unroll
-        for (int m = 0; m < XM; m++) {
-          if (!tn.xin[m]) continue;
-          each_image(tn.xa[m], [&](int aa) {
-            for (int kb = aa; kb <= aa + 3 && kb < nbin; kb++)
-              for (int e = start[kb]; e < start[kb + 1]; e++) {
-                const Base& c = win[list[e]];
-#pragma unroll
-                for (int r = 0; r < TJ; r++)
-#pragma unroll
-                  for (int q = 0; q < TK; q++)
-                    each_image(tn.ya[r], [&](int bb) {
-                      each_image(tn.za[q], [&](int cc) {
-                        const int u = bb - c.j0 + 1;
-                        const int v = (D == 3) ? cc - c.k0 + 1 : 0;
-                        if (u >= 0 && u <= 3 && v >= 0 && v <= 3) cov[m][r][q] = true;
-                      });
-                    });
-              }
-          });
-        }
-      });
-#pragma unroll
-  for (int m = 0; m < XM; m++)
-#pragma unroll
-    for (int r = 0; r < TJ; r++)
-#pragma unroll
-      for (int q = 0; q < TK; q++) {
-        if (!tn.xin[m] || !tn.yin[r] || !tn.zin[q]) continue;
-        cover[(unsigned)tn.x[m] + (unsigned)g.nx * ((unsigned)tn.j[r] + (unsigned)g.ny * (unsigned)tn.k[q])] =
-            cov[m][r][q] ? 1 : 0;
-      }
+// Which interior nodes some marker stencil covers: one thread per (marker,
+// stencil node) sets cover = 1 at the interior node that a fluid or
+// periodic-image stencil node (node type 0, as IBM::interpolate reads it,
+// couplb_ibm.h:66-86) stands for; the caller has cleared the array.  Every
+// write stores 1, so overlapping stencils leave the same bytes in any order.
+template <int D>
+__global__ void __launch_bounds__(256) k_cover_scatter(int n, Geo g, Walls w, const Base* bases,
+                                                       unsigned char* cover) {
+  constexpr int S = stencil_size<D>();
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)n * S) return;
+  const int p = (int)(e / S), s = (int)(e % S);
+  const Base b = bases[p];
+  if (!b.valid) return;
+  const int a = b.i0 - 1 + (s & 3), bb = b.j0 - 1 + ((s >> 2) & 3);
+  const int c = (D == 3) ? b.k0 - 1 + (s >> 4) : 0;
+  const bool in = a >= 0 && a <= g.nx + 1 && bb >= 0 && bb <= g.ny + 1 && (D == 2 || (c >= 0 && c <= g.nz + 1));
+  int wd, wsd;
+  if (in && node_type(w, g, a, bb, c, wd, wsd) == 0)
+    cover[(unsigned)wrap_ghost(a, g.nx) +
+          (unsigned)g.nx * ((unsigned)wrap_ghost(bb, g.ny) +
+                            (unsigned)g.ny * (unsigned)((D == 3) ? wrap_ghost(c, g.nz) : 0))] = 1;
 }
 
 // Fluid velocity u = j/rho and density (compute_macroscopic(false),

@@ -16,6 +16,8 @@
 namespace couplb {
 
 constexpr int MAX_SLOTS = 16;
+// nodes of one 4-point IBM stencil (4^D)
+template <int D> __host__ __device__ constexpr int stencil_size() { return D == 3 ? 64 : 16; }
 constexpr double RHO_MIN = 1e-10;     // couplb_lattice.h:14
 constexpr double DELTA_TOL = 1e-12;   // couplb_lattice.h:15
 

@@ -16,8 +16,6 @@
 
 namespace couplb {
 
-template <int D> __host__ __device__ constexpr int stencil_size() { return D == 3 ? 64 : 16; }
-
 // one thread per (marker p, stencil slot s); slot s = t + 4u + 16v is
 // ghost-inclusive node (i0-1+t, j0-1+u, k0-1+v) as in IBM::spread
 template <int D>

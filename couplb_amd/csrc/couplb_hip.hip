@@ -303,7 +303,7 @@ template <int D, typename S> struct LaunchUgather {
   static void run(couplb_ctx* c, const PullCtx& pc, const BaseSlot& bs, bool cover) {
     if (cover) {
       const int tb = 256;
-      hipLaunchKernelGGL(k_fill_u8, dim3((unsigned)((c->N + tb - 1) / tb)), dim3(tb), 0, c->stream, c->cover,
+      hipLaunchKernelGGL(k_fill_u8, dim3(fill_u8_blocks(c->N)), dim3(256), 0, c->stream, c->cover,
                          c->N, (unsigned char)0);
       const long ne = (long)c->order_n * stencil_size<D>();
       if (ne > 0)
@@ -575,7 +575,7 @@ int build_pairs(couplb_ctx* c, const SlotHost& sl) {
   size_t bytes = c->ptmp_bytes;
   HIPCHK(c, rocprim::radix_sort_pairs(c->ptmp, bytes, c->pkey, c->pkey_s, c->pval, c->pval_s,
                                       (size_t)ne, 0u, bits, c->stream));
-  hipLaunchKernelGGL(k_fill_u8, dim3((unsigned)((c->N + tb - 1) / tb)), dim3(tb), 0, c->stream, c->flag,
+  hipLaunchKernelGGL(k_fill_u8, dim3(fill_u8_blocks(c->N)), dim3(256), 0, c->stream, c->flag,
                      c->N, (unsigned char)0);
   if (c->D == 3)
     hipLaunchKernelGGL(k_pair_weights<3>, dim3(nb), dim3(tb), 0, c->stream, ne, c->pkey_s, c->pval_s,

@@ -133,9 +133,18 @@ __global__ void __launch_bounds__(256) k_pair_spread(int nh, const int* heads, c
   *reinterpret_cast<double4*>(F4 + 4 * (size_t)k) = double4{ax, ay, az, 0.0};
 }
 
+// p[0..n) = v, 16 bytes per thread (p 16-byte aligned, as device
+// allocations are); launch with fill_u8_blocks(n) blocks of 256 threads
 __global__ void k_fill_u8(unsigned char* p, size_t n, unsigned char v) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = v;
+  typedef unsigned int V4 __attribute__((ext_vector_type(4)));
+  const size_t i = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * 16;
+  if (i + 16 <= n) {
+    const unsigned w = (unsigned)v * 0x01010101u;
+    *reinterpret_cast<V4*>(p + i) = V4{w, w, w, w};
+  } else {
+    for (size_t k = i; k < n; k++) p[k] = v;
+  }
 }
+inline unsigned fill_u8_blocks(size_t n) { return (unsigned)((n + 16 * 256 - 1) / (16 * 256)); }
 
 }  // namespace couplb

@@ -106,6 +106,7 @@ struct couplb_ctx {
   int pair_epoch = -1;       // ordering the pairs (and the node flags) were built for
   int *pheads = nullptr, *phlen = nullptr;  // node runs of the sorted pairs
   unsigned* phnode = nullptr;
+  int2* prun = nullptr;            // per node: (first pair, count) of its run
   unsigned char* phflag = nullptr;
   int* pcount = nullptr;           // device count of runs
   int pnh = 0;                     // number of runs
@@ -262,12 +263,14 @@ template <int D, typename S> struct LaunchStep {
         // reduced formats: 16-byte staged row groups (k_step_lds)
         const dim3 grid((unsigned)((c->geo.nx + XC - 1) / XC), (unsigned)c->geo.ny,
                         (unsigned)(c->D == 3 ? c->geo.nz : 1));
-        hipLaunchKernelGGL((k_step_lds<D, S>), grid, dim3(XC), 0, c->stream, a);
+        if (a.pair_mode) hipLaunchKernelGGL((k_step_lds<D, S, true>), grid, dim3(XC), 0, c->stream, a);
+        else hipLaunchKernelGGL((k_step_lds<D, S>), grid, dim3(XC), 0, c->stream, a);
         return;
       }
     }
     const dim3 b = row_block(c->geo.nx);
-    hipLaunchKernelGGL((k_step<D, S>), row_grid(c->geo, b.x), b, 0, c->stream, a);
+    if (a.pair_mode) hipLaunchKernelGGL((k_step<D, S, true>), row_grid(c->geo, b.x), b, 0, c->stream, a);
+    else hipLaunchKernelGGL((k_step<D, S>), row_grid(c->geo, b.x), b, 0, c->stream, a);
   }
 };
 template <int D, typename S> struct LaunchMacro {
@@ -614,6 +617,11 @@ int build_pairs(couplb_ctx* c, const SlotHost& sl) {
   if (nh > 0)
     hipLaunchKernelGGL(k_head_info, dim3((nh + tb - 1) / tb), dim3(tb), 0, c->stream, nh,
                        (const int*)c->pheads, (const unsigned*)c->pkey_s, ne, c->phlen, c->phnode);
+  HIPCHK(c, hipGetLastError());
+  if (!c->prun) HIPCHK(c, hipMalloc(&c->prun, sizeof(int2) * c->N));
+  if (nh > 0)
+    hipLaunchKernelGGL(k_run_index, dim3((nh + tb - 1) / tb), dim3(tb), 0, c->stream, nh,
+                       (const int*)c->pheads, (const int*)c->phlen, (const unsigned*)c->phnode, c->prun);
   HIPCHK(c, hipGetLastError());
   c->pnh = nh;
   c->pne = ne;
@@ -1061,6 +1069,11 @@ StepArgs step_args(couplb_ctx* c, const double g[3], int flags) {
   a.has_ibm = c->nslots > 0;
   a.flag = c->flag;
   a.F4 = c->F4;
+  a.pair_mode = 0;
+  a.prun = nullptr;
+  a.pp = nullptr;
+  a.pw = nullptr;
+  a.fsrc = nullptr;
   return a;
 }
 
@@ -1146,11 +1159,12 @@ int step_impl(couplb_ctx* c, const double g[3], int flags, bool timed) {
       const int rc = build_pairs(c, c->slots[0]);
       if (rc) return rc;
     }
-    if (c->pnh > 0)
-      hipLaunchKernelGGL(k_pair_spread, dim3((unsigned)((c->pnh + 255) / 256)), dim3(256), 0, c->stream,
-                         c->pnh, (const int*)c->pheads, (const int*)c->phlen, (const unsigned*)c->phnode,
-                         (const int*)c->pp, (const double*)c->pw, (const double*)c->fsrc, c->F4);
-    HIPCHK(c, hipGetLastError());
+    // the step sums each flagged node's pair run itself (no node-force pass)
+    a.pair_mode = 1;
+    a.prun = c->prun;
+    a.pp = c->pp;
+    a.pw = c->pw;
+    a.fsrc = c->fsrc;
   } else if (c->nslots > 0) {
     // spread the pending marker records into the per-node force (k_spread)
     c->pair_epoch = -1;  // k_spread rewrites the node flags
@@ -1320,7 +1334,7 @@ int couplb_destroy(couplb_ctx* c) {
   (void)hipFree(c->pkey); (void)hipFree(c->pkey_s); (void)hipFree(c->pval); (void)hipFree(c->pval_s);
   (void)hipFree(c->pp); (void)hipFree(c->pw); (void)hipFree(c->ptmp);
   (void)hipFree(c->pheads); (void)hipFree(c->phlen); (void)hipFree(c->phnode); (void)hipFree(c->phflag);
-  (void)hipFree(c->pcount); (void)hipFree(c->seltmp); (void)hipFree(c->fsrc);
+  (void)hipFree(c->pcount); (void)hipFree(c->seltmp); (void)hipFree(c->fsrc); (void)hipFree(c->prun);
   (void)hipFree(c->ut); (void)hipFree(c->mdf_v); (void)hipFree(c->mdf_S);
   (void)hipFree(c->mdf_F); (void)hipFree(c->mdf_R); (void)hipFree(c->mdf_Ap); (void)hipFree(c->mdf_part);
   (void)hipFree(c->mlp_x); (void)hipFree(c->mlp_y); (void)hipFree(c->mlp_sig);

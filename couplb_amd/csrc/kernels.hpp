@@ -342,7 +342,51 @@ struct StepArgs {
   int has_ibm;          // spread force of the last cycle present
   const unsigned char* flag;
   const double* F4;    // node force (fx, fy, fz, unused) where flag is set
+  // fixed-marker map: instead of F4, the step sums the flagged node's run of
+  // (marker, weight) pairs itself: prun[node] = (first pair, count), pp/pw
+  // the sorted pairs, fsrc three reaction values per sorted marker
+  int pair_mode;
+  const int2* prun;
+  const int* pp;
+  const double* pw;
+  const double* fsrc;
 };
+
+// Sum of one node's run of the fixed-marker map, sum_f src[pp[f]] * pw[f]
+// in pair order (couplb_ibm.h:125-130: F_node += F_marker * w); src holds
+// three values per sorted marker.  U pairs (1 or 4) are loaded per round;
+// the sums are the same.
+template <int U = 4>
+__device__ inline void run_sum(int s, int len, const int* pp, const double* pw, const double* src,
+                               double& ax, double& ay, double& az) {
+#pragma clang fp contract(off)
+  ax = ay = az = 0.0;
+  int f = s;
+  const int end = s + len;
+  for (; U > 1 && f + 4 <= end; f += 4) {
+    int q[4];
+    double w[4], v[4][3];
+#pragma unroll
+    for (int i = 0; i < 4; i++) { q[i] = pp[f + i]; w[i] = pw[f + i]; }
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) v[i][c] = src[3 * (size_t)q[i] + c];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      ax += v[i][0] * w[i];
+      ay += v[i][1] * w[i];
+      az += v[i][2] * w[i];
+    }
+  }
+  for (; f < end; f++) {
+    const int q = pp[f];
+    const double w = pw[f];
+    ax += src[3 * (size_t)q + 0] * w;
+    ay += src[3 * (size_t)q + 1] * w;
+    az += src[3 * (size_t)q + 2] * w;
+  }
+}
 
 // Everything of the step after the pull: IBM force, gravity, macroscopic
 // with the Guo half-force, density bookkeeping, collision.  out(q, v) stores
@@ -351,8 +395,10 @@ struct StepArgs {
 // lay, dst: the buffer collide_exact (FP64) writes (the reduced formats
 // store through out); rho_in, rho_out: the density array read for gravity
 // and the one written (a.pc.lay, a.dst, a.pc.rho_coll, a.rho_out in the
-// single-step kernels)
-template <int D, typename S, typename Out>
+// single-step kernels).  PAIR: flagged nodes take their force from their
+// run of the fixed-marker map (a.prun, a.pp, a.pw, a.fsrc) instead of a.F4;
+// a separate instantiation, so the other steps keep their registers.
+template <int D, typename S, bool PAIR = false, typename Out>
 __device__ inline void step_node(const StepArgs& a, const Layout& lay, S* dst, const double* rho_in,
                                  double* rho_out, int i, int j, int k, unsigned node,
                                  typename Store<S>::T (&f)[Lat<D>::Q], Out out) {
@@ -360,13 +406,25 @@ __device__ inline void step_node(const StepArgs& a, const Layout& lay, S* dst, c
   using T = typename Store<S>::T;
   constexpr int Q = L::Q;
   const Geo& g = a.pc.g;
-  // IBM force spread by the last cycle's markers, present only at flagged nodes
+  // IBM force spread by the last cycle's markers, present only at flagged
+  // nodes: the node's pair run (fixed-marker map) or the tile spread's F4
   T Fx = 0, Fy = 0, Fz = 0;
   if (a.has_ibm && a.flag[node]) {
-    const double4 F = *reinterpret_cast<const double4*>(a.F4 + 4 * (size_t)node);
-    Fx = (T)F.x;
-    Fy = (T)F.y;
-    Fz = (T)F.z;
+    if constexpr (PAIR) {
+      const int2 r = a.prun[node];
+      double ax, ay, az;
+      // FP64: four pairs in flight (measured faster); the reduced formats
+      // keep their occupancy with one
+      run_sum<Store<S>::deviation ? 1 : 4>(r.x, r.y, a.pp, a.pw, a.fsrc, ax, ay, az);
+      Fx = (T)ax;
+      Fy = (T)ay;
+      Fz = (T)az;
+    } else {
+      const double4 F = *reinterpret_cast<const double4*>(a.F4 + 4 * (size_t)node);
+      Fx = (T)F.x;
+      Fy = (T)F.y;
+      Fz = (T)F.z;
+    }
   }
   // density (deviation form for the reduced formats: A = rho - base)
   T A = 0, jx = 0, jy = 0, jz = 0;
@@ -443,13 +501,13 @@ __device__ inline void step_node(const StepArgs& a, const Layout& lay, S* dst, c
   }
 }
 
-template <int D, typename S, typename Out>
+template <int D, typename S, bool PAIR = false, typename Out>
 __device__ inline void step_node(const StepArgs& a, int i, int j, int k, unsigned node,
                                  typename Store<S>::T (&f)[Lat<D>::Q], Out out) {
-  step_node<D, S>(a, a.pc.lay, static_cast<S*>(a.dst), a.pc.rho_coll, a.rho_out, i, j, k, node, f, out);
+  step_node<D, S, PAIR>(a, a.pc.lay, static_cast<S*>(a.dst), a.pc.rho_coll, a.rho_out, i, j, k, node, f, out);
 }
 
-template <int D, typename S>
+template <int D, typename S, bool PAIR = false>
 __global__ void __launch_bounds__(256) k_step(StepArgs a) {
   using L = Lat<D>;
   using T = typename Store<S>::T;
@@ -463,8 +521,8 @@ __global__ void __launch_bounds__(256) k_step(StepArgs a) {
   pull_node<D, S>(a.pc, i, j, k, f);
   S* dst = static_cast<S*>(a.dst);
   const size_t at0 = a.pc.lay.at(0, (unsigned)j + (unsigned)g.ny * (unsigned)k, (unsigned)i);
-  step_node<D, S>(a, i, j, k, node, f,
-                  [&](int q, T v) { Store<S>::store(dst + q * a.pc.lay.pq + at0, v); });
+  step_node<D, S, PAIR>(a, i, j, k, node, f,
+                        [&](int q, T v) { Store<S>::store(dst + q * a.pc.lay.pq + at0, v); });
 }
 
 // nsteps LB steps of a small grid in one launch of one workgroup, with the
@@ -581,7 +639,7 @@ __device__ inline void stage_rows(Geo g, Layout lay, const S* src, S* win, int x
   }
 }
 
-template <int D, typename S>
+template <int D, typename S, bool PAIR = false>
 __global__ void __launch_bounds__(lds_chunk<S>()) k_step_lds(StepArgs a) {
   constexpr int LDS_XC = lds_chunk<S>();
   using L = Lat<D>;
@@ -613,7 +671,7 @@ __global__ void __launch_bounds__(lds_chunk<S>()) k_step_lds(StepArgs a) {
       return Store<S>::decode(win[q][t + G - L::ex(q)]);
     });
   __syncthreads();
-  if (live) step_node<D, S>(a, i, j, k, node, f, [&](int q, T v) { res[q][t] = Store<S>::encode(v); });
+  if (live) step_node<D, S, PAIR>(a, i, j, k, node, f, [&](int q, T v) { res[q][t] = Store<S>::encode(v); });
   __syncthreads();
   S* dst = static_cast<S*>(a.dst);
   const unsigned row = (unsigned)j + (unsigned)g.ny * (unsigned)k;

@@ -86,51 +86,12 @@ __global__ void __launch_bounds__(256) k_head_info(int nh, const int* heads, con
   hnode[h] = k;
 }
 
-// Sum of one node's run, sum_f src[pp[f]] * pw[f] in pair order
-// (couplb_ibm.h:125-130: F_node += F_marker * w); src holds three values
-// per sorted marker.  Four pairs are loaded per round.
-__device__ inline void run_sum(int s, int len, const int* pp, const double* pw, const double* src,
-                               double& ax, double& ay, double& az) {
-#pragma clang fp contract(off)
-  ax = ay = az = 0.0;
-  int f = s;
-  const int end = s + len;
-  for (; f + 4 <= end; f += 4) {
-    int q[4];
-    double w[4], v[4][3];
-#pragma unroll
-    for (int i = 0; i < 4; i++) { q[i] = pp[f + i]; w[i] = pw[f + i]; }
-#pragma unroll
-    for (int i = 0; i < 4; i++)
-#pragma unroll
-      for (int c = 0; c < 3; c++) v[i][c] = src[3 * (size_t)q[i] + c];
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-      ax += v[i][0] * w[i];
-      ay += v[i][1] * w[i];
-      az += v[i][2] * w[i];
-    }
-  }
-  for (; f < end; f++) {
-    const int q = pp[f];
-    const double w = pw[f];
-    ax += src[3 * (size_t)q + 0] * w;
-    ay += src[3 * (size_t)q + 1] * w;
-    az += src[3 * (size_t)q + 2] * w;
-  }
-}
-
-// node forces of the fixed-marker map: F_node = run sum of the marker forces
-__global__ void __launch_bounds__(256) k_pair_spread(int nh, const int* heads, const int* hlen,
-                                                     const unsigned* hnode, const int* pp,
-                                                     const double* pw, const double* src,
-                                                     double* F4) {
+// run of every flagged node: prun[node] = (first pair, pair count), read by
+// the LB step (StepArgs::pair_mode)
+__global__ void k_run_index(int nh, const int* heads, const int* hlen, const unsigned* hnode, int2* prun) {
   const int h = blockIdx.x * blockDim.x + threadIdx.x;
   if (h >= nh) return;
-  double ax, ay, az;
-  run_sum(heads[h], hlen[h], pp, pw, src, ax, ay, az);
-  const unsigned k = hnode[h];
-  *reinterpret_cast<double4*>(F4 + 4 * (size_t)k) = double4{ax, ay, az, 0.0};
+  prun[hnode[h]] = int2{heads[h], hlen[h]};
 }
 
 // p[0..n) = v, 16 bytes per thread (p 16-byte aligned, as device

@@ -74,7 +74,12 @@ int couplb_fix_set_variable_callback(couplb_fix* fx, couplb_var_eval fn, void* u
 int couplb_fix_set_domain(couplb_fix* fx, const couplb_domain* dom);
 int couplb_fix_init(couplb_fix* fx);                     /* FixCoupLB::init */
 int couplb_fix_setup(couplb_fix* fx, const couplb_atoms* a);      /* ::setup */
-int couplb_fix_post_force(couplb_fix* fx, const couplb_atoms* a); /* ::post_force */
+/* ::post_force.  On a grid whose steps fuse (couplb_lbm_fuses_steps) the
+ * LB step of a sub-step with no group atoms is held back and issued with
+ * the following ones as one run, at the latest by the next output, check,
+ * VTK or checkpoint step, IBM sub-step with atoms, setup or context
+ * access, a change of gravity, or after 512 steps; results are the same. */
+int couplb_fix_post_force(couplb_fix* fx, const couplb_atoms* a);
 /* ::end_of_step; VTK output reads the atom arrays handed to the last
  * setup / post_force call (the same timestep's arrays in LAMMPS) */
 int couplb_fix_end_of_step(couplb_fix* fx, long step);
@@ -90,7 +95,7 @@ int couplb_fix_last_diag(const couplb_fix* fx, double out[6]);
 /* last y-profile in physical units: Ny rows each of rho, ux, uy */
 int couplb_fix_last_profile(const couplb_fix* fx, double* rho, double* ux, double* uy);
 /* device context, for state import/export; issues the LB steps the fix
- * still holds back first (see couplb_fix_end_of_step) */
+ * still holds back first (see couplb_fix_post_force) */
 void* couplb_fix_context(couplb_fix* fx);
 /* captured screen output (the reference prints to LAMMPS' screen) */
 const char* couplb_fix_log(const couplb_fix* fx);

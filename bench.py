@@ -641,6 +641,30 @@ def main():
     dev.sync()
     mlups_moving = nodes * mv_steps / (time.perf_counter() - t2) / 1e6
 
+    # the fixed-marker steps with the marker arrays in host memory, as the
+    # LAMMPS fix hands them over (it packs the group into pinned buffers):
+    # every sub-step copies x, v, m to the device and the forces back, and
+    # waits for them (PCIe-inclusive rate, not `value`)
+    hx = torch.from_numpy(x).pin_memory().numpy()
+    hv = torch.from_numpy(v).pin_memory().numpy()
+    hm = torch.from_numpy(m).pin_memory().numpy()
+    hf = torch.zeros((n, 3), dtype=torch.float64).pin_memory().numpy()
+
+    def one_step_host():
+        dev.step(g, couplb_amd.RHO_REFRESHED_NEXT)
+        for _ in range(md):
+            hf[:] = 0.0
+            dev.ibm(hx, hv, hm, hf, positions_unchanged=True, **ibm_kw)
+
+    host_steps = max(4, args.steps // 4)
+    one_step_host()
+    dev.sync()
+    t3 = time.perf_counter()
+    for _ in range(host_steps):
+        one_step_host()
+    dev.sync()
+    mlups_host = nodes * host_steps / (time.perf_counter() - t3) / 1e6
+
     # achievable-bandwidth reference: a plain copy of one population buffer's
     # worth (k_step reads one state and writes the other)
     copy = dev.copy_bandwidth(nodes * Q * s_bytes, 10)
@@ -701,6 +725,7 @@ def main():
                 "hbm_frac_without_gather": round(markers_per_s_wo * IBM_BYTES_PER_MARKER
                                                  / (HBM_PEAK_GBS * 1e9), 4)},
         "mlups_markers_moving": round(mlups_moving, 1),
+        "mlups_marker_arrays_on_host": round(mlups_host, 1),
         "lb_only_mlups": round(lb_only, 1),
         "lb_only_hbm_frac": round(lb_only * 1e6 * b_lup / (HBM_PEAK_GBS * 1e9), 4),
         "mass": diag["mass"],

@@ -112,6 +112,8 @@ def lib() -> C.CDLL:
         "couplb_lbm_steps_async": (ip, [vp, dp, ip, ip]),
         "couplb_lbm_fuses_steps": (ip, [vp]),
         "couplb_set_timer_mask": (ip, [vp, C.c_uint]),
+        "couplb_host_alloc": (vp, [C.c_size_t]),
+        "couplb_host_free": (None, [vp]),
         "couplb_copy_bandwidth": (ip, [vp, C.c_size_t, ip, dp]),
         "couplb_ibm_substep": (ip, [vp, ip, dp, dp, dp, dp, C.POINTER(IbmArgs)]),
         "couplb_ibm_substep_device": (ip, [vp, ip, vp, vp, vp, vp, C.POINTER(IbmArgs)]),

@@ -69,6 +69,8 @@ struct couplb_ctx {
   int cycle = 0;         // state counter for the u stamps
   size_t fused_nodes = 0;  // grids up to this many nodes: k_steps_fused
   unsigned timer_mask = ~0u;  // timed region kinds (bit per kind)
+  double* hstage = nullptr;   // pinned host staging of the host-array IBM forces
+  size_t hstage_n = 0;
   size_t fused_lds = 0;    // its LDS bytes; 0 when steps do not fuse
   hipStream_t stream = nullptr;
   TimerRing tr;
@@ -1331,6 +1333,7 @@ int couplb_destroy(couplb_ctx* c) {
   (void)hipFree(c->dx); (void)hipFree(c->dv); (void)hipFree(c->dm); (void)hipFree(c->df);
   (void)hipFree(c->ufc);
   (void)hipFree(c->staging);
+  if (c->hstage) (void)hipHostFree(c->hstage);
   (void)hipFree(c->pkey); (void)hipFree(c->pkey_s); (void)hipFree(c->pval); (void)hipFree(c->pval_s);
   (void)hipFree(c->pp); (void)hipFree(c->pw); (void)hipFree(c->ptmp);
   (void)hipFree(c->pheads); (void)hipFree(c->phlen); (void)hipFree(c->phnode); (void)hipFree(c->phflag);
@@ -1441,6 +1444,19 @@ int couplb_lbm_steps_async(couplb_ctx* c, const double g[3], int flags, int nste
   return COUPLB_OK;
 }
 
+void* couplb_host_alloc(size_t bytes) {
+  void* p = nullptr;
+  if (hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  return p;
+}
+
+void couplb_host_free(void* p) {
+  if (p) (void)hipHostFree(p);
+}
+
 int couplb_set_timer_mask(couplb_ctx* c, unsigned mask) {
   if (!c) return COUPLB_EINVAL;
   c->timer_mask = mask;
@@ -1473,8 +1489,18 @@ int couplb_ibm_substep(couplb_ctx* c, int n, const double* x, const double* v, c
   HIPCHK(c, hipMemcpyAsync(c->dm, m, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
   rc = ibm_impl(c, n, c->dx, c->dv, c->dm, c->df, 1, a);
   if (rc) return rc;
-  std::vector<double> F((size_t)3 * n);
-  HIPCHK(c, hipMemcpyAsync(F.data(), c->df, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+  // the forces come back through a pinned staging buffer (DMA)
+  if ((size_t)3 * n > c->hstage_n) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->hstage) (void)hipHostFree(c->hstage);
+    c->hstage = nullptr;
+    c->hstage_n = 0;
+    HIPCHK(c, hipHostMalloc(reinterpret_cast<void**>(&c->hstage), sizeof(double) * 3 * (size_t)n,
+                            hipHostMallocDefault));
+    c->hstage_n = (size_t)3 * n;
+  }
+  double* F = c->hstage;
+  HIPCHK(c, hipMemcpyAsync(F, c->df, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   for (size_t i = 0; i < (size_t)3 * n; i++) f[i] += F[i];  // f[i] += F (fix_couplb.cpp:783-785)
   return COUPLB_OK;

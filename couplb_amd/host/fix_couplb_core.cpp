@@ -482,7 +482,7 @@ void FixCoupLBCore::ibm_sub_coupling(const couplb_atoms& a, bool first) {
   ia.positions_unchanged =
       (px_prev.size() == px.size() && std::memcmp(px_prev.data(), px.data(), sizeof(double) * px.size()) == 0)
           ? 1 : 0;
-  px_prev = px;
+  px_prev.assign(px.data(), px.data() + px.size());
   check(couplb_ibm_substep(ctx, ng, px.data(), pv.data(), pm.data(), pf.data(), &ia), "IBM sub-step");
   for (int k = 0; k < ng; k++)
     for (int d = 0; d < 3; d++) a.f[3 * pidx[k] + d] = pf[3 * k + d];

@@ -5,7 +5,9 @@
 // validation messages, unit conversion, callback order and sub-stepping --
 // while every grid operation goes through the C ABI in couplb_hip.h.
 #pragma once
+#include <cstddef>
 #include <cstdio>
+#include <new>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -14,6 +16,38 @@
 #include "../../include/couplb_hip.h"
 
 namespace couplb_host {
+
+// A double array in pinned host memory from the device library
+// (couplb_host_alloc); grows, never shrinks.
+class PinnedVec {
+ public:
+  PinnedVec() = default;
+  ~PinnedVec() { couplb_host_free(p_); }
+  PinnedVec(const PinnedVec&) = delete;
+  PinnedVec& operator=(const PinnedVec&) = delete;
+  void resize(size_t n) {
+    if (n > cap_) {
+      couplb_host_free(p_);
+      p_ = static_cast<double*>(couplb_host_alloc(n * sizeof(double)));
+      cap_ = p_ ? n : 0;
+      if (!p_) throw std::bad_alloc();
+    }
+    n_ = n;
+  }
+  void assign(size_t n, double v) {
+    resize(n);
+    for (size_t i = 0; i < n; i++) p_[i] = v;
+  }
+  double* data() { return p_; }
+  const double* data() const { return p_; }
+  size_t size() const { return n_; }
+  double& operator[](size_t i) { return p_[i]; }
+  double operator[](size_t i) const { return p_[i]; }
+
+ private:
+  double* p_ = nullptr;
+  size_t n_ = 0, cap_ = 0;
+};
 
 // error->all / error->one (LAMMPS aborts; here the C boundary reports it)
 struct fix_error : std::runtime_error {
@@ -92,7 +126,9 @@ class FixCoupLBCore {
   bool have_profile = false;
   std::string screen;
 
-  std::vector<double> px, pv, pm, pf;
+  // packed group atoms in pinned host memory (the sub-step's copies to and
+  // from the device run as DMA)
+  PinnedVec px, pv, pm, pf;
   std::vector<double> px_prev;  // positions of the previous sub-step (reuse test)
   std::vector<int> pidx;
   couplb_atoms atoms_now{};  // the arrays of the current step (setup / post_force)

@@ -113,6 +113,11 @@ int couplb_lbm_fuses_steps(const couplb_ctx* ctx);
  * microseconds between kernels; untimed kinds record none.  Measurement
  * helper, no counterpart in the reference. */
 int couplb_set_timer_mask(couplb_ctx* ctx, unsigned mask);
+/* Pinned (page-locked) host memory for the caller's packing buffers, so
+ * the host-array IBM sub-step's copies run as DMA; null on failure.  No
+ * counterpart in the reference (its arrays never leave the host). */
+void* couplb_host_alloc(size_t bytes);
+void couplb_host_free(void* p);
 
 /* IBM sub-step: replaces the loop body of do_ibm_sub_coupling
  * (fix_couplb.cpp:757-797): fluid velocity at the marker (IBM::interpolate,

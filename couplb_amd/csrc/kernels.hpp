@@ -135,7 +135,9 @@ struct PullCtx {
 // lay, src, rho_coll: the state the wall rules read and the density of the
 // collision that made it (pc.lay, pc.src, pc.rho_coll in global memory for
 // the single-step kernels; LDS copies in k_steps_fused)
-template <int D, typename S, typename Fluid>
+// SCALED: leave the reduced formats' values in their stored scale (the LB
+// step works in it); otherwise divide by it
+template <int D, typename S, bool SCALED = false, typename Fluid>
 __device__ inline void pull_node_with(const PullCtx& pc, const Layout& lay, const S* src,
                                       const double* rho_coll, int i, int j, int k,
                                       typename Store<S>::T (&f)[Lat<D>::Q], Fluid fluid) {
@@ -149,7 +151,7 @@ __device__ inline void pull_node_with(const PullCtx& pc, const Layout& lay, cons
 #pragma unroll
     for (int q = 0; q < Q; q++) {
       f[q] = Store<S>::load(src + lay.at(q, row, i));
-      if (Store<S>::deviation) f[q] *= (T)pc.inv_scale;
+      if (Store<S>::deviation && !SCALED) f[q] *= (T)pc.inv_scale;
     }
     return;
   }
@@ -183,7 +185,7 @@ __device__ inline void pull_node_with(const PullCtx& pc, const Layout& lay, cons
       const int sj = ey == 1 ? jm : (ey == -1 ? jp : j);
       const int sk = (D == 3) ? (ez == 1 ? km : (ez == -1 ? kp : k)) : 0;
       f[q] = fluid(q, si, sj, sk);
-      if (Store<S>::deviation) f[q] *= (T)pc.inv_scale;
+      if (Store<S>::deviation && !SCALED) f[q] *= (T)pc.inv_scale;
     }
     return;
   }
@@ -232,22 +234,22 @@ __device__ inline void pull_node_with(const PullCtx& pc, const Layout& lay, cons
         }
       }
     }
-    if (Store<S>::deviation) f[q] *= (T)pc.inv_scale;
+    if (Store<S>::deviation && !SCALED) f[q] *= (T)pc.inv_scale;
   }
 }
 
-template <int D, typename S, typename Fluid>
+template <int D, typename S, bool SCALED = false, typename Fluid>
 __device__ inline void pull_node_with(const PullCtx& pc, int i, int j, int k,
                                       typename Store<S>::T (&f)[Lat<D>::Q], Fluid fluid) {
-  pull_node_with<D, S>(pc, pc.lay, static_cast<const S*>(pc.src), pc.rho_coll, i, j, k, f, fluid);
+  pull_node_with<D, S, SCALED>(pc, pc.lay, static_cast<const S*>(pc.src), pc.rho_coll, i, j, k, f, fluid);
 }
 
-template <int D, typename S>
+template <int D, typename S, bool SCALED = false>
 __device__ inline void pull_node(const PullCtx& pc, int i, int j, int k,
                                  typename Store<S>::T (&f)[Lat<D>::Q]) {
   const S* src = static_cast<const S*>(pc.src);
   const int ny = pc.g.ny;
-  pull_node_with<D, S>(pc, i, j, k, f, [&](int q, int si, int sj, int sk) {
+  pull_node_with<D, S, SCALED>(pc, i, j, k, f, [&](int q, int si, int sj, int sk) {
     return Store<S>::load(src + pc.lay.at(q, (unsigned)sj + (unsigned)ny * (unsigned)sk, (unsigned)si));
   });
 }
@@ -426,7 +428,9 @@ __device__ inline void step_node(const StepArgs& a, const Layout& lay, S* dst, c
       Fz = (T)F.z;
     }
   }
-  // density (deviation form for the reduced formats: A = rho - base)
+  // density (deviation form for the reduced formats: A = rho - base).  The
+  // reduced formats' populations arrive in their stored scale s (a power of
+  // two), so the sums are s times the unscaled ones, exactly
   T A = 0, jx = 0, jy = 0, jz = 0;
 #pragma unroll
   for (int q = 0; q < Q; q++) {
@@ -435,11 +439,20 @@ __device__ inline void step_node(const StepArgs& a, const Layout& lay, S* dst, c
     if (L::ey(q) == 1) jy += f[q]; else if (L::ey(q) == -1) jy -= f[q];
     if (D == 3) { if (L::ez(q) == 1) jz += f[q]; else if (L::ez(q) == -1) jz -= f[q]; }
   }
+  T As = A;  // A in the stored scale
+  if constexpr (Store<S>::deviation) {
+    const T is = (T)a.pc.inv_scale;
+    A *= is;
+    jx *= is;
+    jy *= is;
+    jz *= is;
+  }
   const T base = (T)a.pc.base;
   T rho = A + base;
   if (!(rho > (T)RHO_MIN)) {  // also catches NaN (couplb_lattice.h:178-186)
     rho = (T)RHO_MIN;
     A = rho - base;
+    As = A * (T)a.pc.scale;
   }
   // gravity: F += g * rho_array (fix_couplb.cpp:706)
   if (a.grav_mode) {
@@ -470,6 +483,9 @@ __device__ inline void step_node(const StepArgs& a, const Layout& lay, S* dst, c
     const T uu = ux * ux + uy * uy + uz * uz;
     const T uf = ux * Fx + uy * Fy + uz * Fz;
     const T scale = (T)a.pc.scale;
+    // The populations stay in the stored scale s: every term below is s
+    // times the unscaled one (s a power of two, so each product and sum is
+    // the unscaled result times s exactly).
     // BGK + Guo per direction (couplb_collision.h:36-58), in deviation form:
     //   feq - w*base = w*(A + rho*(3eu + 4.5eu^2 - 1.5uu))   (couplb_lattice.h:137-144)
     //   S = w*(1-om/2)*(3(e.F - u.F) + 9(e.u)(e.F))
@@ -479,8 +495,8 @@ __device__ inline void step_node(const StepArgs& a, const Layout& lay, S* dst, c
     // (asym = 3 rho eu, Sasym = 3 hom ef) change sign: one evaluation per
     // pair of opposite directions.  f - feq stays one difference, as in the
     // reference's form, so the non-equilibrium part keeps its precision.
-    const T uu15 = (T)1.5 * uu, rho3 = (T)3 * rho;
-    const T hom3 = (T)3 * hom, hom9 = (T)9 * hom, huf3 = hom3 * uf;
+    const T uu15 = (T)1.5 * uu, rho_s = rho * scale, rho3 = ((T)3 * rho) * scale;
+    const T hom3 = ((T)3 * hom) * scale, hom9 = ((T)9 * hom) * scale, huf3 = hom3 * uf;
 #pragma unroll
     for (int q = 0; q < Q; q++) {
       const int o = L::opp(q);
@@ -488,15 +504,15 @@ __device__ inline void step_node(const StepArgs& a, const Layout& lay, S* dst, c
       const T eu = esum<D, T>(q, ux, uy, uz);
       const T ef = esum<D, T>(q, Fx, Fy, Fz);
       const T wq = (T)L::w(q);
-      const T sym = A + rho * ((T)4.5 * eu * eu - uu15);
+      const T sym = As + rho_s * ((T)4.5 * eu * eu - uu15);
       const T Ssym = hom9 * eu * ef - huf3;
       if (o == q) {
-        out(q, (f[q] - om * (f[q] - wq * sym) + wq * Ssym) * scale);
+        out(q, f[q] - om * (f[q] - wq * sym) + wq * Ssym);
         continue;
       }
       const T asym = rho3 * eu, Sasym = hom3 * ef;
-      out(q, (f[q] - om * (f[q] - wq * (sym + asym)) + wq * (Ssym + Sasym)) * scale);
-      out(o, (f[o] - om * (f[o] - wq * (sym - asym)) + wq * (Ssym - Sasym)) * scale);
+      out(q, f[q] - om * (f[q] - wq * (sym + asym)) + wq * (Ssym + Sasym));
+      out(o, f[o] - om * (f[o] - wq * (sym - asym)) + wq * (Ssym - Sasym));
     }
   }
 }
@@ -518,7 +534,7 @@ __global__ void __launch_bounds__(256) k_step(StepArgs a) {
   if (i >= g.nx) return;
   const unsigned node = (unsigned)i + (unsigned)g.nx * ((unsigned)j + (unsigned)g.ny * (unsigned)k);
   T f[Q];
-  pull_node<D, S>(a.pc, i, j, k, f);
+  pull_node<D, S, true>(a.pc, i, j, k, f);
   S* dst = static_cast<S*>(a.dst);
   const size_t at0 = a.pc.lay.at(0, (unsigned)j + (unsigned)g.ny * (unsigned)k, (unsigned)i);
   step_node<D, S, PAIR>(a, i, j, k, node, f,
@@ -573,7 +589,7 @@ __global__ void __launch_bounds__(FUSED_THREADS) k_steps_fused(StepArgs a, int n
       const unsigned row = node / nx, i = node - row * nx;
       const unsigned j = row % ny, k = row / ny;
       T f[Q];
-      pull_node_with<D, S>(a.pc, ll, src, R, (int)i, (int)j, (int)k, f, [&](int q, int si, int sj, int sk) {
+      pull_node_with<D, S, true>(a.pc, ll, src, R, (int)i, (int)j, (int)k, f, [&](int q, int si, int sj, int sk) {
         return Store<S>::load(src + ll.at(q, (unsigned)sj + ny * (unsigned)sk, (unsigned)si));
       });
       const size_t at0 = ll.at(0, row, i);
@@ -667,7 +683,7 @@ __global__ void __launch_bounds__(lds_chunk<S>()) k_step_lds(StepArgs a) {
   const unsigned node = (unsigned)i + (unsigned)g.nx * ((unsigned)j + (unsigned)g.ny * (unsigned)k);
   T f[Q];
   if (live)
-    pull_node_with<D, S>(a.pc, i, j, k, f, [&](int q, int, int, int) {
+    pull_node_with<D, S, true>(a.pc, i, j, k, f, [&](int q, int, int, int) {
       return Store<S>::decode(win[q][t + G - L::ex(q)]);
     });
   __syncthreads();

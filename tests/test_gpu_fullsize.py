@@ -9,13 +9,15 @@ marker set of bench.py), driven through the fix.
   conserved to FP64 reduction precision, and (without gravity) the fluid
   momentum gained in each interval equal to the reactions spread by its
   sub-steps (Guo's identity with the exact spreading normalisation).
+- The reduced storage formats on the same workload against the FP64 path.
 """
 import numpy as np
 import pytest
 
 import couplb_amd
 from bench import make_markers
-from tests.scenarios import atoms_from, copy_atoms, interior, rel_l2, run_device, run_oracle
+from tests.scenarios import (DeviceArray, atoms_from, copy_atoms, interior, rel_l2, run_device,
+                             run_oracle)
 
 pytestmark = pytest.mark.gpu
 
@@ -77,3 +79,38 @@ def test_headline_shape_conserves_mass_and_momentum():
         spread = sum(reaction[s0 - MD + 1:s0 + 1]) / MD
         print(f"  steps {s0}->{s1}: fluid momentum gained {gained}, reactions {spread}", flush=True)
         assert np.allclose(gained, spread, rtol=1e-9, atol=1e-12 * np.abs(spread).max())
+
+
+@pytest.mark.parametrize("storage", ["fp16", "bf16", "fp32"])
+def test_headline_shape_reduced_storage_error(storage):
+    # bench.py's workload (uniform flow 0.02, gravity, the 101,810 fixed
+    # markers) for 20 LB steps with reduced storage against the FP64 device
+    # path (tied to the oracle above): relative L2 of u within 1e-3 for FP16
+    # (the target of the reduced formats), 2e-3 for BF16 (8 significant
+    # bits), 1e-5 for FP32
+    x, v, m, _ = make_markers(N, 100000)
+    n = len(m)
+    dx, dv, dm = DeviceArray(x), DeviceArray(v), DeviceArray(m)
+    kw = dict(lo=(0.0, 0.0, 0.0), inv_dx=1.0, xi_over_dtLB=1.0, vel_scale=1.0, inv_N=1.0 / MD,
+              force_scale=1.0)
+    g = (1e-6, 0.0, 0.0)
+    print(f"case 256^3 {storage} vs FP64, 20 LB steps", flush=True)
+    fields = {}
+    for st in (storage, "fp64"):
+        d = couplb_amd.Device(3, N, N, N, tau=0.8, kernel=couplb_amd.PESKIN4, storage=st, md_per_lb=MD)
+        d.init_equilibrium(1.0, (0.02, 0.0, 0.0))
+        df = DeviceArray(np.zeros((n, 3)))
+        for s in range(1 + 20 * MD):
+            if s and (s - 1) % MD == 0:
+                d.step(g, couplb_amd.RHO_REFRESHED_NEXT)
+            d.ibm_device(n, dx.addr, dv.addr, dm.addr, df.addr, positions_unchanged=s > 0, **kw)
+        d.sync()
+        _, ux, uy, uz = d.macro()
+        fields[st] = np.stack([ux, uy, uz])
+        d.close()
+        del df
+    a, b = fields[storage], fields["fp64"]
+    err = np.linalg.norm(a - b) / np.linalg.norm(b)
+    print(f"  rel-L2(u) {err:.3e}", flush=True)
+    bound = {"fp16": 1e-3, "bf16": 2e-3, "fp32": 1e-5}[storage]
+    assert err <= bound

@@ -28,7 +28,8 @@ struct Walls {
 };
 
 // One IBM marker contribution record, written by k_marker and read by the
-// fused step's force gather.  The weights are the 1-D delta values at the
+// spreading (the tile gather k_spread, or the pair map built from it for
+// fixed markers).  The weights are the 1-D delta values at the
 // four stencil offsets -1..+2 of the base cell (i0, j0, k0) in the
 // reference's ghost-inclusive coordinates (couplb_ibm.h:57-73).
 struct MarkerRec {
@@ -255,7 +256,8 @@ __device__ inline void pull_node(const PullCtx& pc, int i, int j, int k,
 }
 
 // ---------------------------------------------------------------------------
-// The fused step: pull (stream of the previous step) -> IBM force gather ->
+// The fused step: pull (stream of the previous step) -> IBM force (the
+// node's pair-run sum for fixed markers, else the spread node force) ->
 // gravity -> macroscopic with Guo half-force -> BGK + Guo collision -> store.
 // Replaces do_lbm_step (fix_couplb.cpp:659-669): apply_external_force
 // (680-708), compute_macroscopic(true) (couplb_lattice.h:155-202),

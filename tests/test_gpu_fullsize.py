@@ -4,7 +4,8 @@ marker set of bench.py), driven through the fix.
 
 - Against the oracle fix (OpenMP on the host cores) for two LB intervals from
   rest: populations within 1e-12 relative L2, per-atom forces within 1e-11
-  (only the spreading's summation order differs).
+  (only the spreading's summation order differs), with the markers fixed and
+  with the particles moving at every MD step.
 - Size-independent properties over ten intervals on the device alone: mass
   conserved to FP64 reduction precision, and (without gravity) the fluid
   momentum gained in each interval equal to the reactions spread by its
@@ -38,6 +39,39 @@ def test_headline_shape_matches_oracle():
     a = interior(dev.device_state().reshape(19, N + 2, N + 2, N + 2), 3)
     dev.close()
     ref, fo = run_oracle(cmd, 2 * MD, dt=1.0 / MD, atoms_fn=fn)
+    b = interior(ref.f(), 3)
+    err = rel_l2(a, b)
+    del a
+    ferr = max(np.max(np.abs(x - y)) / np.max(np.abs(y)) for x, y in zip(fd, fo))
+    print(f"  rel-L2(f) = {err:.3e}, max per-atom force difference / max = {ferr:.3e}", flush=True)
+    assert err <= 1e-12
+    assert ferr <= 1e-11
+    ref.close()
+
+
+def test_headline_shape_moving_markers_match_oracle():
+    # the particles advance by v dt at every MD step (wrapped into the
+    # periodic box), so every sub-step has new positions: the device orders
+    # the markers, rebuilds the coverage and spreads node-centrically tile by
+    # tile instead of using the fixed-marker pair map
+    cmd = f"flow all couplb {N} {N} {N} 0.1 1.0 kernel peskin4 md_per_lb {MD} gravity 1e-6 0.0 0.0"
+    base = headline_atoms()
+    dt = 1.0 / MD
+
+    def moved(step):
+        a = copy_atoms(base)
+        x = a["x"] + base["v"] * (dt * step)
+        x -= N * np.floor(x / N)
+        x[x >= N] = 0.0
+        a["x"] = np.ascontiguousarray(x)
+        return a
+
+    print(f"case 256^3 FP64, {len(base['mass'])} moving markers, 2 LB intervals: device vs oracle",
+          flush=True)
+    dev, fd = run_device(cmd, 2 * MD, dt=dt, atoms_fn=moved)
+    a = interior(dev.device_state().reshape(19, N + 2, N + 2, N + 2), 3)
+    dev.close()
+    ref, fo = run_oracle(cmd, 2 * MD, dt=dt, atoms_fn=moved)
     b = interior(ref.f(), 3)
     err = rel_l2(a, b)
     del a

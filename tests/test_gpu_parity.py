@@ -164,10 +164,18 @@ def test_fixed_markers_reuse_matches_oracle(walls):
     assert abs(dd["mass"] - od["mass"]) <= 1e-12 * od["mass"]
 
 
-def test_device_reuse_matches_full_recompute():
+# reuse vs recompute bound per storage format: FP64 differs only in the
+# spreading's summation order; a reduced format can also round a stored
+# population to the neighbouring value where that order moved the sum
+REUSE_TOL = {"fp64": 1e-14, "fp32": 1e-9, "fp16": 1e-5, "bf16": 1e-4}
+
+
+@pytest.mark.parametrize("storage", ["fp64", "fp32", "fp16", "bf16"])
+def test_device_reuse_matches_full_recompute(storage):
     """Device API: the same four sub-steps with positions declared unchanged
-    (merged records) and declared new (one record per sub-step) give the
-    same populations and atom forces up to summation order."""
+    (merged records, the fixed-marker map summed inside the step) and declared
+    new (one record per sub-step, spread tile by tile) give the same
+    populations and atom forces up to summation order."""
     import couplb_amd
     from tests.scenarios import DeviceArray
     rng = np.random.default_rng(5)
@@ -175,7 +183,8 @@ def test_device_reuse_matches_full_recompute():
     x = rng.uniform(0.0, 24.0, (n, 3))
     res = []
     for reuse in (True, False):
-        d = couplb_amd.Device(3, 24, 24, 24, tau=0.8, kernel=couplb_amd.PESKIN4, md_per_lb=4)
+        d = couplb_amd.Device(3, 24, 24, 24, tau=0.8, kernel=couplb_amd.PESKIN4, md_per_lb=4,
+                              storage=storage)
         d.init_equilibrium(1.0, (0.02, 0.01, 0.0))
         dx, dm = DeviceArray(x), DeviceArray(np.ones(n))
         dv, df = DeviceArray(np.zeros((n, 3))), DeviceArray(np.zeros((n, 3)))
@@ -192,7 +201,7 @@ def test_device_reuse_matches_full_recompute():
         d.sync()
         res.append((d.get_state(), df.download()))
     (f1, a1), (f2, a2) = res
-    print("reuse vs recompute: rel-L2(f) =", rel_l2(f1, f2), "rel-L2(F_atom) =", rel_l2(a1, a2))
+    print(f"{storage} reuse vs recompute: rel-L2(f) =", rel_l2(f1, f2), "rel-L2(F_atom) =", rel_l2(a1, a2))
     assert np.all(np.isfinite(f1)) and np.abs(a1).max() > 0
-    assert rel_l2(f1, f2) <= 1e-14
-    assert rel_l2(a1, a2) <= 1e-14
+    assert rel_l2(f1, f2) <= REUSE_TOL[storage]
+    assert rel_l2(a1, a2) <= REUSE_TOL[storage]

@@ -16,6 +16,8 @@
 #ifndef COUPLB_FIX_H
 #define COUPLB_FIX_H
 
+#include "couplb_hip.h" /* status codes COUPLB_OK, COUPLB_EINVAL, ... */
+
 #ifdef __cplusplus
 extern "C" {
 #endif

@@ -14,6 +14,8 @@
 #ifndef COUPLB_HIP_H
 #define COUPLB_HIP_H
 
+#include <stddef.h> /* size_t */
+
 #ifdef __cplusplus
 extern "C" {
 #endif

@@ -1,0 +1,73 @@
+"""The LAMMPS adapter (couplb_amd/lammps/fix_couplb.{h,cpp}) built against a
+test-only stand-in for the LAMMPS classes it touches (tests/lammps_mock) and
+run in LAMMPS' call order (fix_couplb.h:24-30: constructor, setmask, init,
+setup, post_force and end_of_step per step): it must return the per-atom
+forces that the same fix core returns through its C entry points."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import couplb_amd
+from tests.scenarios import atoms_from
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+DRIVER = os.path.join(HERE, "lammps_mock", "driver")
+ARGS = ("flow all couplb 12 10 8 0.1 1.0 wall_y 1 1 gravity 1e-5 0.0 0.0 kernel peskin4 "
+        "md_per_lb 2").split()
+DT, STEPS = 0.5, 8
+
+
+@pytest.mark.parametrize("header", ["couplb_hip.h", "couplb_fix.h", "couplb_mlp.h"])
+def test_header_compiles_alone_as_c99(header, tmp_path):
+    src = tmp_path / "t.c"
+    src.write_text(f'#include "{header}"\nint main(void) {{ return 0; }}\n')
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+                    "-fsyntax-only", str(src)], check=True)
+
+
+def test_adapter_compiles_against_lammps_interface():
+    subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only",
+                    "-I", os.path.join(HERE, "lammps_mock"), "-I", os.path.join(ROOT, "include"),
+                    "-I", os.path.join(ROOT, "couplb_amd", "lammps"),
+                    os.path.join(ROOT, "couplb_amd", "lammps", "fix_couplb.cpp")], check=True)
+
+
+@pytest.mark.gpu
+def test_adapter_forces_equal_fix_core(tmp_path):
+    assert os.path.exists(DRIVER), "build the adapter driver first (__graft_entry__.build())"
+    rng = np.random.default_rng(4)
+    n = 24
+    x = rng.uniform([1.0, 2.5, 1.0], [11.0, 7.5, 7.0], (n, 3))
+    v = rng.normal(0, 1e-3, (n, 3))
+    m = rng.uniform(0.5, 2.0, n)
+    atoms = tmp_path / "atoms.bin"
+    with open(atoms, "wb") as fh:
+        np.array([n], np.int32).tofile(fh)
+        for arr in (x, v, m):
+            np.ascontiguousarray(arr, np.float64).tofile(fh)
+    out = tmp_path / "forces.bin"
+    print(f"case adapter: 12x10x8, {n} atoms, {STEPS} MD steps", flush=True)
+    r = subprocess.run([DRIVER, str(atoms), str(out), str(STEPS), str(DT)] + ARGS,
+                       capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    got = np.fromfile(out, np.float64).reshape(STEPS, n, 3)
+
+    fx = couplb_amd.Fix(" ".join(ARGS), dt=DT)
+    fx.init()
+    a = atoms_from(x, v, m)
+    fx.setup(a)
+    want = []
+    for step in range(1, STEPS + 1):
+        fx.step = step
+        a["f"][:] = 0.0
+        fx.post_force(a)
+        fx.end_of_step(step)
+        want.append(a["f"].copy())
+    fx.close()
+    want = np.stack(want)
+    assert np.abs(want).max() > 0
+    assert np.array_equal(got, want)

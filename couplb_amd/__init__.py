@@ -447,6 +447,8 @@ class Fix:
             key = name.decode()
             if key not in self.variables:
                 return 1
+            if not out:         # init: existence check only
+                return 0
             val = self.variables[key]
             out[0] = float(val(self.step) if callable(val) else val)
             return 0

@@ -411,10 +411,19 @@ int ensure_ibm(couplb_ctx* c, int n) {
                                       (size_t)cap, 0u, key_bits(c->geo.nrows), c->stream));
   c->sort_tmp_bytes = bytes;
   HIPCHK(c, hipMalloc(&c->sort_tmp, bytes));
-  for (auto& s : c->slots) {
-    (void)hipFree(s.rec);
-    s.rec = nullptr;
-    HIPCHK(c, hipMalloc(&s.rec, sizeof(MarkerRec) * cap));
+  // record slots: the pending ones (sub-steps of this LB interval not yet
+  // spread) keep their records, so a group that grows within an interval
+  // still spreads what the earlier sub-steps recorded
+  for (int s = 0; s < (int)c->slots.size(); s++) {
+    SlotHost& sl = c->slots[s];
+    MarkerRec* nr = nullptr;
+    HIPCHK(c, hipMalloc(&nr, sizeof(MarkerRec) * cap));
+    if (s < c->nslots && sl.n > 0)
+      HIPCHK(c, hipMemcpyAsync(nr, sl.rec, sizeof(MarkerRec) * (size_t)sl.n, hipMemcpyDeviceToDevice,
+                               c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    (void)hipFree(sl.rec);
+    sl.rec = nr;
   }
   c->cap = cap;
   c->order_n = -1;

@@ -164,6 +164,7 @@ __device__ inline double4 upull_moments(const typename Store<S>::T (&f)[Lat<D>::
     if (L::ey(q) > 0) jy += fv; else if (L::ey(q) < 0) jy -= fv;
     if (L::ez(q) > 0) jz += fv; else if (L::ez(q) < 0) jz -= fv;
   }
+  if (!isfinite(A)) moments_full<D>(f, jx, jy, jz);  // 0 * Inf = NaN, as the reference
   double r = A + base;
   if (r <= RHO_MIN) r = RHO_MIN;
   if (!isfinite(r)) r = RHO_MIN;

@@ -11,6 +11,8 @@
 //   Periodic faces are index wraps; wall faces are coordinate tests; the
 //   reference's ghost planes (couplb_lattice.h:82-104) are never stored.
 #pragma once
+#include <type_traits>
+
 #include "lattice.hpp"
 
 namespace couplb {
@@ -289,6 +291,33 @@ __device__ inline double edot(int q, double v0, double v1, double v2) {
   return s;
 }
 
+// The reference's cx*v0 + cy*v1 + cz*v2 with every product, zeros included:
+// differs from edot only where a component is not finite (0 * Inf = NaN).
+template <int D>
+__device__ inline double edot_full(int q, double v0, double v1, double v2) {
+#pragma clang fp contract(off)
+  using L = Lat<D>;
+  return (double)L::ex(q) * v0 + (double)L::ey(q) * v1 + (double)L::ez(q) * v2;
+}
+
+// First moments sum_q e_q f_q with every product, in q order, as
+// compute_macroscopic sums them (couplb_lattice.h:167-173): used where a
+// population is not finite, so that 0 * Inf = NaN reaches the same
+// components as in the reference.
+template <int D, typename V>
+__device__ inline void moments_full(const V* f, double& jx, double& jy, double& jz) {
+#pragma clang fp contract(off)
+  using L = Lat<D>;
+  jx = jy = jz = 0.0;
+#pragma unroll
+  for (int q = 0; q < L::Q; q++) {
+    const double fv = (double)f[q];
+    jx += (double)L::ex(q) * fv;
+    jy += (double)L::ey(q) * fv;
+    jz += (double)L::ez(q) * fv;
+  }
+}
+
 // e_q . v for the reduced-precision collision: the nonzero terms only, so no
 // multiplications by 0 or +-1 are issued (they cannot be dropped by the
 // compiler under IEEE rules)
@@ -322,16 +351,24 @@ __device__ inline void collide_exact(const double (&f)[Lat<D>::Q], double r, dou
   const double uu = u0 * u0 + u1 * u1 + u2 * u2;
   const double uf = u0 * F0 + u1 * F1 + u2 * F2;
   const double uu_term = div_cs2(0.5 * uu);
+  auto body = [&](auto full) {
 #pragma unroll
-  for (int q = 0; q < Q; q++) {
-    const double w = L::w(q);
-    const double eu = edot<D>(q, u0, u1, u2);
-    const double ef = edot<D>(q, F0, F1, F2);
-    const double feq = w * r * (1.0 + div_cs2(eu) + div_cs4(0.5 * eu * eu) - uu_term);
-    const double Si = w * half_om * (div_cs2(ef - uf) + div_cs4(eu * ef));
-    const double fn = f[q] + (-om * (f[q] - feq) + Si);
-    Store<double>::store(dst + q * pq + at0, fn);
-  }
+    for (int q = 0; q < Q; q++) {
+      const double w = L::w(q);
+      const double eu = full ? edot_full<D>(q, u0, u1, u2) : edot<D>(q, u0, u1, u2);
+      const double ef = full ? edot_full<D>(q, F0, F1, F2) : edot<D>(q, F0, F1, F2);
+      const double feq = w * r * (1.0 + div_cs2(eu) + div_cs4(0.5 * eu * eu) - uu_term);
+      const double Si = w * half_om * (div_cs2(ef - uf) + div_cs4(eu * ef));
+      const double fn = f[q] + (-om * (f[q] - feq) + Si);
+      Store<double>::store(dst + q * pq + at0, fn);
+    }
+  };
+  // a non-finite velocity or force: the products with zero components
+  // matter (0 * Inf = NaN), so every one is formed as in the reference
+  if (isfinite(u0) && isfinite(u1) && isfinite(u2) && isfinite(F0) && isfinite(F1) && isfinite(F2))
+    body(std::false_type{});
+  else
+    body(std::true_type{});
 }
 
 struct StepArgs {
@@ -441,6 +478,11 @@ __device__ inline void step_node(const StepArgs& a, const Layout& lay, S* dst, c
     if (L::ey(q) == 1) jy += f[q]; else if (L::ey(q) == -1) jy -= f[q];
     if (D == 3) { if (L::ez(q) == 1) jz += f[q]; else if (L::ez(q) == -1) jz -= f[q]; }
   }
+  if constexpr (!Store<S>::deviation) {
+    // a non-finite population: the moments with every product, as the
+    // reference forms them (the sums above skip the zero components)
+    if (!isfinite(A)) moments_full<D>(f, jx, jy, jz);
+  }
   T As = A;  // A in the stored scale
   if constexpr (Store<S>::deviation) {
     const T is = (T)a.pc.inv_scale;
@@ -451,7 +493,9 @@ __device__ inline void step_node(const StepArgs& a, const Layout& lay, S* dst, c
   }
   const T base = (T)a.pc.base;
   T rho = A + base;
-  if (!(rho > (T)RHO_MIN)) {  // also catches NaN (couplb_lattice.h:178-186)
+  // couplb_lattice.h:178-186: rho <= RHO_MIN, then any non-finite value
+  // (NaN, +Inf) is clamped to RHO_MIN
+  if (rho <= (T)RHO_MIN || !isfinite(rho)) {
     rho = (T)RHO_MIN;
     A = rho - base;
     As = A * (T)a.pc.scale;
@@ -740,6 +784,7 @@ __global__ void __launch_bounds__(256) k_macro(MacroArgs a) {
       if (L::ey(q) > 0) jy += fv; else if (L::ey(q) < 0) jy -= fv;
       if (L::ez(q) > 0) jz += fv; else if (L::ez(q) < 0) jz -= fv;
     }
+    if (!isfinite(A)) moments_full<D>(f, jx, jy, jz);  // 0 * Inf = NaN, as the reference
     double r = A + a.pc.base;
     int cl = 0;
     if (r <= RHO_MIN) { r = RHO_MIN; cl = 1; }

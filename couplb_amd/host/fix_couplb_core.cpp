@@ -221,8 +221,10 @@ void FixCoupLBCore::init() {
   static const char* axis[3] = {"x", "y", "z"};
   for (int c = 0; c < 3; c++) {
     if (gstyle[c] != EQUAL_G) continue;
-    double v = 0;
-    if (!var_fn || var_fn(var_user, gvar[c].c_str(), &v) != 0)
+    const int rc = var_fn ? var_fn(var_user, gvar[c].c_str(), nullptr) : COUPLB_VAR_MISSING;
+    if (rc == COUPLB_VAR_NOT_EQUAL)
+      throw fix_error("Variable " + gvar[c] + " for gravity " + axis[c] + " in fix couplb must be equal-style");
+    if (rc != 0)
       throw fix_error("Variable " + gvar[c] + " for gravity " + axis[c] + " in fix couplb does not exist");
   }
   const double dx_x = (dom.boxhi[0] - dom.boxlo[0]) / Nx;
@@ -764,6 +766,7 @@ double FixCoupLBCore::scalar(const std::string& n) const {
   if (n == "force_scale") return force_scale;
   if (n == "lbm_steps") return (double)lbm_step_count;
   if (n == "md_sub_count") return md_sub_count;
+  if (n == "md_per_lb") return md_per_lb;
   if (n == "lbm_ms") return couplb_last_kernel_ms(ctx, 0);
   if (n == "ibm_ms") return couplb_last_kernel_ms(ctx, 1);
   return std::nan("");

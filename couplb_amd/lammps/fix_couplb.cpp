@@ -69,15 +69,22 @@ void FixCoupLB::check(int rc)
   if (rc != COUPLB_OK) error->all(FLERR, couplb_fix_last_error(core));
 }
 
-// input->variable lookup for `gravity v_name` (fix_couplb.cpp:263-280, 671-678)
+// input->variable lookup for `gravity v_name` (fix_couplb.cpp:263-280, 671-678).
+// The variables are evaluated again at the next LB step, md_per_lb MD steps
+// later, so the computes they use are scheduled for that step
+// (addstep_compute(ntimestep + md_per_lb), fix_couplb.cpp:677); repeating
+// clearstep / addstep per variable schedules the same step.
 int FixCoupLB::eval_variable(void *self, const char *name, double *value)
 {
   auto *fx = static_cast<FixCoupLB *>(self);
   const int ivar = fx->input->variable->find(name);
-  if (ivar < 0 || !fx->input->variable->equalstyle(ivar)) return 1;
+  if (ivar < 0) return COUPLB_VAR_MISSING;
+  if (!fx->input->variable->equalstyle(ivar)) return COUPLB_VAR_NOT_EQUAL;
+  if (!value) return 0;  // init: existence check only
   fx->modify->clearstep_compute();
   *value = fx->input->variable->compute_equal(ivar);
-  fx->modify->addstep_compute(fx->update->ntimestep + 1);
+  const long md_per_lb = (long) couplb_fix_scalar(fx->core, "md_per_lb");
+  fx->modify->addstep_compute(fx->update->ntimestep + md_per_lb);
   return 0;
 }
 

@@ -60,9 +60,15 @@ typedef struct couplb_atom_extras {
   const double* torque;   /* &atom->torque[0][0], or NULL */
 } couplb_atom_extras;
 
-/* Equal-style variable evaluation for `gravity v_name ...`
- * (input->variable->compute_equal, fix_couplb.cpp:671-678).  Return 0 on
- * success; a nonzero return is the reference's "does not exist" error. */
+/* Equal-style variable lookup and evaluation for `gravity v_name ...`.
+ * value == NULL: existence check only, as FixCoupLB::init does with
+ * input->variable->find / equalstyle (fix_couplb.cpp:263-280), nothing is
+ * evaluated.  Otherwise evaluate into *value (compute_equal at every LB
+ * step, fix_couplb.cpp:671-678).  Return 0 on success,
+ * COUPLB_VAR_MISSING for the reference's "does not exist" error,
+ * COUPLB_VAR_NOT_EQUAL for its "must be equal-style" error. */
+#define COUPLB_VAR_MISSING 1
+#define COUPLB_VAR_NOT_EQUAL 2
 typedef int (*couplb_var_eval)(void* user, const char* name, double* value);
 
 /* arg = {"ID", "group", "couplb", "Nx", "Ny", "Nz", "nu", "rho", keywords...} */

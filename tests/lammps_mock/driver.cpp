@@ -6,6 +6,7 @@
 // driver ATOMS OUT STEPS DT fix-args...
 //   ATOMS: int32 n, then x[3n], v[3n], m[n] (float64); the atoms are fixed
 //   OUT:   the per-atom forces [STEPS][n][3] (float64)
+// Prints the steps the adapter scheduled computes for (addstep_compute).
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -60,6 +61,7 @@ int main(int argc, char **argv)
   Modify modify;
   Update update;
   update.dt = std::atof(argv[4]);
+  variable.ntimestep = &update.ntimestep;
   LAMMPS lmp;
   lmp.atom = &atom;
   lmp.comm = &comm;
@@ -91,5 +93,6 @@ int main(int argc, char **argv)
   if (!o || std::fwrite(out.data(), sizeof(double), out.size(), o) != out.size()) return 6;
   std::fclose(o);
   std::printf("driver: %d atoms, %d steps\n", n, steps);
+  for (long a : modify.addsteps) std::printf("addstep_compute %ld\n", a);
   return 0;
 }

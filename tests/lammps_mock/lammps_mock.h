@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #define FLERR __FILE__, __LINE__
 
@@ -47,11 +48,13 @@ class Error {
   }
 };
 
+// one equal-style variable, gx = 1e-5 * (1 + 0.01 * step)
 class Variable {
  public:
-  int find(const char *) { return -1; }
-  int equalstyle(int) { return 0; }
-  double compute_equal(int) { return 0.0; }
+  const long *ntimestep = nullptr;
+  int find(const char *name) { return std::string(name) == "gx" ? 0 : -1; }
+  int equalstyle(int ivar) { return ivar == 0; }
+  double compute_equal(int) { return 1e-5 * (1.0 + 0.01 * (ntimestep ? (double) *ntimestep : 0.0)); }
 };
 
 class Input {
@@ -59,10 +62,12 @@ class Input {
   Variable *variable = nullptr;
 };
 
+// records the steps the computes are scheduled for
 class Modify {
  public:
+  std::vector<long> addsteps;
   void clearstep_compute() {}
-  void addstep_compute(long) {}
+  void addstep_compute(long step) { addsteps.push_back(step); }
 };
 
 class Update {

@@ -166,3 +166,19 @@ class DeviceArray:
     def __del__(self):
         if self._hip is not None and self.ptr:
             self._hip.hipFree(self.ptr)
+
+
+def bad_density_state(n, seed=3):
+    """Near-equilibrium populations with five broken nodes: all NaN, density
+    -1, density 0 (next to the moving wall), one +Inf population and one
+    -Inf population."""
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = n
+    e, w = oracle.lattice(3)
+    f = w[:, None, None, None] * (1.0 + 0.01 * rng.standard_normal((19, nz + 2, ny + 2, nx + 2)))
+    f[:, 3, 3, 2] = np.nan
+    f[:, 2, 4, 4] = -w
+    f[:, 4, ny, 6] = 0.0
+    f[1, 3, 2, 7] = np.inf
+    f[3, 5, 5, 3] = -np.inf
+    return f

@@ -4,7 +4,7 @@ the IBM tile limit, and misuse of the multi-direct-forcing call."""
 import numpy as np
 import pytest
 
-from tests.scenarios import DeviceArray
+from tests.scenarios import DeviceArray, bad_density_state
 
 pytestmark = pytest.mark.gpu
 
@@ -75,3 +75,38 @@ def test_mdf_must_be_the_only_ibm_call_of_an_interval():
     st = d.mdf_device(2, x.addr, V.addr, None, fb.addr, (0, 0, 0), 1.0, 1.0, 1.0, tol=1e-8,
                       positions_unchanged=True)
     assert st["converged"]
+
+
+@pytest.mark.parametrize("walls", [((0, 0), (1, 2), (0, 0)), ((0, 0), (0, 0), (0, 0))])
+def test_density_clamp_matches_the_oracle(walls):
+    # couplb_lattice.h:178-186: rho <= 1e-10 and non-finite rho (NaN, +Inf)
+    # are clamped to 1e-10 before the collision; the count comes from the
+    # next compute_macroscopic(false) (fix_couplb.cpp:838)
+    import couplb_amd
+    import oracle
+    from tests.scenarios import interior
+    n = (9, 7, 6)
+    face = [[[0.0] * 3, [0.0] * 3] for _ in range(3)]
+    face[1][1] = [0.02, 0.0, -0.01]
+    f0 = bad_density_state(n)
+    print("case density clamp", walls, flush=True)
+    g = oracle.Grid(3, n, tau=0.7, walls=walls, face_vel=face)
+    g.field(oracle.F_F)[:] = f0.reshape(-1)
+    d = couplb_amd.Device(3, *n, tau=0.7, walls=walls, face_vel=face)
+    d.set_state(f0)
+    o = g.ops
+    for step in range(2):
+        o.compute_macroscopic(g.g, 1)
+        o.collide(g.g)
+        o.exchange(g.g)
+        o.stream(g.g)
+        d.step((0.0, 0.0, 0.0), 0)
+        fo = interior(g.field(oracle.F_F).reshape(19, n[2] + 2, n[1] + 2, n[0] + 2), 3)
+        fd = interior(d.get_state(), 3)
+        assert np.isnan(fo).any() and np.isinf(np.abs(fo)).sum() + np.isnan(fo).sum() < fo.size
+        # FP64 steps without markers equal the reference bit for bit,
+        # non-finite values included
+        assert np.array_equal(fd, fo, equal_nan=True), (step, np.argwhere(~((fd == fo) | (np.isnan(fd) & np.isnan(fo))))[:5])
+        clamps_ref = o.compute_macroscopic(g.g, 0)
+        assert clamps_ref >= 5
+        assert d.diagnostics()["clamps"] == clamps_ref

@@ -205,3 +205,31 @@ def test_device_reuse_matches_full_recompute(storage):
     assert np.all(np.isfinite(f1)) and np.abs(a1).max() > 0
     assert rel_l2(f1, f2) <= REUSE_TOL[storage]
     assert rel_l2(a1, a2) <= REUSE_TOL[storage]
+
+
+def test_group_growing_within_an_interval_matches_oracle():
+    """The group gains atoms between the sub-steps of one LB interval
+    (md_per_lb 2): the device grows its marker buffers while the earlier
+    sub-step's records are still pending, and must spread them all, as the
+    reference spreads every sub-step as it happens (fix_couplb.cpp:741-792)."""
+    rng = np.random.default_rng(11)
+    n_all = 240
+    base = atoms_from(rng.uniform(0.0, 12.0, (n_all, 3)), rng.normal(0.0, 1e-3, (n_all, 3)),
+                      rng.uniform(0.5, 1.5, n_all))
+    sizes = {0: 50, 1: 50, 2: 51, 3: 51, 4: 110, 5: 60, 6: 240, 7: 240, 8: 30}
+
+    def atoms(step):
+        a = moving(base)(step)
+        a["mask"] = (np.arange(n_all) < sizes[step]).astype(np.int32)
+        return a
+
+    cmd = "grow all couplb 12 12 12 0.1 1.0 md_per_lb 2 kernel peskin4 wall_y 1 2 wall_vel 0.01 0.0 0.0"
+    print("case group growing within an interval", flush=True)
+    dev, fd = run_device(cmd, 8, dimension=3, dt=0.5, atoms_fn=atoms)
+    ref, fo = run_oracle(cmd, 8, dimension=3, dt=0.5, atoms_fn=atoms)
+    a = interior(dev.device_state().reshape(19, 14, 14, 14), 3)
+    b = interior(ref.f(), 3)
+    err = rel_l2(a, b)
+    ferr = max(rel_l2(p, q) for p, q in zip(fd, fo))
+    print(f"rel-L2(f) = {err:.3e}, rel-L2(F_atom) = {ferr:.3e}")
+    assert err <= TOL and ferr <= 1e-11

@@ -15,8 +15,7 @@ from tests.scenarios import atoms_from
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 DRIVER = os.path.join(HERE, "lammps_mock", "driver")
-ARGS = ("flow all couplb 12 10 8 0.1 1.0 wall_y 1 1 gravity 1e-5 0.0 0.0 kernel peskin4 "
-        "md_per_lb 2").split()
+ARGS = "flow all couplb 12 10 8 0.1 1.0 wall_y 1 1 kernel peskin4 md_per_lb 2".split()
 DT, STEPS = 0.5, 8
 
 
@@ -35,8 +34,18 @@ def test_adapter_compiles_against_lammps_interface():
                     os.path.join(ROOT, "couplb_amd", "lammps", "fix_couplb.cpp")], check=True)
 
 
+GRAVITY = {
+    "constant": dict(args=["gravity", "1e-5", "0.0", "0.0"]),
+    # the stand-in's equal-style variable gx = 1e-5 (1 + 0.01 step)
+    "variable": dict(args=["gravity", "v_gx", "0.0", "0.0"],
+                     variables=dict(gx=lambda step: 1e-5 * (1.0 + 0.01 * step))),
+}
+
+
 @pytest.mark.gpu
-def test_adapter_forces_equal_fix_core(tmp_path):
+@pytest.mark.parametrize("gravity", sorted(GRAVITY))
+def test_adapter_forces_equal_fix_core(tmp_path, gravity):
+    args = ARGS + GRAVITY[gravity].get("args", [])
     assert os.path.exists(DRIVER), "build the adapter driver first (__graft_entry__.build())"
     rng = np.random.default_rng(4)
     n = 24
@@ -49,14 +58,14 @@ def test_adapter_forces_equal_fix_core(tmp_path):
         for arr in (x, v, m):
             np.ascontiguousarray(arr, np.float64).tofile(fh)
     out = tmp_path / "forces.bin"
-    print(f"case adapter: 12x10x8, {n} atoms, {STEPS} MD steps", flush=True)
-    r = subprocess.run([DRIVER, str(atoms), str(out), str(STEPS), str(DT)] + ARGS,
+    print(f"case adapter: 12x10x8, {n} atoms, {STEPS} MD steps, {gravity} gravity", flush=True)
+    r = subprocess.run([DRIVER, str(atoms), str(out), str(STEPS), str(DT)] + args,
                        capture_output=True, text=True, timeout=120)
     print(r.stdout, r.stderr, flush=True)
     assert r.returncode == 0
     got = np.fromfile(out, np.float64).reshape(STEPS, n, 3)
 
-    fx = couplb_amd.Fix(" ".join(ARGS), dt=DT)
+    fx = couplb_amd.Fix(" ".join(args), dt=DT, variables=GRAVITY[gravity].get("variables"))
     fx.init()
     a = atoms_from(x, v, m)
     fx.setup(a)
@@ -71,3 +80,8 @@ def test_adapter_forces_equal_fix_core(tmp_path):
     want = np.stack(want)
     assert np.abs(want).max() > 0
     assert np.array_equal(got, want)
+    # a gravity variable is evaluated at every LB step (MD steps 1, 3, 5, 7
+    # with md_per_lb 2) and its computes are scheduled for the next one,
+    # md_per_lb MD steps later (fix_couplb.cpp:677)
+    sched = [int(l.split()[1]) for l in r.stdout.splitlines() if l.startswith("addstep_compute")]
+    assert sched == ([s + 2 for s in range(1, STEPS + 1, 2)] if gravity == "variable" else [])

@@ -68,6 +68,34 @@ def test_oracle_matches_reference_headers_live(walls):
     assert all(np.array_equal(p, q) for p, q in zip(fa, fb))
 
 
+@pytest.mark.skipif(not oracle.ref_available(), reason="oracle/_ref not built")
+def test_density_clamp_and_non_finite_values_match_reference_headers():
+    # couplb_lattice.h:178-186 clamps rho <= 1e-10 and non-finite rho; the
+    # moment sums keep every product, so 0 * Inf = NaN spreads as it does
+    # in the reference
+    from tests.scenarios import bad_density_state
+    n = (9, 7, 6)
+    walls = ((0, 0), (1, 2), (0, 0))
+    face = [[[0.0] * 3, [0.0] * 3] for _ in range(3)]
+    face[1][1] = [0.02, 0.0, -0.01]
+    f0 = bad_density_state(n)
+    grids = [oracle.Grid(3, n, tau=0.7, walls=walls, face_vel=face, numerics=k) for k in ("oracle", "ref")]
+    for g in grids:
+        g.field(oracle.F_F)[:] = f0.reshape(-1)
+    for step in range(2):
+        clamps = []
+        for g in grids:
+            o = g.ops
+            clamps.append(o.compute_macroscopic(g.g, 1))
+            o.collide(g.g)
+            o.exchange(g.g)
+            o.stream(g.g)
+        a, b = (g.field(oracle.F_F) for g in grids)
+        assert np.array_equal(a, b, equal_nan=True)
+        assert clamps[0] == clamps[1] >= 5
+        assert np.isnan(a).any() and np.isfinite(a).any()
+
+
 # -------------------------------------------------------------------------
 # 2. mathematics
 # -------------------------------------------------------------------------

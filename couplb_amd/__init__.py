@@ -121,6 +121,7 @@ def lib() -> C.CDLL:
         "couplb_get_macro": (ip, [vp, dp, dp, dp, dp]),
         "couplb_get_node_forces": (ip, [vp, dp, ip]),
         "couplb_diagnostics": (ip, [vp, dp]),
+        "couplb_diagnostics_noalloc": (ip, [vp, dp]),
         "couplb_profile_y": (ip, [vp, dp, dp, dp]),
         "couplb_synchronize": (ip, [vp]),
         "couplb_stream": (vp, [vp]),

@@ -1,6 +1,7 @@
 // couplb_hip.hip -- device context and C ABI (include/couplb_hip.h).
 #include <cstring>
 #include <algorithm>
+#include <mutex>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -24,7 +25,37 @@
 
 using namespace couplb;
 
+// Device memory of couplb_diagnostics_noalloc: MACRO_PARTS block partials of
+// k_macro_parts (9 doubles each) and the six results of k_macro_reduce,
+// 18,496 bytes on every GPU the library runs on, held until the process
+// ends.  Host code reaches it only through the address hipGetSymbolAddress
+// returns for the current device (scratch_on); the name itself is not a
+// device address on the host.
+static __device__ __align__(256) double scratch[MACRO_PARTS * 9 + 8];
+constexpr size_t SCRATCH_OUT = (size_t)MACRO_PARTS * 9;  // offset of the results
+
 namespace {
+
+// Address of `scratch` on each device, looked up once per device with that
+// device current (couplb_create).
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mu;
+double* scratch_dev[MAX_DEVICES] = {};
+
+// the current device's scratch address, or null if the lookup failed
+double* scratch_on(int dev) {
+  if (dev < 0 || dev >= MAX_DEVICES) return nullptr;
+  std::lock_guard<std::mutex> lk(scratch_mu);
+  if (!scratch_dev[dev]) {
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+    scratch_dev[dev] = static_cast<double*>(ptr);
+  }
+  return scratch_dev[dev];
+}
 
 // Kernel timing without host synchronisation: each timed region records a
 // pair of events into a ring; a pair is read back (hipEventSynchronize) only
@@ -146,6 +177,7 @@ struct couplb_ctx {
   int slot_table_n = -1;       // descriptors currently uploaded
   double* staging = nullptr;  // ghost-inclusive FP64 state for import/export
   size_t staging_len = 0;
+  double* scratch = nullptr;  // this device's `scratch` (couplb_diagnostics_noalloc)
 };
 
 namespace {
@@ -280,6 +312,12 @@ template <int D, typename S> struct LaunchMacro {
     const dim3 b = row_block(c->geo.nx);
     hipLaunchKernelGGL((k_macro<D, S>), dim3(1, c->geo.ny, c->D == 3 ? c->geo.nz : 1), b, 0,
                        c->stream, a);
+  }
+};
+template <int D, typename S> struct LaunchMacroParts {
+  static void run(couplb_ctx* c, const MacroArgs& a) {
+    const dim3 b = row_block(c->geo.nx);  // k_macro's block: the same row sums
+    hipLaunchKernelGGL((k_macro_parts<D, S>), dim3(MACRO_PARTS), b, 0, c->stream, a);
   }
 };
 template <int D, typename S> struct LaunchImport {
@@ -1285,6 +1323,8 @@ int couplb_create(const couplb_params* p, couplb_ctx** out) {
   if (hipSetDevice(dev) != hipSuccess) return bad("hipSetDevice failed");
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
     return bad("hipStreamCreate failed");
+  c->scratch = scratch_on(dev);
+  if (!c->scratch) return bad("hipGetSymbolAddress of the diagnostics scratch failed");
   for (int e = 0; e < TimerRing::R; e++)
     if (hipEventCreate(&c->tr.a[e]) != hipSuccess || hipEventCreate(&c->tr.b[e]) != hipSuccess)
       return bad("hipEventCreate failed");
@@ -1567,6 +1607,30 @@ int couplb_diagnostics(couplb_ctx* c, double out[6]) {
   double h[6];
   HIPCHK(c, hipMemcpyAsync(h, c->red, sizeof(double) * 6, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  out[0] = h[0]; out[1] = h[1]; out[2] = h[2]; out[3] = h[3];
+  out[4] = std::sqrt(h[4]);
+  out[5] = h[5];
+  return COUPLB_OK;
+}
+
+int couplb_diagnostics_noalloc(couplb_ctx* c, double out[6]) {
+  if (!c || !out) return fail(c, COUPLB_EINVAL, "null argument");
+  if (!c->scratch) return fail(c, COUPLB_ESTATE, "no diagnostics scratch on this device");
+  // every partial is written by its block: nothing to clear first
+  MacroArgs a;
+  a.pc = make_pull(c);
+  a.rho = a.ux = a.uy = a.uz = nullptr;
+  a.partial = c->scratch;
+  dispatch<LaunchMacroParts>(c, a);
+  HIPCHK(c, hipGetLastError());
+  hipLaunchKernelGGL(k_macro_reduce, dim3(1), dim3(256), 0, c->stream, (const double*)c->scratch,
+                     MACRO_PARTS, 1, c->scratch + SCRATCH_OUT, (double*)nullptr);
+  HIPCHK(c, hipGetLastError());
+  double h[6];
+  HIPCHK(c, hipMemcpyAsync(h, c->scratch + SCRATCH_OUT, sizeof(double) * 6, hipMemcpyDeviceToHost,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->rho_fresh = 1;  // compute_macroscopic(false), as couplb_diagnostics
   out[0] = h[0]; out[1] = h[1]; out[2] = h[2]; out[3] = h[3];
   out[4] = std::sqrt(h[4]);
   out[5] = h[5];

@@ -763,14 +763,14 @@ struct MacroArgs {
   double* partial;  // [ny*nz][9]
 };
 
+// The nine sums of row (j, k), left in red[c][0] for every thread of the
+// block (all threads call it; it ends with a barrier).
 template <int D, typename S>
-__global__ void __launch_bounds__(256) k_macro(MacroArgs a) {
+__device__ inline void macro_row(const MacroArgs& a, int j, int k, double (*red)[256]) {
   using L = Lat<D>;
   using T = typename Store<S>::T;
   constexpr int Q = L::Q;
   const Geo& g = a.pc.g;
-  const int j = blockIdx.y, k = blockIdx.z;
-  __shared__ double red[9][256];
   double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (int i = threadIdx.x; i < g.nx; i += blockDim.x) {
     T f[Q];
@@ -815,12 +815,41 @@ __global__ void __launch_bounds__(256) k_macro(MacroArgs a) {
     }
     __syncthreads();
   }
-  if (threadIdx.x < 9) {
-    a.partial[((size_t)k * g.ny + j) * 9 + threadIdx.x] = red[threadIdx.x][0];
-  }
 }
 
-// Deterministic second stage: out[0..5] global sums, prof[3*ny] y-profile.
+template <int D, typename S>
+__global__ void __launch_bounds__(256) k_macro(MacroArgs a) {
+  __shared__ double red[9][256];
+  const int j = blockIdx.y, k = blockIdx.z;
+  macro_row<D, S>(a, j, k, red);
+  if (threadIdx.x < 9) a.partial[((size_t)k * a.pc.g.ny + j) * 9 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// The same sums with a fixed number of partials, MACRO_PARTS, whatever the
+// grid size: block t sums the row partials of rows t, t + MACRO_PARTS, ...
+// in increasing order, exactly as thread t of k_macro_reduce sums k_macro's
+// row partials, so k_macro_reduce over these partials (ny = MACRO_PARTS,
+// nz = 1) returns the bits of k_macro + k_macro_reduce.  Every block writes
+// its partial (zeros when it has no rows): nothing has to be cleared first.
+constexpr int MACRO_PARTS = 256;  // k_macro_reduce's block size
+template <int D, typename S>
+__global__ void __launch_bounds__(256) k_macro_parts(MacroArgs a) {
+  __shared__ double red[9][256];
+  const Geo& g = a.pc.g;
+  const int nrow = g.ny * (D == 3 ? g.nz : 1);
+  double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int r = blockIdx.x; r < nrow; r += MACRO_PARTS) {
+    macro_row<D, S>(a, r % g.ny, r / g.ny, red);
+    if (threadIdx.x == 0)
+      for (int c = 0; c < 9; c++) acc[c] = (c == 4) ? fmax(acc[c], red[c][0]) : acc[c] + red[c][0];
+    __syncthreads();  // red is rewritten by the next row
+  }
+  if (threadIdx.x == 0)
+    for (int c = 0; c < 9; c++) a.partial[(size_t)blockIdx.x * 9 + c] = acc[c];
+}
+
+// Deterministic second stage: out[0..5] global sums, prof[3*ny] y-profile
+// (prof may be null).
 // One block; fixed summation order.
 __global__ void __launch_bounds__(256) k_macro_reduce(const double* partial, int ny, int nz,
                                                       double* out, double* prof) {
@@ -845,6 +874,7 @@ __global__ void __launch_bounds__(256) k_macro_reduce(const double* partial, int
     __syncthreads();
   }
   if (threadIdx.x < 6) out[threadIdx.x] = red[threadIdx.x][0];
+  if (!prof) return;
   for (int j = threadIdx.x; j < ny; j += blockDim.x) {
     double sr = 0, su = 0, sv = 0;
     for (int k = 0; k < nz; k++) {

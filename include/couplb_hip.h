@@ -230,6 +230,17 @@ int couplb_get_node_forces(couplb_ctx* ctx, double* f, int fold);
  * out = {mass, momx, momy, momz, umax, clamp_count}.  Also a refresh. */
 int couplb_diagnostics(couplb_ctx* ctx, double out[6]);
 
+/* couplb_diagnostics without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same out[6], bit for bit.  Its partial sums live in a fixed 18,496-byte
+ * array of device memory that the library holds on every GPU it runs on
+ * (256 block partials whatever the grid size, so no grid is too large); the
+ * array is shared by every context on that GPU and every call rewrites all
+ * of it, so calls that overlap on one GPU (contexts on different streams or
+ * host threads) race.  Also a refresh. */
+int couplb_diagnostics_noalloc(couplb_ctx* ctx, double out[6]);
+
 /* y-binned sums over x and z of rho, ux, uy (write_profile,
  * fix_couplb.cpp:903-928); each array has ny entries.  Also a refresh. */
 int couplb_profile_y(couplb_ctx* ctx, double* rho_sum, double* ux_sum, double* uy_sum);

@@ -92,6 +92,7 @@ struct couplb_ctx {
   void* f[2] = {nullptr, nullptr};
   int cur = 0;           // buffer holding the state
   int local = 1;         // state is post-stream (create / import)
+  int prev_valid = 0;    // a step since the state was set: f[cur ^ 1] holds its input
   double* rho = nullptr; // collision density (interior layout)
   int rho_fresh = 0;     // density refreshed since the last step (P1)
   int has_moving = 0;
@@ -335,11 +336,11 @@ template <int D, typename S> struct LaunchInitEq {
   }
 };
 template <int D, typename S> struct LaunchExport {
-  static void run(couplb_ctx* c, const PullCtx& pc, double* fout) {
+  static void run(couplb_ctx* c, const PullCtx& pc, const void* prev, double* fout) {
     const int gx = c->geo.nx + 2, gy = c->geo.ny + 2, gz = (D == 3) ? c->geo.nz + 2 : 1;
     const dim3 b = row_block(gx);
     const dim3 grid((unsigned)((gx + b.x - 1) / b.x), (unsigned)gy, (unsigned)gz);
-    hipLaunchKernelGGL((k_export<D, S>), grid, b, 0, c->stream, pc, fout);
+    hipLaunchKernelGGL((k_export<D, S>), grid, b, 0, c->stream, pc, prev, fout);
   }
 };
 template <int D, typename S> struct LaunchUgather {
@@ -1184,6 +1185,7 @@ int fused_steps(couplb_ctx* c, const double g[3], int flags, int n) {
     dispatch<LaunchFused>(c, a, m);
     HIPCHK(c, hipGetLastError());
     c->cur ^= (m & 1);
+    c->prev_valid = 1;
     c->rho_fresh = 0;
     c->cycle += m;
     n -= m;
@@ -1242,6 +1244,7 @@ int step_impl(couplb_ctx* c, const double g[3], int flags, bool timed) {
   if (c->nslots > 0) c->prev_step_epoch = (c->nslots == 1) ? c->slots[0].epoch : -1;
   c->cur = dst;
   c->local = 0;
+  c->prev_valid = 1;
   c->nslots = 0;
   c->rho_fresh = 0;
   c->cycle++;
@@ -1433,6 +1436,7 @@ int couplb_set_state_f(couplb_ctx* c, const double* fh) {
   dispatch<LaunchImport>(c, (const double*)c->staging, c->f[0]);
   HIPCHK(c, hipGetLastError());
   c->local = 1;
+  c->prev_valid = 0;
   c->nslots = 0;
   c->cycle++;
   // read_checkpoint ends with compute_macroscopic(false) (couplb_io.h:817):
@@ -1456,6 +1460,7 @@ int couplb_init_equilibrium(couplb_ctx* c, double rho, double ux, double uy, dou
                      c->N, rho);
   HIPCHK(c, hipGetLastError());
   c->local = 1;
+  c->prev_valid = 0;
   c->nslots = 0;
   c->cycle++;
   c->rho_fresh = 0;  // like Grid::init_equilibrium: the array holds rho0
@@ -1468,7 +1473,10 @@ int couplb_get_state_f(couplb_ctx* c, double* fh) {
   int rc = ensure_staging(c);
   if (rc) return rc;
   PullCtx pc = make_pull(c);
-  dispatch<LaunchExport>(c, pc, c->staging);
+  // the other buffer holds the post-collision state of the step before the
+  // last one (or, one step after the state was set, that state itself)
+  const void* prev = c->prev_valid ? c->f[c->cur ^ 1] : nullptr;
+  dispatch<LaunchExport>(c, pc, prev, c->staging);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipMemcpyAsync(fh, c->staging, sizeof(double) * couplb_state_len(c),
                            hipMemcpyDeviceToHost, c->stream));

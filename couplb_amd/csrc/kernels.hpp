@@ -644,12 +644,20 @@ __global__ void __launch_bounds__(FUSED_THREADS) k_steps_fused(StepArgs a, int n
     }
   }
   __syncthreads();
+  // the last state, and the one before it in the other buffer, where the
+  // same steps launched one by one would leave them (the state before the
+  // last is what a checkpoint's ghost planes are made of)
   const S* fin = (nsteps & 1) ? B : A;
+  const S* before = (nsteps & 1) ? A : B;
   S* gdst = static_cast<S*>((nsteps & 1) ? a.dst : const_cast<void*>(a.pc.src));
+  S* gother = static_cast<S*>((nsteps & 1) ? const_cast<void*>(a.pc.src) : a.dst);
   for (unsigned node = threadIdx.x; node < N; node += blockDim.x) {
     const unsigned row = node / nx, i = node - row * nx;
 #pragma unroll
-    for (int q = 0; q < Q; q++) gdst[a.pc.lay.at(q, row, i)] = fin[ll.at(q, row, i)];
+    for (int q = 0; q < Q; q++) {
+      gdst[a.pc.lay.at(q, row, i)] = fin[ll.at(q, row, i)];
+      gother[a.pc.lay.at(q, row, i)] = before[ll.at(q, row, i)];
+    }
     a.rho_out[node] = R[node];
   }
 }
@@ -936,8 +944,19 @@ __global__ void k_fill(double* p, size_t n, double v) {
   if (i < n) p[i] = v;
 }
 
+// The reference's f between steps, ghost planes included: interior nodes
+// hold the post-stream populations; a ghost node of a periodic face holds
+// what Streaming::exchange copied into it before the last stream
+// (couplb_streaming.h:136-182, over ghost-inclusive extents in x, y, z
+// order) and the swap (couplb_streaming.h:133) moved into f: the
+// post-collision populations of the step before the last one at its
+// periodic image.  prev is the buffer holding them (the other buffer of the
+// pull scheme), or null when there is no such step since the state was set
+// (then the images of the current state).  Ghost nodes whose image lies on
+// a wall face are never written by the reference after init_equilibrium:
+// w_q * rho0.
 template <int D, typename S>
-__global__ void k_export(PullCtx pc, double* fout) {
+__global__ void k_export(PullCtx pc, const void* prev, double* fout) {
   using L = Lat<D>;
   using T = typename Store<S>::T;
   constexpr int Q = L::Q;
@@ -948,15 +967,32 @@ __global__ void k_export(PullCtx pc, double* fout) {
   if (a >= gx) return;
   const size_t nt = (size_t)gx * gy * gz;
   const size_t gn = (size_t)a + (size_t)gx * ((size_t)b + (size_t)gy * (size_t)c);
-  int wd, wsd;
-  const int t = node_type(pc.w, g, a, b, (D == 3) ? c : 1, wd, wsd);
-  if (t != 0) {
+  const int cc = (D == 3) ? c : 1;
+  // a ghost coordinate on a periodic face maps to its image; one on a wall
+  // face stays a wall ghost
+  const bool gxw = (a == 0 && pc.w.type[0][0] > 0) || (a == g.nx + 1 && pc.w.type[0][1] > 0);
+  const bool gyw = (b == 0 && pc.w.type[1][0] > 0) || (b == g.ny + 1 && pc.w.type[1][1] > 0);
+  const bool gzw = D == 3 && ((cc == 0 && pc.w.type[2][0] > 0) || (cc == g.nz + 1 && pc.w.type[2][1] > 0));
+  if (gxw || gyw || gzw) {
 #pragma unroll
     for (int q = 0; q < Q; q++) fout[q * nt + gn] = L::w(q) * pc.rho0;
     return;
   }
   const int i = wrap_ghost(a, g.nx), j = wrap_ghost(b, g.ny);
-  const int k = (D == 3) ? wrap_ghost(c, g.nz) : 0;
+  const int k = (D == 3) ? wrap_ghost(cc, g.nz) : 0;
+  const bool ghost = a == 0 || a == g.nx + 1 || b == 0 || b == g.ny + 1 ||
+                     (D == 3 && (cc == 0 || cc == g.nz + 1));
+  if (ghost && prev) {
+    const S* p = static_cast<const S*>(prev);
+    const unsigned row = (unsigned)j + (unsigned)g.ny * (unsigned)k;
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+      T v = Store<S>::load(p + pc.lay.at(q, row, (unsigned)i));
+      if (Store<S>::deviation) v *= (T)pc.inv_scale;
+      fout[q * nt + gn] = (double)v + L::w(q) * pc.base;
+    }
+    return;
+  }
   T f[Q];
   pull_node<D, S>(pc, i, j, k, f);
 #pragma unroll

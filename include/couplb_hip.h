@@ -74,9 +74,15 @@ const char* couplb_version(void);
  * gz = nz+2 (3D) or 1 (2D) -- exactly the array couplb_io.h:653-828 writes.
  * set: replaces read_checkpoint's `f = file; f_buf = f;
  * compute_macroscopic(false)` (couplb_io.h:802-817).  Ghost entries are
- * ignored on input.  get: returns the post-stream populations the reference
- * holds between steps; periodic ghosts carry their images, wall ghosts the
- * rest equilibrium w_q*rho0. */
+ * ignored on input.  get: returns the whole f the reference holds between
+ * steps, which its checkpoint writes (couplb_io.h:706): post-stream
+ * populations inside; on periodic ghost planes the images of the
+ * post-collision populations of the step before the last one (what
+ * Streaming::exchange left there before the swap, couplb_streaming.h:
+ * 133-182), or of the current state when no step has run since it was set;
+ * on wall ghost planes the init_equilibrium value w_q*rho0.  One step after
+ * a couplb_set_state_f the reference's ghosts are the imported file's, which
+ * the device does not keep: it returns the images of the imported state. */
 int couplb_set_state_f(couplb_ctx* ctx, const double* f_host);
 int couplb_get_state_f(couplb_ctx* ctx, double* f_host);
 long couplb_state_len(const couplb_ctx* ctx); /* Q*ntotal */

@@ -69,3 +69,46 @@ def test_checkpoint_file_and_restart_match_oracle(tmp_path):
     assert ferr <= 1e-11
     fr.close()
     fo.close()
+
+
+@pytest.mark.parametrize("with_markers", [False, True])
+def test_checkpoint_file_equals_the_oracle_file_ghosts_included(tmp_path, with_markers):
+    # The reference writes the whole ghost-inclusive f (couplb_io.h:706): its
+    # periodic ghost planes hold the images of the post-collision populations
+    # of the step before the last one (exchange before the stream, then the
+    # swap, couplb_streaming.h:133-182), its wall ghost planes the
+    # init_equilibrium values.  Without markers the whole file is identical;
+    # with them the populations differ by the spreading's summation order.
+    pre = str(tmp_path / "ck")
+    cmd = BASE + f" checkpoint 10 {pre}"
+    print("case checkpoint file vs oracle file, markers", with_markers, flush=True)
+    a_dev = markers() if with_markers else None
+    a_ora = markers() if with_markers else None
+    fd = couplb_amd.Fix(cmd, dt=0.5)
+    fd.init()
+    fd.setup(a_dev)
+    fo = oracle.Fix(dim=3, dt=0.5, **parse_cmd(BASE))
+    fo.setup(a_ora)
+    for step in range(1, 11):
+        for fx, a in ((fd, a_dev), (fo, a_ora)):
+            fx.step = step
+            if a is not None:
+                a["f"][:] = 0.0
+            fx.post_force(a)
+            fx.end_of_step(step)
+    fd.close()
+    ref = str(tmp_path / "oracle.clbk")
+    assert oracle.write_checkpoint(ref, 3, 19, N, N, 10, fo.f())
+    got, want = open(pre + ".0.clbk", "rb").read(), open(ref, "rb").read()
+    hdr = len(want) - 8 * int(np.prod(SHAPE))
+    assert len(got) == len(want) and got[:hdr] == want[:hdr]      # header
+    g = np.frombuffer(got[hdr:], np.float64).reshape(SHAPE)
+    w = np.frombuffer(want[hdr:], np.float64).reshape(SHAPE)
+    ghost = np.ones(SHAPE[1:], bool)
+    ghost[1:-1, 1:-1, 1:-1] = False
+    print("  ghost planes rel-L2", rel_l2(g[:, ghost], w[:, ghost]), "bit-exact file", got == want)
+    if with_markers:
+        assert rel_l2(g, w) <= 1e-12 and rel_l2(g[:, ghost], w[:, ghost]) <= 1e-12
+    else:
+        assert got == want
+    fo.close()

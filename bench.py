@@ -7,7 +7,9 @@ constant body force g_x = 1e-6; one rigid sphere of diameter 24 dx at the box
 centre (1810 Fibonacci-lattice markers, held fixed, v = 0) plus 100000 point
 particles at uniform random positions with Gaussian velocities
 (sigma = 1e-3), all of mass 1; Peskin4 kernel, xi_ibm = 1, md_per_lb = 4.
-Positions are prescribed (held fixed), velocities constant.
+Positions are prescribed and held fixed, velocities constant (`value`);
+`mlups_markers_moving` is the same workload with the particles advanced by
+v * dt_MD at every MD sub-step.
 
 A step = one LB step (the fused pull kernel) + md_per_lb IBM sub-steps, with
 the marker arrays resident in HBM.  value = MLUPS = 256^3 * steps / wall
@@ -628,16 +630,30 @@ def main():
     ibm_wo = ibm_avg - ga_tot / max(ibm_n, 1)
     markers_per_s_wo = n / ((ibm_wo + spread_lb / md) * 1e-3) if n and ibm_wo > 0 else 0.0
 
-    # same workload with the positions declared new at every sub-step: the
-    # markers are re-sorted, the coverage rebuilt and the velocity
-    # re-interpolated each time, as for particles that move
+    # same workload with the point particles moving: every MD sub-step
+    # advances them by v * dt_MD (dt_MD = dt_LB / md_per_lb = 0.25; the
+    # sphere has v = 0), on the context stream, and the sub-step sees new
+    # positions: the markers are re-sorted, the interpolation and spreading
+    # weights recomputed, the reactions of all sub-steps spread before the
+    # next LB step, as for particles that a LAMMPS integrator moves
+    ctx_stream = torch.cuda.ExternalStream(dev.stream())
+    xm = tx.clone()
+
+    def one_step_moving():
+        dev.step(g, couplb_amd.RHO_REFRESHED_NEXT)
+        for _ in range(md):
+            with torch.cuda.stream(ctx_stream):
+                xm.add_(tv, alpha=1.0 / md)
+            dev.ibm_device(n, xm.data_ptr(), tv.data_ptr(), tm.data_ptr(), tf.data_ptr(),
+                           positions_unchanged=False, **ibm_kw)
+
     mv_steps = max(4, args.steps // 4)
     dev.set_timer_mask(1 << 0)
-    one_step(moving=True)
+    one_step_moving()
     dev.sync()
     t2 = time.perf_counter()
     for _ in range(mv_steps):
-        one_step(moving=True)
+        one_step_moving()
     dev.sync()
     mlups_moving = nodes * mv_steps / (time.perf_counter() - t2) / 1e6
 
@@ -704,10 +720,12 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": dtype,
-        "data": "synthetic (seeded markers, uniform-flow initial state)",
+        "data": "synthetic (seeded markers at prescribed fixed positions, uniform-flow initial state)",
         "config": {"workload": f"D3Q19 {N}^3 periodic + IBM: Fibonacci sphere D=24 ({nsphere} markers, "
-                               f"fixed) + {args.particles} point particles, Peskin4, xi=1, "
-                               f"md_per_lb={md}, u0=0.02, g_x=1e-6, nu=0.1",
+                               f"fixed) + {args.particles} point particles held at prescribed fixed "
+                               f"positions with constant Gaussian velocities, Peskin4, xi=1, "
+                               f"md_per_lb={md}, u0=0.02, g_x=1e-6, nu=0.1 (moving particles: "
+                               f"mlups_markers_moving)",
                    "grid": [N, N, N], "markers": n, "md_per_lb": md, "storage": args.storage},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),

@@ -11,8 +11,6 @@
 //   Periodic faces are index wraps; wall faces are coordinate tests; the
 //   reference's ghost planes (couplb_lattice.h:82-104) are never stored.
 #pragma once
-#include <type_traits>
-
 #include "lattice.hpp"
 
 namespace couplb {
@@ -291,15 +289,6 @@ __device__ inline double edot(int q, double v0, double v1, double v2) {
   return s;
 }
 
-// The reference's cx*v0 + cy*v1 + cz*v2 with every product, zeros included:
-// differs from edot only where a component is not finite (0 * Inf = NaN).
-template <int D>
-__device__ inline double edot_full(int q, double v0, double v1, double v2) {
-#pragma clang fp contract(off)
-  using L = Lat<D>;
-  return (double)L::ex(q) * v0 + (double)L::ey(q) * v1 + (double)L::ez(q) * v2;
-}
-
 // First moments sum_q e_q f_q with every product, in q order, as
 // compute_macroscopic sums them (couplb_lattice.h:167-173): used where a
 // population is not finite, so that 0 * Inf = NaN reaches the same
@@ -335,7 +324,11 @@ __device__ inline T esum(int q, T v0, T v1, T v2) {
 
 // BGK + Guo in FP64 with the reference's expression order and rounding
 // (couplb_lattice.h:196-200 velocity, couplb_collision.h:36-58 collision):
-// the result equals the reference's to the last bit.
+// the result equals the reference's to the last bit.  With a non-finite
+// velocity or force every output of the node is non-finite here as in the
+// reference (u.u or u.F is), but may be an infinity where the reference's
+// is NaN: the dot products skip the zero components whose products turn an
+// infinity into NaN there.
 template <int D>
 __device__ inline void collide_exact(const double (&f)[Lat<D>::Q], double r, double su, double sv,
                                      double sw, double F0, double F1, double F2, double om,
@@ -351,24 +344,16 @@ __device__ inline void collide_exact(const double (&f)[Lat<D>::Q], double r, dou
   const double uu = u0 * u0 + u1 * u1 + u2 * u2;
   const double uf = u0 * F0 + u1 * F1 + u2 * F2;
   const double uu_term = div_cs2(0.5 * uu);
-  auto body = [&](auto full) {
 #pragma unroll
-    for (int q = 0; q < Q; q++) {
-      const double w = L::w(q);
-      const double eu = full ? edot_full<D>(q, u0, u1, u2) : edot<D>(q, u0, u1, u2);
-      const double ef = full ? edot_full<D>(q, F0, F1, F2) : edot<D>(q, F0, F1, F2);
-      const double feq = w * r * (1.0 + div_cs2(eu) + div_cs4(0.5 * eu * eu) - uu_term);
-      const double Si = w * half_om * (div_cs2(ef - uf) + div_cs4(eu * ef));
-      const double fn = f[q] + (-om * (f[q] - feq) + Si);
-      Store<double>::store(dst + q * pq + at0, fn);
-    }
-  };
-  // a non-finite velocity or force: the products with zero components
-  // matter (0 * Inf = NaN), so every one is formed as in the reference
-  if (isfinite(u0) && isfinite(u1) && isfinite(u2) && isfinite(F0) && isfinite(F1) && isfinite(F2))
-    body(std::false_type{});
-  else
-    body(std::true_type{});
+  for (int q = 0; q < Q; q++) {
+    const double w = L::w(q);
+    const double eu = edot<D>(q, u0, u1, u2);
+    const double ef = edot<D>(q, F0, F1, F2);
+    const double feq = w * r * (1.0 + div_cs2(eu) + div_cs4(0.5 * eu * eu) - uu_term);
+    const double Si = w * half_om * (div_cs2(ef - uf) + div_cs4(eu * ef));
+    const double fn = f[q] + (-om * (f[q] - feq) + Si);
+    Store<double>::store(dst + q * pq + at0, fn);
+  }
 }
 
 struct StepArgs {

@@ -312,3 +312,43 @@ def test_poiseuille3d_deck(tmp_path):
     rel = np.max(np.abs(ux - exact)) / exact.max()
     print(f"  step {last}: max |u - u_exact| / u_max = {100 * rel:.4f} %", flush=True)
     assert rel < 0.01
+
+
+def test_poiseuille2d_deck(tmp_path):
+    # tests/poiseuille2d/in.poiseuille2d:21-30: D2Q9 16x64, walls in y, g_x =
+    # 5e-6, Roma kernel, an empty group, 20000 steps, profile every 400,
+    # check every 200 (box 16 x 64, z from -0.5 to 0.5, boundary p f p).
+    # The whole profile file equals the oracle fix's, and validate.py's rule
+    # (validate.py:75-111) passes: max |u - u_exact| < 1 % of u_max.
+    out, ref = tmp_path / "couplb_poiseuille.dat", tmp_path / "oracle_poiseuille.dat"
+    deck = "flow noatoms couplb 16 64 1 0.1 1.0 wall_y 1 1 gravity 5e-6 0.0 0.0 kernel roma "
+    box = dict(boxlo=(0.0, 0.0, -0.5), boxhi=(16.0, 64.0, 0.5), periodicity=(1, 0, 1))
+    print("case in.poiseuille2d: D2Q9 16x64, 20000 steps", flush=True)
+    fd = couplb_amd.Fix(deck + f"output 400 {out} check_every 200", dimension=2, dt=1.0, **box)
+    fd.init()
+    fd.setup(None)
+    kw = parse_cmd(deck + "check_every 200")
+    fo = oracle.Fix(dim=2, dt=1.0, boxlo=box["boxlo"], boxhi=box["boxhi"], output_every=400,
+                    output_file=str(ref), **kw)
+    fo.setup(None)
+    for step in range(1, 20001):
+        for fx in (fd, fo):
+            fx.step = step
+            fx.post_force(None)
+            fx.end_of_step(step)
+    a = fd.device_state().reshape(9, 1, 66, 18)[:, :, 1:-1, 1:-1]
+    b = fo.f()[:, :, 1:-1, 1:-1]
+    assert np.array_equal(a, b)
+    fd.close()
+    fo.close()
+    assert out.read_text() == ref.read_text()
+    params, last, data = read_profile(out)
+    assert params["Ny"] == 64 and last == 20000 and len(data) == 64
+    y, ux = data[:, 2], data[:, 4]
+    dx = y[1] - y[0]
+    H = params["Ny"] * dx
+    exact = 5e-6 / (2 * params["nu"]) * (y + 0.5 * dx) * (H - 0.5 * dx - y)
+    rel = np.max(np.abs(ux - exact)) / exact.max()
+    print(f"  step {last}: max |u - u_exact| / u_max = {100 * rel:.4f} % (u_max {exact.max():.4e})",
+          flush=True)
+    assert rel < 0.01

@@ -24,9 +24,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libcouplb_hip.so")
 
 OK, EINVAL, EDEVICE, ENOMEM, ESTATE = 0, -1, -2, -3, -4
-FP64, FP32, FP16, BF16 = 0, 1, 2, 3
+FP64, FP32, FP16, BF16, I16 = 0, 1, 2, 3, 4
 ROMA, PESKIN4 = 0, 1
-STORAGE = {"fp64": FP64, "fp32": FP32, "fp16": FP16, "bf16": BF16}
+STORAGE = {"fp64": FP64, "fp32": FP32, "fp16": FP16, "bf16": BF16, "i16": I16}
 RHO_REFRESHED_NEXT = 1
 
 _dp = C.POINTER(C.c_double)
@@ -39,7 +39,7 @@ class Params(C.Structure):
                 ("tau", C.c_double), ("wall", (C.c_int * 2) * 3),
                 ("face_vel", ((C.c_double * 3) * 2) * 3), ("kernel", C.c_int),
                 ("storage", C.c_int), ("rho0", C.c_double), ("md_per_lb", C.c_int),
-                ("device", C.c_int)]
+                ("device", C.c_int), ("storage_range", C.c_double)]
 
 
 class IbmArgs(C.Structure):
@@ -124,6 +124,7 @@ def lib() -> C.CDLL:
         "couplb_diagnostics_noalloc": (ip, [vp, dp]),
         "couplb_profile_y": (ip, [vp, dp, dp, dp]),
         "couplb_synchronize": (ip, [vp]),
+        "couplb_storage_saturations": (ip, [vp, C.POINTER(C.c_ulonglong)]),
         "couplb_stream": (vp, [vp]),
         "couplb_last_kernel_ms": (C.c_double, [vp, ip]),
         "couplb_kernel_ms_total": (ip, [vp, ip, dp, C.POINTER(C.c_long)]),
@@ -174,7 +175,8 @@ class Device:
     """One grid on one GPU (include/couplb_hip.h)."""
 
     def __init__(self, dim, nx, ny, nz, tau, walls=((0, 0), (0, 0), (0, 0)),
-                 face_vel=None, kernel=ROMA, storage=FP64, rho0=1.0, md_per_lb=1, device=0):
+                 face_vel=None, kernel=ROMA, storage=FP64, rho0=1.0, md_per_lb=1, device=0,
+                 storage_range=0.0):
         L = lib()
         p = Params()
         p.dim, p.nx, p.ny, p.nz, p.tau = dim, nx, ny, nz if dim == 3 else 1, tau
@@ -186,6 +188,7 @@ class Device:
         p.kernel = kernel
         p.storage = STORAGE[storage] if isinstance(storage, str) else storage
         p.rho0, p.md_per_lb, p.device = rho0, md_per_lb, device
+        p.storage_range = storage_range
         h = C.c_void_p()
         rc = L.couplb_create(C.byref(p), C.byref(h))
         if rc != OK:
@@ -301,6 +304,11 @@ class Device:
                                               C.byref(st)), "ibm_mdf_device")
         return dict(iterations=st.iterations, converged=bool(st.converged),
                     slip_first=st.slip_first, slip_last=st.slip_last, slip_zero=st.slip_zero)
+
+    def storage_saturations(self):
+        n = C.c_ulonglong()
+        self._chk(lib().couplb_storage_saturations(self.h, C.byref(n)), "storage_saturations")
+        return n.value
 
     def refresh(self):
         self._chk(lib().couplb_refresh_macro(self.h), "refresh_macro")

@@ -32,6 +32,8 @@ using namespace couplb;
 // returns for the current device (scratch_on); the name itself is not a
 // device address on the host.
 static __device__ __align__(256) double scratch[MACRO_PARTS * 9 + 8];
+// populations the 16-bit fixed-point storage clamped to its range
+__device__ unsigned long long couplb::i16_saturations = 0;
 constexpr size_t SCRATCH_OUT = (size_t)MACRO_PARTS * 9;  // offset of the results
 
 namespace {
@@ -242,6 +244,19 @@ size_t round_up(size_t a, size_t b) { return (a + b - 1) / b * b; }
 
 int storage_bytes(int s) { return s == COUPLB_FP64 ? 8 : s == COUPLB_FP32 ? 4 : 2; }
 
+// scale of the stored deviation: 1024 for FP16 (lifts small deviations out
+// of its subnormal range), for the 16-bit fixed point the largest power of
+// two with 32767 * (1 / scale) >= range; 1 otherwise
+float storage_scale(const couplb_params* p) {
+  if (p->storage == COUPLB_FP16) return 1024.0f;
+  if (p->storage != COUPLB_I16) return 1.0f;
+  const double range = p->storage_range > 0 ? p->storage_range : 1.0 / 32.0;
+  double sc = 1.0;
+  while (32767.0 / (2.0 * sc) >= range && sc < 1e30) sc *= 2.0;
+  while (32767.0 / sc < range && sc > 1e-30) sc *= 0.5;
+  return (float)sc;
+}
+
 dim3 row_block(int nx) {
   const int bx = nx > 128 ? 256 : (nx > 64 ? 128 : 64);
   return dim3(bx, 1, 1);
@@ -281,12 +296,14 @@ void dispatch(couplb_ctx* c, Args&&... args) {
     if (s == COUPLB_FP64) F<3, double>::run(c, args...);
     else if (s == COUPLB_FP32) F<3, float>::run(c, args...);
     else if (s == COUPLB_FP16) F<3, _Float16>::run(c, args...);
-    else F<3, bf16_t>::run(c, args...);
+    else if (s == COUPLB_BF16) F<3, bf16_t>::run(c, args...);
+    else F<3, i16_t>::run(c, args...);
   } else {
     if (s == COUPLB_FP64) F<2, double>::run(c, args...);
     else if (s == COUPLB_FP32) F<2, float>::run(c, args...);
     else if (s == COUPLB_FP16) F<2, _Float16>::run(c, args...);
-    else F<2, bf16_t>::run(c, args...);
+    else if (s == COUPLB_BF16) F<2, bf16_t>::run(c, args...);
+    else F<2, i16_t>::run(c, args...);
   }
 }
 
@@ -1271,7 +1288,9 @@ int couplb_create(const couplb_params* p, couplb_ctx** out) {
     return bad("grid extents must be >= 2 (nz = 1 in 2D)");
   if (!(p->tau > 0.5)) return bad("tau <= 0.5 (unstable)");
   if (p->kernel != COUPLB_ROMA && p->kernel != COUPLB_PESKIN4) return bad("unknown IBM kernel");
-  if (p->storage < COUPLB_FP64 || p->storage > COUPLB_BF16) return bad("unknown storage");
+  if (p->storage < COUPLB_FP64 || p->storage > COUPLB_I16) return bad("unknown storage");
+  if (p->storage == COUPLB_I16 && !(p->storage_range >= 0.0 && p->storage_range < 1e6))
+    return bad("storage_range must be >= 0 (0: default)");
   if (!(p->rho0 > 0)) return bad("rho0 must be positive");
   for (int d = 0; d < 3; d++)
     for (int s = 0; s < 2; s++)
@@ -1320,7 +1339,7 @@ int couplb_create(const couplb_params* p, couplb_ctx** out) {
   c->fused_nodes = 1024;
   if (const char* v = std::getenv("COUPLB_FUSED_NODES")) c->fused_nodes = (size_t)std::strtoull(v, nullptr, 10);
   c->elem = storage_bytes(p->storage);
-  c->scale = (p->storage == COUPLB_FP16) ? 1024.0f : 1.0f;
+  c->scale = storage_scale(p);
   c->base = (p->storage == COUPLB_FP64) ? 0.0 : p->rho0;
   int dev = p->device;
   if (hipSetDevice(dev) != hipSuccess) return bad("hipSetDevice failed");
@@ -1658,6 +1677,14 @@ int couplb_profile_y(couplb_ctx* c, double* rs, double* us, double* vs) {
     if (us) us[j] = h[3 * j + 1];
     if (vs) vs[j] = h[3 * j + 2];
   }
+  return COUPLB_OK;
+}
+
+int couplb_storage_saturations(couplb_ctx* c, unsigned long long* count) {
+  if (!c || !count) return fail(c, COUPLB_EINVAL, "null argument");
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpyFromSymbol(count, HIP_SYMBOL(couplb::i16_saturations), sizeof(*count), 0,
+                                hipMemcpyDeviceToHost));
   return COUPLB_OK;
 }
 

@@ -477,10 +477,13 @@ __device__ inline void step_node(const StepArgs& a, const Layout& lay, S* dst, c
     jz *= is;
   }
   const T base = (T)a.pc.base;
+  const T mass_in = As;  // the node's stored populations summed (stored scale)
   T rho = A + base;
   // couplb_lattice.h:178-186: rho <= RHO_MIN, then any non-finite value
   // (NaN, +Inf) is clamped to RHO_MIN
+  bool clamped = false;
   if (rho <= (T)RHO_MIN || !isfinite(rho)) {
+    clamped = true;
     rho = (T)RHO_MIN;
     A = rho - base;
     As = A * (T)a.pc.scale;
@@ -526,8 +529,18 @@ __device__ inline void step_node(const StepArgs& a, const Layout& lay, S* dst, c
     // (asym = 3 rho eu, Sasym = 3 hom ef) change sign: one evaluation per
     // pair of opposite directions.  f - feq stays one difference, as in the
     // reference's form, so the non-equilibrium part keeps its precision.
+    //
+    // Mass: BGK with Guo forcing keeps sum_q f exactly in exact arithmetic.
+    // The rest population (q = 0, which does not stream) is set to the
+    // node's incoming mass minus the other populations as they are stored
+    // after rounding, so the stored node mass changes only by the rounding
+    // of that one value: not at all for the 16-bit fixed point, whose
+    // stored values are integers in this scale (exact float sums).  A
+    // clamped density changes the mass as in the reference: then the rest
+    // population takes the collision formula.
     const T uu15 = (T)1.5 * uu, rho_s = rho * scale, rho3 = ((T)3 * rho) * scale;
     const T hom3 = ((T)3 * hom) * scale, hom9 = ((T)9 * hom) * scale, huf3 = hom3 * uf;
+    T moved = 0, rest = 0;
 #pragma unroll
     for (int q = 0; q < Q; q++) {
       const int o = L::opp(q);
@@ -538,13 +551,18 @@ __device__ inline void step_node(const StepArgs& a, const Layout& lay, S* dst, c
       const T sym = As + rho_s * ((T)4.5 * eu * eu - uu15);
       const T Ssym = hom9 * eu * ef - huf3;
       if (o == q) {
-        out(q, f[q] - om * (f[q] - wq * sym) + wq * Ssym);
+        rest = f[q] - om * (f[q] - wq * sym) + wq * Ssym;
         continue;
       }
       const T asym = rho3 * eu, Sasym = hom3 * ef;
-      out(q, f[q] - om * (f[q] - wq * (sym + asym)) + wq * (Ssym + Sasym));
-      out(o, f[o] - om * (f[o] - wq * (sym - asym)) + wq * (Ssym - Sasym));
+      const T vq = Store<S>::round(f[q] - om * (f[q] - wq * (sym + asym)) + wq * (Ssym + Sasym));
+      const T vo = Store<S>::round(f[o] - om * (f[o] - wq * (sym - asym)) + wq * (Ssym - Sasym));
+      out(q, vq);
+      out(o, vo);
+      moved += vq;
+      moved += vo;
     }
+    out(0, clamped ? rest : mass_in - moved);
   }
 }
 

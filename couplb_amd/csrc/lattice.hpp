@@ -94,7 +94,8 @@ __device__ inline double div_cs4(double x) {
 // Storage formats.  FP64 stores f_i itself (the reference's representation).
 // The reduced formats store the deviation (f_i - w_i*rho0)*scale and compute
 // in float; scale is a power of two (exact) that lifts small deviations out
-// of the half-precision subnormal range.
+// of the half-precision subnormal range, or sets the 16-bit fixed point's
+// resolution.  round(v) is the value v takes once stored.
 // ---------------------------------------------------------------------------
 struct bf16_t { uint16_t bits; };
 
@@ -103,6 +104,7 @@ template <typename S> struct Store;
 template <> struct Store<double> {
   using T = double;                    // arithmetic type
   static constexpr bool deviation = false;
+  __device__ static inline T round(T v) { return v; }
   __device__ static inline T load(const double* p) { return __builtin_nontemporal_load(p); }
   __device__ static inline void store(double* p, T v) { __builtin_nontemporal_store(v, p); }
   __device__ static inline T decode(double v) { return v; }
@@ -112,6 +114,7 @@ template <> struct Store<double> {
 template <> struct Store<float> {
   using T = float;
   static constexpr bool deviation = true;
+  __device__ static inline T round(T v) { return v; }
   __device__ static inline T load(const float* p) { return __builtin_nontemporal_load(p); }
   __device__ static inline void store(float* p, T v) { __builtin_nontemporal_store(v, p); }
   __device__ static inline T decode(float v) { return v; }
@@ -121,6 +124,7 @@ template <> struct Store<float> {
 template <> struct Store<_Float16> {
   using T = float;
   static constexpr bool deviation = true;
+  __device__ static inline T round(T v) { return (float)(_Float16)v; }
   __device__ static inline T load(const _Float16* p) {
     return (float)__builtin_nontemporal_load(p);
   }
@@ -146,6 +150,34 @@ template <> struct Store<bf16_t> {
     const uint32_t r = u + 0x7FFFu + ((u >> 16) & 1u);
     return bf16_t{(uint16_t)(r >> 16)};
   }
+  __device__ static inline T round(T v) { return decode(encode(v)); }
+};
+
+// 16-bit fixed point: the deviation times scale (a power of two), rounded
+// to the nearest integer in [-32767, 32767].  In the stored scale every
+// value is an integer, so sums of populations are exact in float and the
+// step can keep each node's mass exactly.  Values beyond the range are
+// clamped to it and counted in i16_saturations (per GPU, since load).
+struct i16_t { int16_t v; };
+extern __device__ unsigned long long i16_saturations;
+
+template <> struct Store<i16_t> {
+  using T = float;
+  static constexpr bool deviation = true;
+  __device__ static inline T load(const i16_t* p) {
+    return (float)__builtin_nontemporal_load(&p->v);
+  }
+  __device__ static inline void store(i16_t* p, T v) { __builtin_nontemporal_store(encode(v).v, &p->v); }
+  __device__ static inline T decode(i16_t v) { return (float)v.v; }
+  __device__ static inline i16_t encode(T v) {
+    float r = __builtin_rintf(v);  // round to nearest, ties to even
+    if (!(r >= -32767.0f && r <= 32767.0f)) {
+      atomicAdd(&i16_saturations, 1ull);
+      r = (r > 0.0f) ? 32767.0f : -32767.0f;
+    }
+    return i16_t{(int16_t)(int)r};
+  }
+  __device__ static inline T round(T v) { return decode(encode(v)); }
 };
 
 }  // namespace couplb

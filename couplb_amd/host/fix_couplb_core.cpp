@@ -27,13 +27,13 @@ constexpr uint32_t CLBK_MAGIC = 0x434C424Bu;  // couplb_io.h:19-23
 constexpr int32_t CLBK_VERSION = 1;
 
 // keyword whitelist, fix_couplb.cpp:22-36, plus this build's extensions
-// `storage` (population precision) and `wall_vel_face` (per-face wall
-// velocity)
+// `storage` (population precision), `storage_range` (range of the 16-bit
+// fixed point) and `wall_vel_face` (per-face wall velocity)
 bool is_keyword(const char* s) {
   static const char* kw[] = {"md_per_lb", "xi_ibm", "gravity", "wall_x", "wall_y", "wall_z",
                              "wall_vel", "output", "check_every", "kernel", "vtk", "vtk_region",
-                             "vtk_solid", "checkpoint", "restart", "storage", "wall_vel_face",
-                             nullptr};
+                             "vtk_solid", "checkpoint", "restart", "storage", "storage_range",
+                             "wall_vel_face", nullptr};
   for (int i = 0; kw[i]; i++)
     if (std::strcmp(s, kw[i]) == 0) return true;
   return false;
@@ -161,7 +161,13 @@ FixCoupLBCore::FixCoupLBCore(int narg, const char* const* arg, const couplb_doma
       else if (!std::strcmp(s, "fp32")) storage = COUPLB_FP32;
       else if (!std::strcmp(s, "fp16")) storage = COUPLB_FP16;
       else if (!std::strcmp(s, "bf16")) storage = COUPLB_BF16;
-      else throw fix_error("fix couplb storage: use fp64, fp32, fp16 or bf16");
+      else if (!std::strcmp(s, "i16")) storage = COUPLB_I16;
+      else throw fix_error("fix couplb storage: use fp64, fp32, fp16, bf16 or i16");
+      iarg += 2;
+    } else if (!std::strcmp(k, "storage_range")) {
+      need(1, "fix couplb storage_range: need 1 value");
+      storage_range = to_double(arg[iarg + 1]);
+      if (!(storage_range > 0.0)) throw fix_error("fix couplb storage_range must be > 0");
       iarg += 2;
     } else if (!std::strcmp(k, "vtk")) {
       need(2, "fix couplb vtk: need N prefix");
@@ -349,6 +355,7 @@ void FixCoupLBCore::init() {
     }
   p.kernel = kernel;
   p.storage = storage;
+  p.storage_range = storage_range;
   p.rho0 = 1.0;  // rho_lb, fix_couplb.h:41
   p.md_per_lb = md_per_lb;
   p.device = 0;
@@ -375,7 +382,8 @@ void FixCoupLBCore::init() {
   say("CoupLB: dx=%.4e dt_LBM=%.4e vel_scale=%.4e force_scale=%.4e\n", dx, dt_LBM, vel_scale, force_scale);
   say("CoupLB: kernel=%s xi_ibm=%.4f\n", kernel == COUPLB_ROMA ? "roma" : "peskin4", xi_ibm);
   say("CoupLB: device %s storage=%s\n", couplb_version(),
-      storage == COUPLB_FP64 ? "fp64" : storage == COUPLB_FP32 ? "fp32" : storage == COUPLB_FP16 ? "fp16" : "bf16");
+      storage == COUPLB_FP64 ? "fp64" : storage == COUPLB_FP32 ? "fp32" : storage == COUPLB_FP16 ? "fp16"
+      : storage == COUPLB_BF16 ? "bf16" : "i16");
 }
 
 // update_gravity_variables, fix_couplb.cpp:671-678
@@ -524,6 +532,14 @@ void FixCoupLBCore::check_stability_precomputed(long step) {
   if (ma > MA_WARN) say("CoupLB WARNING: Ma=%.4f at step %ld\n", ma, step);
   say("CoupLB step %ld: Ma=%.4f mass=%.8e mom=(%.4e,%.4e,%.4e)\n", step, ma, out[0], out[1], out[2], out[3]);
   if (out[5] > 0) say("CoupLB WARNING: %d nodes had density clamped at step %ld\n", (int)out[5], step);
+  if (storage == COUPLB_I16) {
+    unsigned long long sat = 0;
+    check(couplb_storage_saturations(ctx, &sat), "storage saturations");
+    if (sat > i16_saturations_seen)
+      say("CoupLB WARNING: %llu populations exceeded storage_range by step %ld\n",
+          sat - i16_saturations_seen, step);
+    i16_saturations_seen = sat;
+  }
 }
 
 // write_profile, fix_couplb.cpp:903-979 (one device grid: no gathers)

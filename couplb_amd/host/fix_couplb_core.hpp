@@ -106,6 +106,8 @@ class FixCoupLBCore {
   bool do_restart = false;
   int kernel = COUPLB_ROMA;
   int storage = COUPLB_FP64;
+  double storage_range = 0.0;                 // COUPLB_I16 range (0: library default)
+  unsigned long long i16_saturations_seen = 0;  // counter value at the last check
   bool is3d = true;
 
   double dx = 0, inv_dx = 0, dt_LBM = 0, vel_scale = 0, nu_lb = 0, tau = 0, force_scale = 0;

@@ -31,6 +31,8 @@ extern "C" {
 #define COUPLB_FP32 1 /* f_i - w_i*rho0 in float, float arithmetic */
 #define COUPLB_FP16 2 /* f_i - w_i*rho0 in IEEE half, float arithmetic */
 #define COUPLB_BF16 3 /* f_i - w_i*rho0 in bfloat16, float arithmetic */
+#define COUPLB_I16 4  /* f_i - w_i*rho0 in 16-bit fixed point (storage_range),
+                         float arithmetic, node mass kept exactly */
 
 /* IBM delta kernels, couplb_ibm.h:11-33 / fix_couplb.cpp:175-180 */
 #define COUPLB_ROMA 0
@@ -58,10 +60,13 @@ typedef struct couplb_params {
   double face_vel[3][2][3]; /* velocity of each type-2 face; the reference
                                gives every face the same one */
   int kernel;             /* COUPLB_ROMA or COUPLB_PESKIN4 */
-  int storage;            /* COUPLB_FP64 .. COUPLB_BF16 */
+  int storage;            /* COUPLB_FP64 .. COUPLB_I16 */
   double rho0;            /* lattice reference density (fix_couplb.h:41: 1.0) */
   int md_per_lb;          /* IBM sub-steps per LB step (sizes record slots) */
   int device;             /* HIP device ordinal */
+  double storage_range;   /* COUPLB_I16: largest |f_i - w_i*rho0| to hold
+                             (the resolution is the largest power of two
+                             with 32767 steps inside it); 0: 1/32 */
 } couplb_params;
 
 int couplb_create(const couplb_params* p, couplb_ctx** out);
@@ -250,6 +255,11 @@ int couplb_diagnostics_noalloc(couplb_ctx* ctx, double out[6]);
 /* y-binned sums over x and z of rho, ux, uy (write_profile,
  * fix_couplb.cpp:903-928); each array has ny entries.  Also a refresh. */
 int couplb_profile_y(couplb_ctx* ctx, double* rho_sum, double* ux_sum, double* uy_sum);
+
+/* Populations the 16-bit fixed-point storage (COUPLB_I16) had to clamp to
+ * its range on this context's GPU since the library was loaded (all
+ * contexts of that GPU together).  No counterpart in the reference. */
+int couplb_storage_saturations(couplb_ctx* ctx, unsigned long long* count);
 
 int couplb_synchronize(couplb_ctx* ctx);
 /* Measurement helper, no counterpart in the reference: read + write GB/s of

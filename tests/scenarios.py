@@ -40,7 +40,7 @@ def parse_cmd(cmd):
             kw["output_every"] = int(a[i + 1]); i += 3
         elif k == "kernel":
             kw["kernel"] = oracle.ROMA if a[i + 1] == "roma" else oracle.PESKIN4; i += 2
-        elif k == "storage":
+        elif k in ("storage", "storage_range"):
             i += 2
         else:
             raise ValueError(k)

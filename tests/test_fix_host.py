@@ -21,7 +21,8 @@ from couplb_amd import CoupLBError, Fix
     ("f all couplb 8 8 8 0.1 1.0 gravity 1 2", "gravity: need 3 values"),
     ("f all couplb 8 8 8 0.1 1.0 check_every -1", "check_every cannot be negative"),
     ("f all couplb 8 8 8 0.1 1.0 vtk_region 0 1 2", "vtk_region: need 4 values"),
-    ("f all couplb 8 8 8 0.1 1.0 storage fp8", "use fp64, fp32, fp16 or bf16"),
+    ("f all couplb 8 8 8 0.1 1.0 storage fp8", "use fp64, fp32, fp16, bf16 or i16"),
+        ("f all couplb 8 8 8 0.1 1.0 storage i16 storage_range 0", "storage_range must be > 0"),
     ("f all couplb 8 8 8 0.1 1.0 vtk 10 v vtk_solid id vtk_solid x", "vtk_solid: keyword specified more than once"),
     ("f all couplb 8 8 8 0.1 1.0 vtk_solid", "vtk_solid: need at least one attribute"),
     ("f all couplb 8 8 8 0.1 1.0 vtk -1 v", "vtk frequency cannot be negative"),

@@ -15,13 +15,13 @@
 #include <rocprim/iterator/counting_iterator.hpp>
 
 #include "../../include/couplb_hip.h"
-#include "gather.hpp"
 #include "ibm.hpp"
 #include "kernels.hpp"
 #include "mdf.hpp"
 #include "mdf_matrix.hpp"
 #include "mlp.hpp"
 #include "spread_pairs.hpp"
+#include "spread_rows.hpp"
 
 using namespace couplb;
 
@@ -75,7 +75,6 @@ struct TimerRing {
 };
 
 struct SlotHost {
-  int* rowstart = nullptr;
   MarkerRec* rec = nullptr;
   int n = 0;
   int epoch = -1;  // marker ordering the records were built from
@@ -126,6 +125,7 @@ struct couplb_ctx {
   void* sort_tmp = nullptr;
   size_t sort_tmp_bytes = 0;
   double *dx = nullptr, *dv = nullptr, *dm = nullptr, *df = nullptr;
+  int* moved = nullptr;  // count + list of markers whose base cell moved (cap + 1)
   int order_n = -1;       // marker count of the cached ordering
   double order_key[5] = {0, 0, 0, 0, -1};  // lo, inv_dx, kernel of the cached ordering
   int pos_epoch = 0;      // incremented whenever the ordering is rebuilt
@@ -176,6 +176,15 @@ struct couplb_ctx {
   int ugather_cycle = -1; // state counter of the last velocity gather
   std::vector<SlotHost> slots;
   int nslots = 0;
+  int f4_partial = 0;  // F4 / flag already hold the spread records of earlier
+                       // sub-steps of this interval (slots were full)
+  // merged, sorted record keys of the pending slots (spread_rows.hpp)
+  unsigned long long *mkey = nullptr, *mkey_s = nullptr;
+  unsigned *mval = nullptr, *mval_s = nullptr;
+  int* mrow = nullptr;
+  void* msort_tmp = nullptr;
+  size_t msort_bytes = 0;
+  long mcap = 0;
   Slot* slot_table = nullptr;  // device copy of the slot descriptors
   int slot_table_n = -1;       // descriptors currently uploaded
   double* staging = nullptr;  // ghost-inclusive FP64 state for import/export
@@ -265,13 +274,6 @@ dim3 row_grid(const Geo& g, int bx) {
   return dim3((unsigned)((g.nx + bx - 1) / bx), (unsigned)g.ny, (unsigned)(g.D == 3 ? g.nz : 1));
 }
 
-// IBM tile kernels: one block per tile of rows (gather.hpp, tile_shape)
-dim3 tile_grid(const Geo& g) {
-  const TileShape ts = tile_shape(g.D, g.nx);
-  return dim3(1, (unsigned)((g.ny + ts.TJ - 1) / ts.TJ),
-              (unsigned)(g.D == 3 ? (g.nz + ts.TK - 1) / ts.TK : 1));
-}
-
 PullCtx make_pull(couplb_ctx* c) {
   PullCtx pc;
   pc.src = c->f[c->cur];
@@ -330,6 +332,17 @@ template <int D, typename S> struct LaunchMacro {
     const dim3 b = row_block(c->geo.nx);
     hipLaunchKernelGGL((k_macro<D, S>), dim3(1, c->geo.ny, c->D == 3 ? c->geo.nz : 1), b, 0,
                        c->stream, a);
+  }
+};
+template <int D, typename S> struct LaunchFixup {
+  static void run(couplb_ctx* c, int n, const double* x, const IbmParams& ip, const PullCtx& pc) {
+    const int tb = 256;
+    int* count = c->moved;         // moved[0]: count, moved + 1: list
+    (void)hipMemsetAsync(count, 0, sizeof(int), c->stream);
+    hipLaunchKernelGGL(k_moved_markers, dim3((n + tb - 1) / tb), dim3(tb), 0, c->stream, n, c->perm, x, ip,
+                       c->geo, (const Base*)c->bases, count, count + 1);
+    hipLaunchKernelGGL((k_moved_cover<D, S>), dim3(MOVED_BLOCKS), dim3(tb), 0, c->stream, c->perm, x, ip, pc,
+                       (const int*)count, (const int*)(count + 1), c->u4, c->stamp, c->cycle);
   }
 };
 template <int D, typename S> struct LaunchMacroParts {
@@ -429,8 +442,6 @@ unsigned key_bits(size_t nrows) {
 }
 
 int ensure_ibm(couplb_ctx* c, int n) {
-  if (c->geo.nx > MAX_NX_IBM)
-    return fail(c, COUPLB_EINVAL, "IBM coupling supports rows of at most 1024 nodes in x");
   if (!c->u4) {
     HIPCHK(c, hipMalloc(&c->u4, sizeof(double) * 4 * c->N));
     HIPCHK(c, hipMalloc(&c->stamp, sizeof(int) * c->N));
@@ -456,6 +467,9 @@ int ensure_ibm(couplb_ctx* c, int n) {
   HIPCHK(c, hipMalloc(&c->dm, sizeof(double) * cap));
   HIPCHK(c, hipMalloc(&c->df, sizeof(double) * 3 * cap));
   HIPCHK(c, hipMalloc(&c->ufc, sizeof(double) * 3 * cap));
+  (void)hipFree(c->moved);
+  c->moved = nullptr;
+  HIPCHK(c, hipMalloc(&c->moved, sizeof(int) * ((size_t)cap + 1)));
   (void)hipFree(c->fsrc);
   c->fsrc = nullptr;
   HIPCHK(c, hipMalloc(&c->fsrc, sizeof(double) * 3 * cap));
@@ -493,7 +507,6 @@ int ensure_slot(couplb_ctx* c, int s) {
     if ((int)c->slots.size() >= MAX_SLOTS)
       return fail(c, COUPLB_ESTATE, "more than 16 IBM sub-steps between two LB steps");
     SlotHost h;
-    HIPCHK(c, hipMalloc(&h.rowstart, sizeof(int) * (c->geo.nrows + 1)));
     HIPCHK(c, hipMalloc(&h.rec, sizeof(MarkerRec) * std::max(1, c->cap)));
     c->slots.push_back(h);
   }
@@ -513,13 +526,14 @@ IbmParams make_ibm_params(const couplb_ctx* c, const couplb_ibm_args* a) {
 }
 
 // Marker ordering by base row (kept while the caller declares the positions
-// unchanged) and the fluid velocity at covered nodes of the current state
-// (once per state).  Sets *reorder when the ordering was rebuilt.
+// unchanged, and over the later sub-steps of an LB interval: keep_order)
+// and the fluid velocity at covered nodes of the current state (once per
+// state).  Sets *reorder when the ordering was rebuilt.
 int prepare_markers(couplb_ctx* c, int n, const double* x, const IbmParams& ip,
-                    int positions_unchanged, bool* reorder_out) {
+                    int positions_unchanged, bool keep_order, bool* reorder_out) {
   const int tb = 256, nb = (n + tb - 1) / tb;
   const double key[5] = {ip.lo[0], ip.lo[1], ip.lo[2], ip.inv_dx, (double)c->p.kernel};
-  const bool reorder = !(positions_unchanged && c->order_n == n &&
+  const bool reorder = !((positions_unchanged || keep_order) && c->order_n == n &&
                          std::memcmp(key, c->order_key, sizeof(key)) == 0);
   if (reorder) {
     hipLaunchKernelGGL(k_keys, dim3(nb), dim3(tb), 0, c->stream, n, x, ip, c->geo, c->keys, c->vals);
@@ -549,9 +563,17 @@ int prepare_markers(couplb_ctx* c, int n, const double* x, const IbmParams& ip,
     c->cover_epoch = c->pos_epoch;
     c->ugather_cycle = c->cycle;
   }
+  if (!reorder && !positions_unchanged) {
+    // moved markers on the kept ordering: velocities of nodes they newly reach
+    PullCtx pc = make_pull(c);
+    dispatch<LaunchFixup>(c, n, x, ip, pc);
+    HIPCHK(c, hipGetLastError());
+  }
   *reorder_out = reorder;
   return COUPLB_OK;
 }
+
+int fold_slots(couplb_ctx* c);
 
 int ibm_impl(couplb_ctx* c, int n, const double* x, const double* v, const double* m,
              double* f_dev, int fmode, const couplb_ibm_args* a) {
@@ -565,11 +587,14 @@ int ibm_impl(couplb_ctx* c, int n, const double* x, const double* v, const doubl
   rc = timer_begin(c, 1, &tslot);
   if (rc) return rc;
   bool reorder = false;
-  rc = prepare_markers(c, n, x, ip, a->positions_unchanged, &reorder);
+  // a later sub-step of the interval keeps the interval's marker ordering
+  const bool later = c->nslots > 0 || c->f4_partial;
+  rc = prepare_markers(c, n, x, ip, a->positions_unchanged, later, &reorder);
   if (rc) return rc;
   // Same positions and same LB state as the previous sub-step of this
   // interval: u_f is known and only the force is new (k_marker_force).
-  const bool merge = !reorder && c->nslots > 0 && c->slots[c->nslots - 1].epoch == c->pos_epoch &&
+  const bool merge = !reorder && a->positions_unchanged && c->nslots > 0 &&
+                     c->slots[c->nslots - 1].epoch == c->pos_epoch &&
                      c->slots[c->nslots - 1].n == n && c->uf_cycle == c->cycle &&
                      c->uf_epoch == c->pos_epoch;
   if (merge) {
@@ -579,6 +604,10 @@ int ibm_impl(couplb_ctx* c, int n, const double* x, const double* v, const doubl
     HIPCHK(c, hipGetLastError());
     c->rho_fresh = 1;  // compute_macroscopic(false) at fix_couplb.cpp:758
     return timer_end(c, tslot);
+  }
+  if (c->nslots == MAX_SLOTS) {  // every slot pending: spread them into F4 first
+    rc = fold_slots(c);
+    if (rc) return rc;
   }
   const int s = c->nslots;
   rc = ensure_slot(c, s);
@@ -594,8 +623,6 @@ int ibm_impl(couplb_ctx* c, int n, const double* x, const double* v, const doubl
                        fmode, ip, c->geo, c->walls, c->u4, sl.rec, c->ufc, s == 0 ? c->fsrc : nullptr);
   }
   HIPCHK(c, hipGetLastError());
-  HIPCHK(c, hipMemcpyAsync(sl.rowstart, c->rowstart_cur, sizeof(int) * (c->geo.nrows + 1),
-                           hipMemcpyDeviceToDevice, c->stream));
   rc = timer_end(c, tslot);
   if (rc) return rc;
   sl.n = n;
@@ -819,7 +846,7 @@ int mdf_impl(couplb_ctx* c, int n, const double* x, const double* V, const doubl
   if (!(a->tol > 0) || a->max_iter < 1 || !(a->relax > 0) ||
       (a->method != COUPLB_MDF_RICHARDSON && a->method != COUPLB_MDF_CG))
     return fail(c, COUPLB_EINVAL, "MDF needs tol > 0, max_iter >= 1, relax > 0 and a known method");
-  if (c->nslots != 0)
+  if (c->nslots != 0 || c->f4_partial)
     return fail(c, COUPLB_ESTATE, "MDF must be the only IBM call between two LB steps");
   if (st) *st = couplb_mdf_stats{0, 0, 0.0, 0.0, -1.0};
   if (n == 0) return COUPLB_OK;
@@ -872,7 +899,7 @@ int mdf_impl(couplb_ctx* c, int n, const double* x, const double* V, const doubl
   rc = timer_begin(c, 3, &tslot);
   if (rc) return rc;
   bool reorder = false;
-  rc = prepare_markers(c, n, x, ip, a->ibm.positions_unchanged, &reorder);
+  rc = prepare_markers(c, n, x, ip, a->ibm.positions_unchanged, false, &reorder);
   if (rc) return rc;
   SlotHost& sl = c->slots[0];
   MdfScalars* Sd = c->mdf_S;
@@ -885,8 +912,6 @@ int mdf_impl(couplb_ctx* c, int n, const double* x, const double* V, const doubl
     hipLaunchKernelGGL(k_mdf_init<2>, dim3(nb), dim3(tb), 0, c->stream, n, c->perm, x, V, F0i, ip, c->geo,
                        c->walls, sl.rec, c->mdf_v, c->mdf_F, c->fsrc, c->mm_gam);
   HIPCHK(c, hipGetLastError());
-  HIPCHK(c, hipMemcpyAsync(sl.rowstart, c->rowstart_cur, sizeof(int) * (c->geo.nrows + 1),
-                           hipMemcpyDeviceToDevice, c->stream));
   sl.n = n;
   sl.epoch = c->pos_epoch;
   if (c->pair_epoch != sl.epoch) {
@@ -1081,41 +1106,97 @@ int mdf_impl(couplb_ctx* c, int n, const double* x, const double* V, const doubl
   return COUPLB_OK;
 }
 
-// k_spread of the pending record slots into F4 / flag (fold: add the
-// periodic ghost images into the interior)
-int tile_spread(couplb_ctx* c, double* F4, unsigned char* flag, int fold) {
+// The pending record slots spread into F4 / flag (spread_rows.hpp): the
+// records' base-cell keys of all slots merged and radix sorted, a row table,
+// then one wavefront per 64-node row segment.  fold: add the periodic ghost
+// images into the interior; accumulate: add to the forces F4 / flag hold.
+int row_spread(couplb_ctx* c, double* F4, unsigned char* flag, int fold, int accumulate) {
+  long total = 0;
+  for (int s = 0; s < c->nslots; s++) total += c->slots[s].n;
+  const unsigned long long xw = spread_xw(c->geo);
+  const unsigned long long sentinel = (unsigned long long)c->geo.nrows * xw;
+  unsigned bits = 1;
+  while (bits < 64 && (1ull << bits) <= sentinel) bits++;
+  if (total > c->mcap) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    (void)hipFree(c->mkey); (void)hipFree(c->mkey_s); (void)hipFree(c->mval); (void)hipFree(c->mval_s);
+    (void)hipFree(c->msort_tmp);
+    c->mkey = c->mkey_s = nullptr; c->mval = c->mval_s = nullptr; c->msort_tmp = nullptr; c->mcap = 0;
+    const long cap = std::max(total, 2 * c->mcap);
+    HIPCHK(c, hipMalloc(&c->mkey, sizeof(unsigned long long) * cap));
+    HIPCHK(c, hipMalloc(&c->mkey_s, sizeof(unsigned long long) * cap));
+    HIPCHK(c, hipMalloc(&c->mval, sizeof(unsigned) * cap));
+    HIPCHK(c, hipMalloc(&c->mval_s, sizeof(unsigned) * cap));
+    size_t bytes = 0;
+    HIPCHK(c, rocprim::radix_sort_pairs(nullptr, bytes, c->mkey, c->mkey_s, c->mval, c->mval_s, (size_t)cap,
+                                        0u, 64u, c->stream));
+    HIPCHK(c, hipMalloc(&c->msort_tmp, bytes));
+    c->msort_bytes = bytes;
+    c->mcap = cap;
+  }
+  if (!c->mrow) HIPCHK(c, hipMalloc(&c->mrow, sizeof(int) * (c->geo.nrows + 1)));
   if (!c->slot_table) HIPCHK(c, hipMalloc(&c->slot_table, sizeof(Slot) * MAX_SLOTS));
   if (c->slot_table_n != (int)c->slots.size()) {
     Slot h[MAX_SLOTS];
-    for (size_t s = 0; s < c->slots.size(); s++) {
-      h[s].rowstart = c->slots[s].rowstart;
-      h[s].rec = c->slots[s].rec;
-    }
+    for (size_t s = 0; s < c->slots.size(); s++) h[s].rec = c->slots[s].rec;
     HIPCHK(c, hipMemcpy(c->slot_table, h, sizeof(Slot) * c->slots.size(), hipMemcpyHostToDevice));
     c->slot_table_n = (int)c->slots.size();
   }
-  SpreadArgs sa;
-  sa.g = c->geo;
-  sa.w = c->walls;
-  sa.nslots = c->nslots;
-  sa.slots = c->slot_table;
-  sa.F4 = F4;
-  sa.flag = flag;
-  sa.fold = fold;
-  const TileShape ts = tile_shape(c->D, c->geo.nx);
-#define SPREAD_LAUNCH(DD, TJ, TK, XM) \
-  hipLaunchKernelGGL((k_spread<DD, TJ, TK, XM>), tile_grid(c->geo), dim3(TILE_THREADS), 0, c->stream, sa)
-  if (c->D == 3) {
-    if (ts.XM == 1) SPREAD_LAUNCH(3, 2, 2, 1);
-    else if (ts.XM == 2) SPREAD_LAUNCH(3, 2, 1, 2);
-    else SPREAD_LAUNCH(3, 1, 1, 4);
-  } else {
-    if (ts.XM == 1) SPREAD_LAUNCH(2, 4, 1, 1);
-    else if (ts.XM == 2) SPREAD_LAUNCH(2, 2, 1, 2);
-    else SPREAD_LAUNCH(2, 1, 1, 4);
+  const int tb = 256;
+  long off = 0;
+  for (int s = 0; s < c->nslots; s++) {
+    const int n = c->slots[s].n;
+    if (n <= 0) continue;
+    if (c->D == 3)
+      hipLaunchKernelGGL(k_rec_keys<3>, dim3((n + tb - 1) / tb), dim3(tb), 0, c->stream, n,
+                         (const MarkerRec*)c->slots[s].rec, c->geo, (unsigned)s, sentinel, c->mkey + off,
+                         c->mval + off);
+    else
+      hipLaunchKernelGGL(k_rec_keys<2>, dim3((n + tb - 1) / tb), dim3(tb), 0, c->stream, n,
+                         (const MarkerRec*)c->slots[s].rec, c->geo, (unsigned)s, sentinel, c->mkey + off,
+                         c->mval + off);
+    off += n;
   }
-#undef SPREAD_LAUNCH
   HIPCHK(c, hipGetLastError());
+  if (total > 0) {
+    size_t bytes = c->msort_bytes;
+    HIPCHK(c, rocprim::radix_sort_pairs(c->msort_tmp, bytes, c->mkey, c->mkey_s, c->mval, c->mval_s,
+                                        (size_t)total, 0u, bits, c->stream));
+  }
+  hipLaunchKernelGGL(k_key_rowstart, dim3((c->geo.nrows + 1 + tb - 1) / tb), dim3(tb), 0, c->stream,
+                     (const unsigned long long*)c->mkey_s, (int)total, c->geo.nrows, xw, c->mrow);
+  RowSpreadArgs ra;
+  ra.g = c->geo;
+  ra.w = c->walls;
+  ra.key = c->mkey_s;
+  ra.val = c->mval_s;
+  ra.rowstart = c->mrow;
+  ra.slots = c->slot_table;
+  ra.F4 = F4;
+  ra.flag = flag;
+  ra.fold = fold;
+  ra.accumulate = accumulate;
+  const unsigned gx = (unsigned)((c->geo.nx + ST_XS - 1) / ST_XS);
+  if (c->D == 3)
+    hipLaunchKernelGGL(k_spread_rows<3>,
+                       dim3(gx, (unsigned)((c->geo.ny + ST_TJ - 1) / ST_TJ), (unsigned)((c->geo.nz + ST_TK - 1) / ST_TK)),
+                       dim3(ST_THREADS), 0, c->stream, ra);
+  else
+    hipLaunchKernelGGL(k_spread_rows<2>, dim3(gx, (unsigned)((c->geo.ny + ST_TJ2 - 1) / ST_TJ2), 1u),
+                       dim3(ST_THREADS), 0, c->stream, ra);
+  HIPCHK(c, hipGetLastError());
+  return COUPLB_OK;
+}
+
+// All record slots in use: spread them into F4 (added to what it holds)
+// and free the slots for the following sub-steps of the interval.  Any
+// number of sub-steps per LB step is then spread, in sub-step order.
+int fold_slots(couplb_ctx* c) {
+  const int rc = row_spread(c, c->F4, c->flag, 1, c->f4_partial);
+  if (rc) return rc;
+  c->pair_epoch = -1;  // the node flags now describe F4
+  c->f4_partial = 1;
+  c->nslots = 0;
   return COUPLB_OK;
 }
 
@@ -1133,7 +1214,7 @@ StepArgs step_args(couplb_ctx* c, const double g[3], int flags) {
   a.write_rho = (grav && !(flags & COUPLB_STEP_RHO_REFRESHED_NEXT)) ? 1 : 0;
   a.has_moving = c->has_moving;
   a.rho_out = c->rho;
-  a.has_ibm = c->nslots > 0;
+  a.has_ibm = c->nslots > 0 || c->f4_partial;
   a.flag = c->flag;
   a.F4 = c->F4;
   a.pair_mode = 0;
@@ -1195,7 +1276,7 @@ template <int D, typename S> struct FusedSetup {
 // duration of one launch); the context ends where n step_impl calls leave it.
 constexpr int FUSED_MAX_RUN = 1 << 16;
 int fused_steps(couplb_ctx* c, const double g[3], int flags, int n) {
-  if (c->nslots > 0 || c->local) return COUPLB_EINVAL;
+  if (c->nslots > 0 || c->f4_partial || c->local) return COUPLB_EINVAL;
   while (n > 0) {
     const int m = std::min(n, FUSED_MAX_RUN);
     const StepArgs a = step_args(c, g, flags);
@@ -1215,7 +1296,7 @@ int step_impl(couplb_ctx* c, const double g[3], int flags, bool timed) {
   const int dst = c->cur ^ 1;
   const int S = (c->D == 3) ? stencil_size<3>() : stencil_size<2>();
   // markers at the positions of the previous LB interval: constant map
-  const bool fixed = c->nslots == 1 && c->slots[0].epoch == c->prev_step_epoch &&
+  const bool fixed = !c->f4_partial && c->nslots == 1 && c->slots[0].epoch == c->prev_step_epoch &&
                      (long)c->slots[0].n * S < (long)INT32_MAX;
   int tspread = -1;
   if (fixed) {
@@ -1234,13 +1315,13 @@ int step_impl(couplb_ctx* c, const double g[3], int flags, bool timed) {
     a.pw = c->pw;
     a.fsrc = c->fsrc;
   } else if (c->nslots > 0) {
-    // spread the pending marker records into the per-node force (k_spread)
-    c->pair_epoch = -1;  // k_spread rewrites the node flags
+    // spread the pending marker records into the per-node force
+    c->pair_epoch = -1;  // the spreading rewrites the node flags
     if (timed) {
       const int rc = timer_begin(c, 2, &tspread);
       if (rc) return rc;
     }
-    const int rc = tile_spread(c, c->F4, c->flag, 1);
+    const int rc = row_spread(c, c->F4, c->flag, 1, c->f4_partial);
     if (rc) return rc;
   }
   if (tspread >= 0) {
@@ -1263,6 +1344,7 @@ int step_impl(couplb_ctx* c, const double g[3], int flags, bool timed) {
   c->local = 0;
   c->prev_valid = 1;
   c->nslots = 0;
+  c->f4_partial = 0;
   c->rho_fresh = 0;
   c->cycle++;
   return COUPLB_OK;
@@ -1417,7 +1499,10 @@ int couplb_destroy(couplb_ctx* c) {
   if (c->mdf_host) (void)hipHostFree(c->mdf_host);
   for (auto& e : c->mdf_ev)
     if (e) (void)hipEventDestroy(e);
-  for (auto& s : c->slots) { (void)hipFree(s.rowstart); (void)hipFree(s.rec); }
+  for (auto& s : c->slots) (void)hipFree(s.rec);
+  (void)hipFree(c->moved);
+  (void)hipFree(c->mkey); (void)hipFree(c->mkey_s); (void)hipFree(c->mval); (void)hipFree(c->mval_s);
+  (void)hipFree(c->mrow); (void)hipFree(c->msort_tmp);
   (void)hipFree(c->slot_table);
   for (int e = 0; e < TimerRing::R; e++) {
     if (c->tr.a[e]) (void)hipEventDestroy(c->tr.a[e]);
@@ -1457,6 +1542,7 @@ int couplb_set_state_f(couplb_ctx* c, const double* fh) {
   c->local = 1;
   c->prev_valid = 0;
   c->nslots = 0;
+  c->f4_partial = 0;
   c->cycle++;
   // read_checkpoint ends with compute_macroscopic(false) (couplb_io.h:817):
   // the density array then holds the restored density.
@@ -1481,6 +1567,7 @@ int couplb_init_equilibrium(couplb_ctx* c, double rho, double ux, double uy, dou
   c->local = 1;
   c->prev_valid = 0;
   c->nslots = 0;
+  c->f4_partial = 0;
   c->cycle++;
   c->rho_fresh = 0;  // like Grid::init_equilibrium: the array holds rho0
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1604,7 +1691,7 @@ int couplb_get_macro(couplb_ctx* c, double* rho, double* ux, double* uy, double*
 int couplb_get_node_forces(couplb_ctx* c, double* f, int fold) {
   if (!c || !f) return COUPLB_EINVAL;
   const size_t N = c->N;
-  if (c->nslots == 0) {
+  if (c->nslots == 0 && !c->f4_partial) {
     std::memset(f, 0, sizeof(double) * 3 * N);
     return COUPLB_OK;
   }
@@ -1617,8 +1704,15 @@ int couplb_get_node_forces(couplb_ctx* c, double* f, int fold) {
   double* out = c->staging + 4 * N + (N + 7) / 8;
   if ((size_t)(out + 3 * N - c->staging) > c->staging_len)
     return fail(c, COUPLB_ESTATE, "staging buffer too small for the node forces");
-  rc = tile_spread(c, F4x, flagx, fold ? 1 : 0);
-  if (rc) return rc;
+  if (c->f4_partial) {
+    // records of earlier sub-steps already spread (and folded) into F4
+    HIPCHK(c, hipMemcpyAsync(F4x, c->F4, sizeof(double) * 4 * N, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(flagx, c->flag, N, hipMemcpyDeviceToDevice, c->stream));
+  }
+  if (c->nslots > 0) {
+    rc = row_spread(c, F4x, flagx, fold ? 1 : 0, c->f4_partial);
+    if (rc) return rc;
+  }
   hipLaunchKernelGGL(k_force_export, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, c->stream, N,
                      (const double*)F4x, (const unsigned char*)flagx, out);
   HIPCHK(c, hipGetLastError());

@@ -7,7 +7,7 @@
 // contraction off, so floor() picks the same base cell and the delta values
 // round exactly as the reference's couplb_ibm.h:11-55 compiled without FMA.
 #pragma once
-#include "gather.hpp"
+#include "spread_rows.hpp"
 
 namespace couplb {
 
@@ -229,6 +229,59 @@ __global__ void __launch_bounds__(lds_chunk<S>()) k_upull_lds(PullCtx pc, const 
   });
   *reinterpret_cast<double4*>(u4 + 4 * (size_t)node) = upull_moments<D, S>(f, pc.base);
   stamp[node] = cycle;
+}
+
+// A later sub-step of the same LB state with moved markers keeps the
+// interval's marker ordering, coverage and gathered velocities.  A marker
+// whose base cell changed since the ordering was built may reach nodes the
+// gather did not cover.  k_moved_markers lists those markers (one thread
+// per marker; the list order does not matter); k_moved_cover then computes
+// the velocity of their stencil nodes not yet gathered for this state (the
+// same pull and moments as k_upull, stamped), one wavefront per listed
+// marker and one lane per stencil node, over a fixed grid that walks the
+// list.  Two markers may compute the same node; they store the same values.
+__global__ void __launch_bounds__(256) k_moved_markers(int n, const int* perm, const double* x, IbmParams ip,
+                                                       Geo g, const Base* bases, int* count, int* list) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  double lx, ly, lz;
+  to_local(ip, g.D, x + 3 * (size_t)perm[p], lx, ly, lz);
+  if (!(isfinite(lx) && isfinite(ly) && isfinite(lz) && fabs(lx) < 1e9 && fabs(ly) < 1e9 && fabs(lz) < 1e9))
+    return;
+  const int i0 = (int)floor(lx), j0 = (int)floor(ly), k0 = (g.D == 3) ? (int)floor(lz) : 0;
+  const Base b = bases[p];
+  if (b.valid && b.i0 == i0 && b.j0 == j0 && b.k0 == k0) return;  // covered when ordered
+  list[atomicAdd(count, 1)] = p;
+}
+
+constexpr int MOVED_BLOCKS = 256;
+template <int D, typename S>
+__global__ void __launch_bounds__(256) k_moved_cover(const int* perm, const double* x, IbmParams ip,
+                                                     PullCtx pc, const int* count, const int* list,
+                                                     double* u4, int* stamp, int cycle) {
+  using T = typename Store<S>::T;
+  const Geo& g = pc.g;
+  const int lane = threadIdx.x & 63;
+  const int nm = *count;
+  for (int m = blockIdx.x * 4 + (threadIdx.x >> 6); m < nm; m += MOVED_BLOCKS * 4) {
+    if (lane >= stencil_size<D>()) continue;
+    const int p = list[m];
+    double lx, ly, lz;
+    to_local(ip, D, x + 3 * (size_t)perm[p], lx, ly, lz);
+    const int i0 = (int)floor(lx), j0 = (int)floor(ly), k0 = (D == 3) ? (int)floor(lz) : 0;
+    const int a = i0 - 1 + (lane & 3), bb = j0 - 1 + ((lane >> 2) & 3);
+    const int c = (D == 3) ? k0 - 1 + (lane >> 4) : 0;
+    if (a < 0 || a > g.nx + 1 || bb < 0 || bb > g.ny + 1 || (D == 3 && (c < 0 || c > g.nz + 1))) continue;
+    int wd, wsd;
+    if (node_type(pc.w, g, a, bb, c, wd, wsd) != 0) continue;
+    const int i = wrap_ghost(a, g.nx), j = wrap_ghost(bb, g.ny), k = (D == 3) ? wrap_ghost(c, g.nz) : 0;
+    const unsigned node = (unsigned)i + (unsigned)g.nx * ((unsigned)j + (unsigned)g.ny * (unsigned)k);
+    if (stamp[node] == cycle) continue;
+    T f[Lat<D>::Q];
+    pull_node<D, S>(pc, i, j, k, f);
+    *reinterpret_cast<double4*>(u4 + 4 * (size_t)node) = upull_moments<D, S>(f, pc.base);
+    stamp[node] = cycle;
+  }
 }
 
 // Interpolation sums of one marker over its stencil plane c = k0-1+vv (3D;

@@ -11,6 +11,8 @@
 //   Periodic faces are index wraps; wall faces are coordinate tests; the
 //   reference's ghost planes (couplb_lattice.h:82-104) are never stored.
 #pragma once
+#include <cstddef>
+
 #include "lattice.hpp"
 
 namespace couplb {
@@ -28,8 +30,8 @@ struct Walls {
 };
 
 // One IBM marker contribution record, written by k_marker and read by the
-// spreading (the tile gather k_spread, or the pair map built from it for
-// fixed markers).  The weights are the 1-D delta values at the
+// spreading (the row gather k_spread_rows, or the pair map built from it
+// for fixed markers).  The weights are the 1-D delta values at the
 // four stencil offsets -1..+2 of the base cell (i0, j0, k0) in the
 // reference's ghost-inclusive coordinates (couplb_ibm.h:57-73).
 struct MarkerRec {
@@ -39,9 +41,11 @@ struct MarkerRec {
   int i0, j0, k0, valid;
 };
 
+static_assert(sizeof(MarkerRec) % 16 == 0 && offsetof(MarkerRec, fz) == 15 * sizeof(double),
+              "the spreading loads a record's first 16 doubles as 16-byte groups");
+
 struct Slot {
-  const int* rowstart;   // [nrows+1] into rec
-  const MarkerRec* rec;  // sorted by base row
+  const MarkerRec* rec;  // one record per marker, in the sorted marker order
 };
 
 struct Geo {

@@ -55,13 +55,13 @@ def couette():
 
 
 def test_c5_mdf_with_the_trained_network():
-    import torch
+    # device buffers through the library's own HIP runtime (DeviceArray)
     import couplb_amd
-    cuda = torch.device("cuda:0")
+    from tests.scenarios import DeviceArray
     x, ns = spheres()
     n = len(x)
-    tx = torch.from_numpy(x).to(cuda)
-    tV = torch.zeros((n, 3), dtype=torch.float64, device=cuda)
+    tx = DeviceArray(x)
+    tV = DeviceArray(np.zeros((n, 3)))
 
     def device(storage="fp64"):
         return couplb_amd.Device(3, N, N, N, tau=0.8, walls=WALLS, face_vel=FACE,
@@ -70,20 +70,21 @@ def test_c5_mdf_with_the_trained_network():
     print("case C5 256^3, 64 spheres: training data", flush=True)
     A = device()
     A.set_state(couette())
-    feats = torch.zeros((TRAIN * n, 67), dtype=torch.float32, device=cuda)
-    targs = torch.zeros((TRAIN * n, 3), dtype=torch.float32, device=cuda)
-    fb = torch.zeros((n, 3), dtype=torch.float64, device=cuda)
+    feats = DeviceArray(np.zeros((TRAIN * n, 67)), np.float32)
+    targs = DeviceArray(np.zeros((TRAIN * n, 3)), np.float32)
+    fb = DeviceArray(np.zeros((n, 3)))
     for s in range(TRAIN):
-        st = A.mdf_device(n, tx.data_ptr(), tV.data_ptr(), None, fb.data_ptr(), positions_unchanged=s > 0,
-                          features_ptr=feats[s * n].data_ptr(), targets_ptr=targs[s * n].data_ptr(), **KW)
+        st = A.mdf_device(n, tx.addr, tV.addr, None, fb.addr, positions_unchanged=s > 0,
+                          features_ptr=feats.addr + s * n * 67 * 4, targets_ptr=targs.addr + s * n * 3 * 4,
+                          **KW)
         assert st["converged"]
         A.step((0.0, 0.0, 0.0), 0)
     A.sync()
     net = couplb_amd.Mlp(seed=11)
-    l0, _ = net.gradient_device(len(feats), feats.data_ptr(), targs.data_ptr())
-    net.train_device(len(feats), feats.data_ptr(), targs.data_ptr(), batch=4096, epochs=8, lr=2e-3, seed=5)
-    l1, _ = net.gradient_device(len(feats), feats.data_ptr(), targs.data_ptr())
-    print(f"  network loss {l0:.4g} -> {l1:.4g} on {len(feats)} samples", flush=True)
+    l0, _ = net.gradient_device(TRAIN * n, feats.addr, targs.addr)
+    net.train_device(TRAIN * n, feats.addr, targs.addr, batch=4096, epochs=8, lr=2e-3, seed=5)
+    l1, _ = net.gradient_device(TRAIN * n, feats.addr, targs.addr)
+    print(f"  network loss {l0:.4g} -> {l1:.4g} on {TRAIN * n} samples", flush=True)
     assert l1 < l0
     start = A.get_state()
     A.close()
@@ -92,20 +93,19 @@ def test_c5_mdf_with_the_trained_network():
         print("case C5 held-out steps:", mode, storage, flush=True)
         d = device(storage)
         d.set_state(start)
-        f = torch.zeros((n, 3), dtype=torch.float64, device=cuda)
-        fprev = torch.zeros_like(f)
+        f = DeviceArray(np.zeros((n, 3)))
+        fprev = DeviceArray(np.zeros((n, 3)))
         passes, totals = [], []
         for s in range(HELD):
-            st = d.mdf_device(n, tx.data_ptr(), tV.data_ptr(),
-                              fprev.data_ptr() if (mode == "previous" and s > 0) else None, f.data_ptr(),
-                              positions_unchanged=s > 0, warm_start=net if mode == "network" else None, **KW)
+            st = d.mdf_device(n, tx.addr, tV.addr, fprev.addr if (mode == "previous" and s > 0) else None,
+                              f.addr, positions_unchanged=s > 0,
+                              warm_start=net if mode == "network" else None, **KW)
             d.sync()
             assert st["converged"] and st["slip_last"] < 1e-6
             passes.append(st["iterations"])
-            fh = f.cpu().numpy()
+            fh = f.download()
             totals.append(fh.reshape(64, ns, 3).sum(1))
-            fprev.copy_(-f)
-            torch.cuda.synchronize()
+            fprev.upload(-fh)          # F (lattice units) = -f_body with force_scale 1
             d.step((0.0, 0.0, 0.0), 0)
         d.sync()
         u = np.stack(d.macro()[1:])

@@ -1,6 +1,6 @@
 """Edge cases of the device IBM entry points: empty marker sets, markers
 that cannot be placed (non-finite or far outside the box), rows longer than
-the IBM tile limit, and misuse of the multi-direct-forcing call."""
+1024 nodes, misuse of the multi-direct-forcing call, and density clamps."""
 import numpy as np
 import pytest
 
@@ -48,15 +48,30 @@ def test_unplaceable_markers_get_the_pure_damping_force():
     assert np.array_equal(d.get_state()[:, 1:-1, 1:-1, 1:-1], g.get_state()[:, 1:-1, 1:-1, 1:-1])
 
 
-def test_rows_longer_than_the_ibm_tile_limit_are_rejected():
-    import couplb_amd
-    d = couplb_amd.Device(3, 1100, 4, 4, tau=0.8)
-    d.init_equilibrium(1.0, (0.0, 0.0, 0.0))
-    x = np.array([[10.5, 2.0, 2.0]])
-    with pytest.raises(couplb_amd.CoupLBError, match="1024"):
-        d.ibm(x, np.zeros((1, 3)), np.ones(1), np.zeros((1, 3)), **KW)
-    d.step((0.0, 0.0, 0.0), 0)      # the LB step itself has no such limit
-    d.sync()
+def test_rows_longer_than_1024_nodes_match_the_oracle():
+    # the reference's IBM has no row-length limit (couplb_ibm.h:57-133);
+    # markers at both x ends (periodic fold) and in the middle of 1100-node rows
+    from tests.scenarios import atoms_from, interior, rel_l2, run_device, run_oracle
+    rng = np.random.default_rng(8)
+    xs = np.concatenate([rng.uniform(0.0, 3.0, 6), rng.uniform(540.0, 560.0, 6), rng.uniform(1097.0, 1100.0, 6)])
+    x = np.stack([xs, rng.uniform(0.5, 5.5, 18), rng.uniform(0.5, 4.5, 18)], 1)
+    base = atoms_from(x, rng.normal(0, 1e-3, (18, 3)), np.ones(18))
+
+    def atoms(step):
+        a = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in base.items()}
+        a["x"] = a["x"] + a["v"] * 0.5 * step
+        return a
+
+    cmd = "long all couplb 1100 6 5 0.1 1.0 md_per_lb 2 kernel peskin4 gravity 1e-5 0.0 0.0"
+    print("case IBM on 1100-node rows", flush=True)
+    dev, fd = run_device(cmd, 6, dimension=3, dt=0.5, atoms_fn=atoms)
+    ref, fo = run_oracle(cmd, 6, dimension=3, dt=0.5, atoms_fn=atoms)
+    a = interior(dev.device_state().reshape(19, 7, 8, 1102), 3)
+    b = interior(ref.f(), 3)
+    err = rel_l2(a, b)
+    ferr = max(rel_l2(p, q) for p, q in zip(fd, fo))
+    print(f"  rel-L2(f) {err:.3e} rel-L2(F_atom) {ferr:.3e}")
+    assert err <= 1e-12 and ferr <= 1e-11
 
 
 def test_mdf_must_be_the_only_ibm_call_of_an_interval():

@@ -233,3 +233,30 @@ def test_group_growing_within_an_interval_matches_oracle():
     ferr = max(rel_l2(p, q) for p, q in zip(fd, fo))
     print(f"rel-L2(f) = {err:.3e}, rel-L2(F_atom) = {ferr:.3e}")
     assert err <= TOL and ferr <= 1e-11
+
+
+@pytest.mark.parametrize("md", [20, 37])
+def test_more_sub_steps_than_record_slots_match_oracle(md):
+    """md_per_lb above the 16 record slots (fix_couplb.cpp:111-114 accepts
+    any md_per_lb >= 1): with moving markers every sub-step records, the
+    full slots are spread into the node forces and reused."""
+    rng = np.random.default_rng(md)
+    n = 30
+    base = atoms_from(rng.uniform(0.0, 12.0, (n, 3)), rng.normal(0.0, 1e-3, (n, 3)), rng.uniform(0.5, 1.5, n))
+
+    def atoms(step):
+        a = copy_atoms(base)
+        a["x"] = a["x"] + 0.2 * np.sin(0.05 * step + np.arange(n))[:, None]
+        return a
+
+    cmd = f"many all couplb 12 12 12 0.1 1.0 md_per_lb {md} kernel roma wall_z 1 1 gravity 1e-5 0.0 0.0"
+    print("case md_per_lb", md, flush=True)
+    steps = 2 * md + 3
+    dev, fd = run_device(cmd, steps, dimension=3, dt=1.0 / md, atoms_fn=atoms)
+    ref, fo = run_oracle(cmd, steps, dimension=3, dt=1.0 / md, atoms_fn=atoms)
+    a = interior(dev.device_state().reshape(19, 14, 14, 14), 3)
+    b = interior(ref.f(), 3)
+    err = rel_l2(a, b)
+    ferr = max(rel_l2(p, q) for p, q in zip(fd, fo))
+    print(f"rel-L2(f) = {err:.3e}, rel-L2(F_atom) = {ferr:.3e}")
+    assert err <= TOL and ferr <= 1e-11

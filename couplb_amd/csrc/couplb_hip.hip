@@ -1,6 +1,7 @@
 // couplb_hip.hip -- device context and C ABI (include/couplb_hip.h).
 #include <cstring>
 #include <algorithm>
+#include <functional>
 #include <mutex>
 #include <cmath>
 #include <cstdio>
@@ -125,7 +126,6 @@ struct couplb_ctx {
   void* sort_tmp = nullptr;
   size_t sort_tmp_bytes = 0;
   double *dx = nullptr, *dv = nullptr, *dm = nullptr, *df = nullptr;
-  int* moved = nullptr;  // count + list of markers whose base cell moved (cap + 1)
   int order_n = -1;       // marker count of the cached ordering
   double order_key[5] = {0, 0, 0, 0, -1};  // lo, inv_dx, kernel of the cached ordering
   int pos_epoch = 0;      // incremented whenever the ordering is rebuilt
@@ -150,6 +150,23 @@ struct couplb_ctx {
   size_t seltmp_bytes = 0;
   double* fsrc = nullptr;          // record slot 0's force per sorted marker (3*cap)
   int prev_step_epoch = -1;  // ordering of the single record slot at the previous LB step
+  // moving-marker map (spread_pairs.hpp k_pairs_base): the same pair arrays
+  // built from the ordering's base cells, the per-step force of every
+  // (marker, stencil slot), and the spreading of the records that left their
+  // base cell (F4 where flag2 is set)
+  int mpair_epoch = -1;      // ordering the moving-marker map was built for
+  double* pcon = nullptr;
+  long pcon_cap = 0;
+  unsigned char* flag2 = nullptr;
+  unsigned char* tflag = nullptr;  // tiles reached by records that left their cell
+  int* tlist = nullptr;            // their list, then its length
+  int ntiles = 0;
+  int order_age = 0;         // LB intervals since the marker ordering was built
+  int reorder_every = 8;     // moving markers keep an ordering this many intervals
+  int kept_move = 0;         // positions changed on the kept ordering since it was built
+  double* vcache = nullptr;  // stencil velocities of moving markers (k_marker VelCache)
+  int4* vbase = nullptr;
+  int vcache_cap = 0;
   // multi-direct forcing (mdf.hpp)
   double* ut = nullptr;            // operator node values at pair nodes, 3 per node
   double* mdf_v = nullptr;         // target velocity per sorted marker (3*cap)
@@ -178,15 +195,15 @@ struct couplb_ctx {
   int nslots = 0;
   int f4_partial = 0;  // F4 / flag already hold the spread records of earlier
                        // sub-steps of this interval (slots were full)
-  // merged, sorted record keys of the pending slots (spread_rows.hpp)
-  unsigned long long *mkey = nullptr, *mkey_s = nullptr;
-  unsigned *mval = nullptr, *mval_s = nullptr;
-  int* mrow = nullptr;
-  void* msort_tmp = nullptr;
-  size_t msort_bytes = 0;
-  long mcap = 0;
-  Slot* slot_table = nullptr;  // device copy of the slot descriptors
-  int slot_table_n = -1;       // descriptors currently uploaded
+  // binned records of the pending slots (spread_rows.hpp)
+  int *sb_cnt = nullptr, *sb_start = nullptr;   // bucket counts / starts, nb + 1 each
+  int *sb_bk = nullptr, *sb_rank = nullptr;     // bucket and arrival rank per record
+  unsigned long long *sb_ent = nullptr, *sb_ent2 = nullptr;  // binned records (spread_rows.hpp entry_of)
+  int* sb_large = nullptr;                      // buckets sorted by k_bucket_sort_large, + count
+  void* sb_scan_tmp = nullptr;
+  size_t sb_scan_bytes = 0;
+  long sb_cap = 0;                              // records the per-record arrays hold
+  int sb_nb = 0;                                // buckets
   double* staging = nullptr;  // ghost-inclusive FP64 state for import/export
   size_t staging_len = 0;
   double* scratch = nullptr;  // this device's `scratch` (couplb_diagnostics_noalloc)
@@ -317,13 +334,15 @@ template <int D, typename S> struct LaunchStep {
         // reduced formats: 16-byte staged row groups (k_step_lds)
         const dim3 grid((unsigned)((c->geo.nx + XC - 1) / XC), (unsigned)c->geo.ny,
                         (unsigned)(c->D == 3 ? c->geo.nz : 1));
-        if (a.pair_mode) hipLaunchKernelGGL((k_step_lds<D, S, true>), grid, dim3(XC), 0, c->stream, a);
+        if (a.pair_mode == 1) hipLaunchKernelGGL((k_step_lds<D, S, 1>), grid, dim3(XC), 0, c->stream, a);
+        else if (a.pair_mode == 2) hipLaunchKernelGGL((k_step_lds<D, S, 2>), grid, dim3(XC), 0, c->stream, a);
         else hipLaunchKernelGGL((k_step_lds<D, S>), grid, dim3(XC), 0, c->stream, a);
         return;
       }
     }
     const dim3 b = row_block(c->geo.nx);
-    if (a.pair_mode) hipLaunchKernelGGL((k_step<D, S, true>), row_grid(c->geo, b.x), b, 0, c->stream, a);
+    if (a.pair_mode == 1) hipLaunchKernelGGL((k_step<D, S, 1>), row_grid(c->geo, b.x), b, 0, c->stream, a);
+    else if (a.pair_mode == 2) hipLaunchKernelGGL((k_step<D, S, 2>), row_grid(c->geo, b.x), b, 0, c->stream, a);
     else hipLaunchKernelGGL((k_step<D, S>), row_grid(c->geo, b.x), b, 0, c->stream, a);
   }
 };
@@ -336,13 +355,9 @@ template <int D, typename S> struct LaunchMacro {
 };
 template <int D, typename S> struct LaunchFixup {
   static void run(couplb_ctx* c, int n, const double* x, const IbmParams& ip, const PullCtx& pc) {
-    const int tb = 256;
-    int* count = c->moved;         // moved[0]: count, moved + 1: list
-    (void)hipMemsetAsync(count, 0, sizeof(int), c->stream);
-    hipLaunchKernelGGL(k_moved_markers, dim3((n + tb - 1) / tb), dim3(tb), 0, c->stream, n, c->perm, x, ip,
-                       c->geo, (const Base*)c->bases, count, count + 1);
-    hipLaunchKernelGGL((k_moved_cover<D, S>), dim3(MOVED_BLOCKS), dim3(tb), 0, c->stream, c->perm, x, ip, pc,
-                       (const int*)count, (const int*)(count + 1), c->u4, c->stamp, c->cycle);
+    const int tb = 256;  // one lane per marker
+    hipLaunchKernelGGL((k_moved_fix<D, S>), dim3((unsigned)((n + tb - 1) / tb)), dim3(tb), 0, c->stream, n,
+                       c->perm, x, ip, pc, (const Base*)c->bases, c->u4, c->stamp, c->cycle);
   }
 };
 template <int D, typename S> struct LaunchMacroParts {
@@ -389,7 +404,8 @@ template <int D, typename S> struct LaunchUgather {
     // than a quarter of the nodes (the covered-node count of the fixed-marker
     // map, when it belongs to this ordering); measured faster for 16-bit
     // storage and slower for FP64
-    const bool dense = sizeof(S) < 8 && c->pair_epoch == c->pos_epoch && 4 * (size_t)c->pnh > c->N;
+    const bool dense = sizeof(S) < 8 && (c->pair_epoch == c->pos_epoch || c->mpair_epoch == c->pos_epoch) &&
+                       4 * (size_t)c->pnh > c->N;
     constexpr int G = LDS_GB / (int)sizeof(S), XC = lds_chunk<S>();
     const bool staged = !pc.local && c->geo.nx % G == 0 && c->lay.pq % G == 0 && c->lay.pr % G == 0;
     if (dense && staged) {
@@ -467,9 +483,6 @@ int ensure_ibm(couplb_ctx* c, int n) {
   HIPCHK(c, hipMalloc(&c->dm, sizeof(double) * cap));
   HIPCHK(c, hipMalloc(&c->df, sizeof(double) * 3 * cap));
   HIPCHK(c, hipMalloc(&c->ufc, sizeof(double) * 3 * cap));
-  (void)hipFree(c->moved);
-  c->moved = nullptr;
-  HIPCHK(c, hipMalloc(&c->moved, sizeof(int) * ((size_t)cap + 1)));
   (void)hipFree(c->fsrc);
   c->fsrc = nullptr;
   HIPCHK(c, hipMalloc(&c->fsrc, sizeof(double) * 3 * cap));
@@ -498,7 +511,6 @@ int ensure_ibm(couplb_ctx* c, int n) {
   c->cap = cap;
   c->order_n = -1;
   c->uf_epoch = -1;
-  c->slot_table_n = -1;
   return COUPLB_OK;
 }
 
@@ -548,6 +560,10 @@ int prepare_markers(couplb_ctx* c, int n, const double* x, const IbmParams& ip,
     c->order_n = n;
     std::memcpy(c->order_key, key, sizeof(key));
     c->pos_epoch++;
+    c->order_age = 0;
+    c->kept_move = 0;
+  } else if (!positions_unchanged) {
+    c->kept_move = 1;  // positions changed on the kept ordering
   }
   if (reorder || c->ugather_cycle != c->cycle) {
     // coverage depends on the marker cells only, the velocities on the state
@@ -587,9 +603,13 @@ int ibm_impl(couplb_ctx* c, int n, const double* x, const double* v, const doubl
   rc = timer_begin(c, 1, &tslot);
   if (rc) return rc;
   bool reorder = false;
-  // a later sub-step of the interval keeps the interval's marker ordering
+  // a later sub-step of the interval keeps the interval's marker ordering;
+  // moving markers keep it over reorder_every intervals once the LB step has
+  // built its moving-marker map (markers that leave their base cell are
+  // spread apart until the next ordering)
   const bool later = c->nslots > 0 || c->f4_partial;
-  rc = prepare_markers(c, n, x, ip, a->positions_unchanged, later, &reorder);
+  const bool keep = later || (c->mpair_epoch == c->pos_epoch && c->order_age < c->reorder_every);
+  rc = prepare_markers(c, n, x, ip, a->positions_unchanged, keep, &reorder);
   if (rc) return rc;
   // Same positions and same LB state as the previous sub-step of this
   // interval: u_f is known and only the force is new (k_marker_force).
@@ -614,13 +634,32 @@ int ibm_impl(couplb_ctx* c, int n, const double* x, const double* v, const doubl
   if (rc) return rc;
   if (c->slots[s].rec == nullptr || c->cap < n) return fail(c, COUPLB_ENOMEM, "slot capacity");
   SlotHost& sl = c->slots[s];
+  // markers that move on a kept ordering: stencil velocities cached for the
+  // later sub-steps of the state (k_marker VelCache)
+  VelCache vc{nullptr, nullptr, 0, c->cycle};
+  if (c->kept_move) {
+    const int lpm = (c->D == 3) ? 4 : 1;
+    if (c->vcache_cap < c->cap) {
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      (void)hipFree(c->vcache); (void)hipFree(c->vbase);
+      c->vcache = nullptr; c->vbase = nullptr; c->vcache_cap = 0;
+      HIPCHK(c, hipMalloc(&c->vcache, sizeof(double) * 48 * lpm * (size_t)c->cap));
+      HIPCHK(c, hipMalloc(&c->vbase, sizeof(int4) * (size_t)c->cap));
+      // no base cell matches state -1
+      HIPCHK(c, hipMemsetAsync(c->vbase, 0xFF, sizeof(int4) * (size_t)c->cap, c->stream));
+      c->vcache_cap = c->cap;
+    }
+    vc.u = c->vcache;
+    vc.base = c->vbase;
+    vc.L = (long)lpm * c->vcache_cap;
+  }
   if (c->D == 3) {
     const int nb4 = (4 * n + tb - 1) / tb;  // four lanes per marker
     hipLaunchKernelGGL((k_marker<3>), dim3(nb4), dim3(tb), 0, c->stream, n, c->perm, x, v, m, f_dev,
-                       fmode, ip, c->geo, c->walls, c->u4, sl.rec, c->ufc, s == 0 ? c->fsrc : nullptr);
+                       fmode, ip, c->geo, c->walls, c->u4, sl.rec, c->ufc, s == 0 ? c->fsrc : nullptr, vc);
   } else {
     hipLaunchKernelGGL((k_marker<2>), dim3(nb), dim3(tb), 0, c->stream, n, c->perm, x, v, m, f_dev,
-                       fmode, ip, c->geo, c->walls, c->u4, sl.rec, c->ufc, s == 0 ? c->fsrc : nullptr);
+                       fmode, ip, c->geo, c->walls, c->u4, sl.rec, c->ufc, s == 0 ? c->fsrc : nullptr, vc);
   }
   HIPCHK(c, hipGetLastError());
   rc = timer_end(c, tslot);
@@ -634,54 +673,36 @@ int ibm_impl(couplb_ctx* c, int n, const double* x, const double* v, const doubl
   return COUPLB_OK;
 }
 
-// Sparse spreading map of the single record slot (spread_pairs.hpp); also
-// rewrites every node flag.
-int build_pairs(couplb_ctx* c, const SlotHost& sl) {
-  const int S = (c->D == 3) ? stencil_size<3>() : stencil_size<2>();
-  const long ne = (long)sl.n * S;
-  const unsigned sentinel = (unsigned)c->N;
-  const unsigned bits = key_bits(c->N);
-  if (ne > c->pcap) {
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    (void)hipFree(c->pkey); (void)hipFree(c->pkey_s); (void)hipFree(c->pval); (void)hipFree(c->pval_s);
-    (void)hipFree(c->pp); (void)hipFree(c->pw); (void)hipFree(c->ptmp);
-    c->pkey = c->pkey_s = nullptr; c->pval = c->pval_s = c->pp = nullptr; c->pw = nullptr; c->ptmp = nullptr;
-    c->pcap = 0;
-    HIPCHK(c, hipMalloc(&c->pkey, sizeof(unsigned) * ne));
-    HIPCHK(c, hipMalloc(&c->pkey_s, sizeof(unsigned) * ne));
-    HIPCHK(c, hipMalloc(&c->pval, sizeof(int) * ne));
-    HIPCHK(c, hipMalloc(&c->pval_s, sizeof(int) * ne));
-    HIPCHK(c, hipMalloc(&c->pp, sizeof(int) * ne));
-    HIPCHK(c, hipMalloc(&c->pw, sizeof(double) * ne));
-    size_t bytes = 0;
-    HIPCHK(c, rocprim::radix_sort_pairs(nullptr, bytes, c->pkey, c->pkey_s, c->pval, c->pval_s,
-                                        (size_t)ne, 0u, bits, c->stream));
-    HIPCHK(c, hipMalloc(&c->ptmp, bytes));
-    c->ptmp_bytes = bytes;
-    c->pcap = ne;
-  }
+// Pair arrays of ne (marker, stencil slot) pairs: keys, values and the
+// sort's scratch.
+int ensure_pairs(couplb_ctx* c, long ne) {
+  if (ne <= c->pcap) return COUPLB_OK;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  (void)hipFree(c->pkey); (void)hipFree(c->pkey_s); (void)hipFree(c->pval); (void)hipFree(c->pval_s);
+  (void)hipFree(c->pp); (void)hipFree(c->pw); (void)hipFree(c->ptmp);
+  c->pkey = c->pkey_s = nullptr; c->pval = c->pval_s = c->pp = nullptr; c->pw = nullptr; c->ptmp = nullptr;
+  c->pcap = 0;
+  HIPCHK(c, hipMalloc(&c->pkey, sizeof(unsigned) * ne));
+  HIPCHK(c, hipMalloc(&c->pkey_s, sizeof(unsigned) * ne));
+  HIPCHK(c, hipMalloc(&c->pval, sizeof(int) * ne));
+  HIPCHK(c, hipMalloc(&c->pval_s, sizeof(int) * ne));
+  HIPCHK(c, hipMalloc(&c->pp, sizeof(int) * ne));
+  HIPCHK(c, hipMalloc(&c->pw, sizeof(double) * ne));
+  size_t bytes = 0;
+  HIPCHK(c, rocprim::radix_sort_pairs(nullptr, bytes, c->pkey, c->pkey_s, c->pval, c->pval_s, (size_t)ne, 0u,
+                                      key_bits(c->N), c->stream));
+  HIPCHK(c, hipMalloc(&c->ptmp, bytes));
+  c->ptmp_bytes = bytes;
+  c->pcap = ne;
+  return COUPLB_OK;
+}
+
+// Node runs of the sorted pairs: first pair of each run, its length and
+// node, and the per-node run index prun.
+int build_runs(couplb_ctx* c, long ne) {
   const int tb = 256;
   const unsigned nb = (unsigned)((ne + tb - 1) / tb);
-  if (c->D == 3)
-    hipLaunchKernelGGL(k_pairs<3>, dim3(nb), dim3(tb), 0, c->stream, sl.n, c->geo, c->walls, sl.rec,
-                       sentinel, c->pkey, c->pval);
-  else
-    hipLaunchKernelGGL(k_pairs<2>, dim3(nb), dim3(tb), 0, c->stream, sl.n, c->geo, c->walls, sl.rec,
-                       sentinel, c->pkey, c->pval);
-  HIPCHK(c, hipGetLastError());
-  size_t bytes = c->ptmp_bytes;
-  HIPCHK(c, rocprim::radix_sort_pairs(c->ptmp, bytes, c->pkey, c->pkey_s, c->pval, c->pval_s,
-                                      (size_t)ne, 0u, bits, c->stream));
-  hipLaunchKernelGGL(k_fill_u8, dim3(fill_u8_blocks(c->N)), dim3(256), 0, c->stream, c->flag,
-                     c->N, (unsigned char)0);
-  if (c->D == 3)
-    hipLaunchKernelGGL(k_pair_weights<3>, dim3(nb), dim3(tb), 0, c->stream, ne, c->pkey_s, c->pval_s,
-                       sentinel, sl.rec, c->pp, c->pw, c->flag);
-  else
-    hipLaunchKernelGGL(k_pair_weights<2>, dim3(nb), dim3(tb), 0, c->stream, ne, c->pkey_s, c->pval_s,
-                       sentinel, sl.rec, c->pp, c->pw, c->flag);
-  HIPCHK(c, hipGetLastError());
-  // node runs: first pair of each run, its length and node
+  const unsigned sentinel = (unsigned)c->N;
   if (ne > c->pcap_runs) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     (void)hipFree(c->pheads); (void)hipFree(c->phlen); (void)hipFree(c->phnode); (void)hipFree(c->phflag);
@@ -719,7 +740,84 @@ int build_pairs(couplb_ctx* c, const SlotHost& sl) {
   HIPCHK(c, hipGetLastError());
   c->pnh = nh;
   c->pne = ne;
+  return COUPLB_OK;
+}
+
+// Sparse spreading map of the single record slot (spread_pairs.hpp); also
+// rewrites every node flag.
+int build_pairs(couplb_ctx* c, const SlotHost& sl) {
+  const int S = (c->D == 3) ? stencil_size<3>() : stencil_size<2>();
+  const long ne = (long)sl.n * S;
+  const unsigned sentinel = (unsigned)c->N;
+  const unsigned bits = key_bits(c->N);
+  int rc = ensure_pairs(c, ne);
+  if (rc) return rc;
+  const int tb = 256;
+  const unsigned nb = (unsigned)((ne + tb - 1) / tb);
+  if (c->D == 3)
+    hipLaunchKernelGGL(k_pairs<3>, dim3(nb), dim3(tb), 0, c->stream, sl.n, c->geo, c->walls, sl.rec,
+                       sentinel, c->pkey, c->pval);
+  else
+    hipLaunchKernelGGL(k_pairs<2>, dim3(nb), dim3(tb), 0, c->stream, sl.n, c->geo, c->walls, sl.rec,
+                       sentinel, c->pkey, c->pval);
+  HIPCHK(c, hipGetLastError());
+  size_t bytes = c->ptmp_bytes;
+  HIPCHK(c, rocprim::radix_sort_pairs(c->ptmp, bytes, c->pkey, c->pkey_s, c->pval, c->pval_s,
+                                      (size_t)ne, 0u, bits, c->stream));
+  hipLaunchKernelGGL(k_fill_u8, dim3(fill_u8_blocks(c->N)), dim3(256), 0, c->stream, c->flag,
+                     c->N, (unsigned char)0);
+  if (c->D == 3)
+    hipLaunchKernelGGL(k_pair_weights<3>, dim3(nb), dim3(tb), 0, c->stream, ne, c->pkey_s, c->pval_s,
+                       sentinel, sl.rec, c->pp, c->pw, c->flag);
+  else
+    hipLaunchKernelGGL(k_pair_weights<2>, dim3(nb), dim3(tb), 0, c->stream, ne, c->pkey_s, c->pval_s,
+                       sentinel, sl.rec, c->pp, c->pw, c->flag);
+  HIPCHK(c, hipGetLastError());
+  rc = build_runs(c, ne);
+  if (rc) return rc;
   c->pair_epoch = sl.epoch;
+  c->mpair_epoch = -1;  // the pair arrays and node flags now hold this map
+  return COUPLB_OK;
+}
+
+// The moving-marker map of the current ordering (spread_pairs.hpp
+// k_pairs_base): pairs from the ordering's base cells and their runs;
+// rewrites every node flag.
+int build_moving_map(couplb_ctx* c, int n) {
+  const int S = (c->D == 3) ? stencil_size<3>() : stencil_size<2>();
+  const long ne = (long)n * S;
+  const unsigned sentinel = (unsigned)c->N;
+  int rc = ensure_pairs(c, ne);
+  if (rc) return rc;
+  if (ne > c->pcon_cap) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    (void)hipFree(c->pcon);
+    c->pcon = nullptr;
+    c->pcon_cap = 0;
+    HIPCHK(c, hipMalloc(&c->pcon, sizeof(double) * 4 * ne));
+    c->pcon_cap = ne;
+  }
+  const int tb = 256;
+  const unsigned nb = (unsigned)((ne + tb - 1) / tb);
+  if (c->D == 3)
+    hipLaunchKernelGGL(k_pairs_base<3>, dim3(nb), dim3(tb), 0, c->stream, n, c->geo, c->walls,
+                       (const Base*)c->bases, sentinel, c->pkey, c->pval);
+  else
+    hipLaunchKernelGGL(k_pairs_base<2>, dim3(nb), dim3(tb), 0, c->stream, n, c->geo, c->walls,
+                       (const Base*)c->bases, sentinel, c->pkey, c->pval);
+  HIPCHK(c, hipGetLastError());
+  size_t bytes = c->ptmp_bytes;
+  HIPCHK(c, rocprim::radix_sort_pairs(c->ptmp, bytes, c->pkey, c->pkey_s, c->pval, c->pval_s, (size_t)ne, 0u,
+                                      key_bits(c->N), c->stream));
+  hipLaunchKernelGGL(k_fill_u8, dim3(fill_u8_blocks(c->N)), dim3(256), 0, c->stream, c->flag, c->N,
+                     (unsigned char)0);
+  hipLaunchKernelGGL(k_map_flags, dim3(nb), dim3(tb), 0, c->stream, ne, (const unsigned*)c->pkey_s, sentinel,
+                     c->flag);
+  HIPCHK(c, hipGetLastError());
+  rc = build_runs(c, ne);
+  if (rc) return rc;
+  c->mpair_epoch = c->pos_epoch;
+  c->pair_epoch = -1;  // the pair arrays and node flags now hold this map
   return COUPLB_OK;
 }
 
@@ -1107,83 +1205,146 @@ int mdf_impl(couplb_ctx* c, int n, const double* x, const double* V, const doubl
 }
 
 // The pending record slots spread into F4 / flag (spread_rows.hpp): the
-// records' base-cell keys of all slots merged and radix sorted, a row table,
-// then one wavefront per 64-node row segment.  fold: add the periodic ghost
-// images into the interior; accumulate: add to the forces F4 / flag hold.
-int row_spread(couplb_ctx* c, double* F4, unsigned char* flag, int fold, int accumulate) {
-  long total = 0;
-  for (int s = 0; s < c->nslots; s++) total += c->slots[s].n;
-  const unsigned long long xw = spread_xw(c->geo);
-  const unsigned long long sentinel = (unsigned long long)c->geo.nrows * xw;
-  unsigned bits = 1;
-  while (bits < 64 && (1ull << bits) <= sentinel) bits++;
-  if (total > c->mcap) {
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    (void)hipFree(c->mkey); (void)hipFree(c->mkey_s); (void)hipFree(c->mval); (void)hipFree(c->mval_s);
-    (void)hipFree(c->msort_tmp);
-    c->mkey = c->mkey_s = nullptr; c->mval = c->mval_s = nullptr; c->msort_tmp = nullptr; c->mcap = 0;
-    const long cap = std::max(total, 2 * c->mcap);
-    HIPCHK(c, hipMalloc(&c->mkey, sizeof(unsigned long long) * cap));
-    HIPCHK(c, hipMalloc(&c->mkey_s, sizeof(unsigned long long) * cap));
-    HIPCHK(c, hipMalloc(&c->mval, sizeof(unsigned) * cap));
-    HIPCHK(c, hipMalloc(&c->mval_s, sizeof(unsigned) * cap));
-    size_t bytes = 0;
-    HIPCHK(c, rocprim::radix_sort_pairs(nullptr, bytes, c->mkey, c->mkey_s, c->mval, c->mval_s, (size_t)cap,
-                                        0u, 64u, c->stream));
-    HIPCHK(c, hipMalloc(&c->msort_tmp, bytes));
-    c->msort_bytes = bytes;
-    c->mcap = cap;
+// records of all slots binned by base row and i0 segment (counting sort,
+// each bucket ordered by slot and marker), then one wavefront per tile of
+// nodes.  fold: add the periodic ghost images into the interior; accumulate:
+// add to the forces F4 / flag hold.  moved_only: only the records that left
+// their ordered base cell (the moving-marker map holds the others), only the
+// tiles they reach, F4 and flag written only where they add a force.
+// bin: when given, launches pass 1 itself (spread_rows.hpp bin_record for
+// every record) in place of k_rec_count.
+int row_spread(couplb_ctx* c, double* F4, unsigned char* flag, int fold, int accumulate,
+               bool moved_only = false,
+               const std::function<int(const SlotSpan&, const RecBin&)>* bin = nullptr) {
+  SlotSpan ss;
+  ss.bases = moved_only ? (const Base*)c->bases : nullptr;
+  ss.nslots = c->nslots;
+  ss.off[0] = 0;
+  for (int s = 0; s < c->nslots; s++) {
+    ss.rec[s] = c->slots[s].rec;
+    ss.off[s + 1] = ss.off[s] + std::max(0, c->slots[s].n);
   }
-  if (!c->mrow) HIPCHK(c, hipMalloc(&c->mrow, sizeof(int) * (c->geo.nrows + 1)));
-  if (!c->slot_table) HIPCHK(c, hipMalloc(&c->slot_table, sizeof(Slot) * MAX_SLOTS));
-  if (c->slot_table_n != (int)c->slots.size()) {
-    Slot h[MAX_SLOTS];
-    for (size_t s = 0; s < c->slots.size(); s++) h[s].rec = c->slots[s].rec;
-    HIPCHK(c, hipMemcpy(c->slot_table, h, sizeof(Slot) * c->slots.size(), hipMemcpyHostToDevice));
-    c->slot_table_n = (int)c->slots.size();
+  for (int s = c->nslots; s < MAX_SLOTS; s++) ss.rec[s] = nullptr;
+  const long total = ss.off[c->nslots];
+  const int nb = c->geo.nrows * spread_nxb(c->geo);
+  if (nb != c->sb_nb) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    (void)hipFree(c->sb_cnt); (void)hipFree(c->sb_start); (void)hipFree(c->sb_scan_tmp);
+    c->sb_cnt = c->sb_start = nullptr;
+    c->sb_scan_tmp = nullptr;
+    c->sb_nb = 0;
+    HIPCHK(c, hipMalloc(&c->sb_cnt, sizeof(int) * ((size_t)nb + 1)));
+    HIPCHK(c, hipMalloc(&c->sb_start, sizeof(int) * ((size_t)nb + 1)));
+    size_t bytes = 0;
+    HIPCHK(c, rocprim::exclusive_scan(nullptr, bytes, c->sb_cnt, c->sb_start, 0, (size_t)nb + 1,
+                                      rocprim::plus<int>(), c->stream));
+    HIPCHK(c, hipMalloc(&c->sb_scan_tmp, bytes));
+    c->sb_scan_bytes = bytes;
+    c->sb_nb = nb;
+  }
+  if (total > c->sb_cap || c->sb_cap == 0) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    (void)hipFree(c->sb_bk); (void)hipFree(c->sb_rank); (void)hipFree(c->sb_ent); (void)hipFree(c->sb_ent2);
+    (void)hipFree(c->sb_large);
+    c->sb_bk = c->sb_rank = c->sb_large = nullptr;
+    c->sb_ent = c->sb_ent2 = nullptr;
+    const long cap = std::max(std::max(total, 2 * c->sb_cap), 64L);
+    c->sb_cap = 0;
+    HIPCHK(c, hipMalloc(&c->sb_bk, sizeof(int) * cap));
+    HIPCHK(c, hipMalloc(&c->sb_rank, sizeof(int) * cap));
+    HIPCHK(c, hipMalloc(&c->sb_ent, sizeof(unsigned long long) * cap));
+    HIPCHK(c, hipMalloc(&c->sb_ent2, sizeof(unsigned long long) * cap));
+    // at most cap / (SB_SMALL + 1) buckets are large; their count follows
+    HIPCHK(c, hipMalloc(&c->sb_large, sizeof(int) * ((size_t)cap / (SB_SMALL + 1) + 2)));
+    c->sb_cap = cap;
   }
   const int tb = 256;
-  long off = 0;
-  for (int s = 0; s < c->nslots; s++) {
-    const int n = c->slots[s].n;
-    if (n <= 0) continue;
-    if (c->D == 3)
-      hipLaunchKernelGGL(k_rec_keys<3>, dim3((n + tb - 1) / tb), dim3(tb), 0, c->stream, n,
-                         (const MarkerRec*)c->slots[s].rec, c->geo, (unsigned)s, sentinel, c->mkey + off,
-                         c->mval + off);
-    else
-      hipLaunchKernelGGL(k_rec_keys<2>, dim3((n + tb - 1) / tb), dim3(tb), 0, c->stream, n,
-                         (const MarkerRec*)c->slots[s].rec, c->geo, (unsigned)s, sentinel, c->mkey + off,
-                         c->mval + off);
-    off += n;
-  }
+  int* nlarge = c->sb_large + ((size_t)c->sb_cap / (SB_SMALL + 1) + 1);
+  hipLaunchKernelGGL(k_fill_int, dim3((unsigned)(((size_t)nb + 1 + tb - 1) / tb)), dim3(tb), 0, c->stream, c->sb_cnt,
+                     (size_t)nb + 1, 0);
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, nlarge, (size_t)1, 0);
   HIPCHK(c, hipGetLastError());
-  if (total > 0) {
-    size_t bytes = c->msort_bytes;
-    HIPCHK(c, rocprim::radix_sort_pairs(c->msort_tmp, bytes, c->mkey, c->mkey_s, c->mval, c->mval_s,
-                                        (size_t)total, 0u, bits, c->stream));
+  // tiles of the spreading (TileShape) and, for moved_only, their marks
+  int ntx, nty, ntz;
+  if (c->D == 3) {
+    typedef TileShape<3> T;
+    ntx = (c->geo.nx + T::X - 1) / T::X; nty = (c->geo.ny + T::Y - 1) / T::Y; ntz = (c->geo.nz + T::Z - 1) / T::Z;
+  } else {
+    typedef TileShape<2> T;
+    ntx = (c->geo.nx + T::X - 1) / T::X; nty = (c->geo.ny + T::Y - 1) / T::Y; ntz = 1;
   }
-  hipLaunchKernelGGL(k_key_rowstart, dim3((c->geo.nrows + 1 + tb - 1) / tb), dim3(tb), 0, c->stream,
-                     (const unsigned long long*)c->mkey_s, (int)total, c->geo.nrows, xw, c->mrow);
-  RowSpreadArgs ra;
-  ra.g = c->geo;
-  ra.w = c->walls;
-  ra.key = c->mkey_s;
-  ra.val = c->mval_s;
-  ra.rowstart = c->mrow;
-  ra.slots = c->slot_table;
-  ra.F4 = F4;
-  ra.flag = flag;
-  ra.fold = fold;
-  ra.accumulate = accumulate;
-  const unsigned gx = (unsigned)((c->geo.nx + ST_XS - 1) / ST_XS);
+  const int ntiles = ntx * nty * ntz;
+  if (moved_only && ntiles != c->ntiles) {
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    (void)hipFree(c->tflag); (void)hipFree(c->tlist);
+    c->tflag = nullptr; c->tlist = nullptr; c->ntiles = 0;
+    HIPCHK(c, hipMalloc(&c->tflag, (size_t)ntiles));
+    HIPCHK(c, hipMalloc(&c->tlist, sizeof(int) * ((size_t)ntiles + 1)));
+    hipLaunchKernelGGL(k_fill_u8, dim3(fill_u8_blocks((size_t)ntiles)), dim3(256), 0, c->stream, c->tflag,
+                       (size_t)ntiles, (unsigned char)0);
+    c->ntiles = ntiles;
+  }
+  if (moved_only)
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, c->tlist + ntiles, (size_t)1, 0);
+  const RecBin rb{c->sb_cnt, c->sb_bk, c->sb_rank, moved_only ? c->tflag : nullptr, fold};
+  if (bin) {
+    const int rc = (*bin)(ss, rb);
+    if (rc) return rc;
+  } else if (total > 0) {
+    const unsigned gb = (unsigned)((total + tb - 1) / tb);
+    if (c->D == 3)
+      hipLaunchKernelGGL(k_rec_count<3>, dim3(gb), dim3(tb), 0, c->stream, ss, c->geo, rb);
+    else
+      hipLaunchKernelGGL(k_rec_count<2>, dim3(gb), dim3(tb), 0, c->stream, ss, c->geo, rb);
+    HIPCHK(c, hipGetLastError());
+  }
+  size_t bytes = c->sb_scan_bytes;
+  HIPCHK(c, rocprim::exclusive_scan(c->sb_scan_tmp, bytes, c->sb_cnt, c->sb_start, 0, (size_t)nb + 1,
+                                    rocprim::plus<int>(), c->stream));
+  if (total > 0) {
+    const unsigned gb = (unsigned)((total + tb - 1) / tb);
+    if (c->D == 3)
+      hipLaunchKernelGGL(k_rec_place<3>, dim3(gb), dim3(tb), 0, c->stream, ss, c->geo, c->walls, fold,
+                         (const int*)c->sb_bk, (const int*)c->sb_rank, (const int*)c->sb_start, c->sb_ent,
+                         rb.tflag);
+    else
+      hipLaunchKernelGGL(k_rec_place<2>, dim3(gb), dim3(tb), 0, c->stream, ss, c->geo, c->walls, fold,
+                         (const int*)c->sb_bk, (const int*)c->sb_rank, (const int*)c->sb_start, c->sb_ent,
+                         rb.tflag);
+    hipLaunchKernelGGL(k_bucket_sort, dim3((unsigned)((nb + tb - 1) / tb)), dim3(tb), 0, c->stream, nb,
+                       (const int*)c->sb_start, c->sb_ent, c->sb_large, nlarge);
+    hipLaunchKernelGGL(k_bucket_sort_large, dim3(256), dim3(tb), 0, c->stream, (const int*)c->sb_large,
+                       (const int*)nlarge, (const int*)c->sb_start, c->sb_ent, c->sb_ent2);
+    HIPCHK(c, hipGetLastError());
+  }
+  if (moved_only)
+    hipLaunchKernelGGL(k_tile_list, dim3((unsigned)((ntiles + tb - 1) / tb)), dim3(tb), 0, c->stream, ntiles,
+                       c->tflag, c->tlist, c->tlist + ntiles);
+  TileSpreadArgs ta;
+  ta.tiles = moved_only ? c->tlist : nullptr;
+  ta.ntiles = moved_only ? c->tlist + ntiles : nullptr;
+  ta.g = c->geo;
+  ta.w = c->walls;
+  ta.bstart = c->sb_start;
+  ta.ent = c->sb_ent;
+  ta.ss = ss;
+  ta.F4 = F4;
+  ta.flag = flag;
+  ta.fold = fold;
+  ta.accumulate = accumulate;
+  // a periodic axis of fewer than 4 nodes lets one stencil reach a node and
+  // its ghost image together
+  ta.serial = 0;
+  const int ext[3] = {c->geo.nx, c->geo.ny, c->geo.nz};
+  for (int d = 0; d < c->D; d++)
+    if (fold && ext[d] < 4 && (c->walls.type[d][0] == 0 || c->walls.type[d][1] == 0)) ta.serial = 1;
+  // moved_only: a fixed grid walks the tile list
+  const dim3 grid = moved_only ? dim3((unsigned)std::min(ntiles, 2048), 1u, 1u)
+                               : dim3((unsigned)ntx, (unsigned)nty, (unsigned)ntz);
   if (c->D == 3)
-    hipLaunchKernelGGL(k_spread_rows<3>,
-                       dim3(gx, (unsigned)((c->geo.ny + ST_TJ - 1) / ST_TJ), (unsigned)((c->geo.nz + ST_TK - 1) / ST_TK)),
-                       dim3(ST_THREADS), 0, c->stream, ra);
+    hipLaunchKernelGGL(k_spread_tiles<3>, grid, dim3(64), 0, c->stream, ta);
   else
-    hipLaunchKernelGGL(k_spread_rows<2>, dim3(gx, (unsigned)((c->geo.ny + ST_TJ2 - 1) / ST_TJ2), 1u),
-                       dim3(ST_THREADS), 0, c->stream, ra);
+    hipLaunchKernelGGL(k_spread_tiles<2>, grid, dim3(64), 0, c->stream, ta);
   HIPCHK(c, hipGetLastError());
   return COUPLB_OK;
 }
@@ -1195,6 +1356,7 @@ int fold_slots(couplb_ctx* c) {
   const int rc = row_spread(c, c->F4, c->flag, 1, c->f4_partial);
   if (rc) return rc;
   c->pair_epoch = -1;  // the node flags now describe F4
+  c->mpair_epoch = -1;
   c->f4_partial = 1;
   c->nslots = 0;
   return COUPLB_OK;
@@ -1222,6 +1384,10 @@ StepArgs step_args(couplb_ctx* c, const double g[3], int flags) {
   a.pp = nullptr;
   a.pw = nullptr;
   a.fsrc = nullptr;
+  a.pcon = nullptr;
+  a.pidx = nullptr;
+  a.flag2 = nullptr;
+  a.has_f2 = 0;
   return a;
 }
 
@@ -1297,9 +1463,50 @@ int step_impl(couplb_ctx* c, const double g[3], int flags, bool timed) {
   const int S = (c->D == 3) ? stencil_size<3>() : stencil_size<2>();
   // markers at the positions of the previous LB interval: constant map
   const bool fixed = !c->f4_partial && c->nslots == 1 && c->slots[0].epoch == c->prev_step_epoch &&
-                     (long)c->slots[0].n * S < (long)INT32_MAX;
+                     !c->kept_move && (long)c->slots[0].n * S < (long)INT32_MAX;
+  // moving markers whose sub-steps all kept one ordering: the moving-marker
+  // map of that ordering, the records that left their base cell spread apart
+  bool moving = !fixed && !c->f4_partial && c->nslots > 0 && c->order_n > 0 &&
+                (long)c->order_n * S < (long)INT32_MAX;
+  for (int s = 0; moving && s < c->nslots; s++)
+    moving = c->slots[s].epoch == c->pos_epoch && c->slots[s].n == c->order_n;
   int tspread = -1;
-  if (fixed) {
+  if (moving) {
+    if (timed) {
+      const int rc = timer_begin(c, 2, &tspread);
+      if (rc) return rc;
+    }
+    if (c->mpair_epoch != c->pos_epoch) {
+      const int rc = build_moving_map(c, c->order_n);
+      if (rc) return rc;
+    }
+    // the pair forces of the records on their ordered base cell (the same
+    // kernel bins the others), then those others spread into F4 where flag2
+    // is set
+    if (!c->flag2) HIPCHK(c, hipMalloc(&c->flag2, c->N));
+    hipLaunchKernelGGL(k_fill_u8, dim3(fill_u8_blocks(c->N)), dim3(256), 0, c->stream, c->flag2, c->N,
+                       (unsigned char)0);
+    const std::function<int(const SlotSpan&, const RecBin&)> con = [c, S](const SlotSpan& ss, const RecBin& rb) {
+      const long nt = (long)c->order_n * ((c->D == 3) ? 4 : 1);  // threads: one per stencil plane
+      const int tb = 256;
+      if (c->D == 3)
+        hipLaunchKernelGGL(k_pair_con<3>, dim3((unsigned)((nt + tb - 1) / tb)), dim3(tb), 0, c->stream,
+                           c->order_n, ss, c->pcon, c->geo, rb);
+      else
+        hipLaunchKernelGGL(k_pair_con<2>, dim3((unsigned)((nt + tb - 1) / tb)), dim3(tb), 0, c->stream,
+                           c->order_n, ss, c->pcon, c->geo, rb);
+      HIPCHK(c, hipGetLastError());
+      return COUPLB_OK;
+    };
+    const int rc = row_spread(c, c->F4, c->flag2, 1, 0, true, &con);
+    if (rc) return rc;
+    a.pair_mode = 2;
+    a.prun = c->prun;
+    a.pcon = c->pcon;
+    a.pidx = c->pval_s;
+    a.flag2 = c->flag2;
+    a.has_f2 = 1;
+  } else if (fixed) {
     if (timed) {
       const int rc = timer_begin(c, 2, &tspread);
       if (rc) return rc;
@@ -1317,6 +1524,7 @@ int step_impl(couplb_ctx* c, const double g[3], int flags, bool timed) {
   } else if (c->nslots > 0) {
     // spread the pending marker records into the per-node force
     c->pair_epoch = -1;  // the spreading rewrites the node flags
+    c->mpair_epoch = -1;
     if (timed) {
       const int rc = timer_begin(c, 2, &tspread);
       if (rc) return rc;
@@ -1339,7 +1547,10 @@ int step_impl(couplb_ctx* c, const double g[3], int flags, bool timed) {
     const int rc = timer_end(c, tstep);
     if (rc) return rc;
   }
-  if (c->nslots > 0) c->prev_step_epoch = (c->nslots == 1) ? c->slots[0].epoch : -1;
+  if (c->nslots > 0) {
+    c->prev_step_epoch = (c->nslots == 1) ? c->slots[0].epoch : -1;
+    c->order_age++;
+  }
   c->cur = dst;
   c->local = 0;
   c->prev_valid = 1;
@@ -1420,6 +1631,9 @@ int couplb_create(const couplb_params* p, couplb_ctx** out) {
   // COUPLB_FUSED_NODES overrides the bound (0 switches the fusion off)
   c->fused_nodes = 1024;
   if (const char* v = std::getenv("COUPLB_FUSED_NODES")) c->fused_nodes = (size_t)std::strtoull(v, nullptr, 10);
+  // LB intervals a moving-marker ordering is kept (0: a new ordering every
+  // interval)
+  if (const char* v = std::getenv("COUPLB_REORDER_EVERY")) c->reorder_every = (int)std::strtol(v, nullptr, 10);
   c->elem = storage_bytes(p->storage);
   c->scale = storage_scale(p);
   c->base = (p->storage == COUPLB_FP64) ? 0.0 : p->rho0;
@@ -1500,10 +1714,11 @@ int couplb_destroy(couplb_ctx* c) {
   for (auto& e : c->mdf_ev)
     if (e) (void)hipEventDestroy(e);
   for (auto& s : c->slots) (void)hipFree(s.rec);
-  (void)hipFree(c->moved);
-  (void)hipFree(c->mkey); (void)hipFree(c->mkey_s); (void)hipFree(c->mval); (void)hipFree(c->mval_s);
-  (void)hipFree(c->mrow); (void)hipFree(c->msort_tmp);
-  (void)hipFree(c->slot_table);
+  (void)hipFree(c->vcache); (void)hipFree(c->vbase);
+  (void)hipFree(c->pcon); (void)hipFree(c->flag2); (void)hipFree(c->tflag);
+  (void)hipFree(c->tlist);
+  (void)hipFree(c->sb_cnt); (void)hipFree(c->sb_start); (void)hipFree(c->sb_bk); (void)hipFree(c->sb_rank);
+  (void)hipFree(c->sb_ent); (void)hipFree(c->sb_ent2); (void)hipFree(c->sb_large); (void)hipFree(c->sb_scan_tmp);
   for (int e = 0; e < TimerRing::R; e++) {
     if (c->tr.a[e]) (void)hipEventDestroy(c->tr.a[e]);
     if (c->tr.b[e]) (void)hipEventDestroy(c->tr.b[e]);

@@ -1,7 +1,9 @@
 // ibm.hpp -- immersed-boundary kernels: marker ordering by base row, fluid
-// velocity at covered nodes, one-marker-per-lane interpolation + penalty
-// force + spread record.  Spreading itself happens inside the fused step
-// (kernels.hpp, k_step) as a deterministic gather over the sorted records.
+// velocity at covered nodes, four-lanes-per-marker interpolation + penalty
+// force + spread record.  Spreading is a deterministic gather: over the
+// node-sorted (marker, stencil node) map inside the LB step (k_step, from
+// spread_pairs.hpp's maps) and, for records off the map, over binned
+// records per tile of nodes (spread_rows.hpp).
 //
 // All arithmetic that decides stencil placement or weights runs with FP
 // contraction off, so floor() picks the same base cell and the delta values
@@ -67,7 +69,6 @@ __device__ inline void to_local(const IbmParams& ip, int D, const double* x,
   lz = (D == 3) ? ((x[2] - ip.lo[2]) * ip.inv_dx + 1.0) : 0.0;
 }
 
-struct Base { int i0, j0, k0, valid; };
 
 // Row key of the marker's base cell; markers whose stencil misses the
 // ghost-inclusive grid get the sentinel nrows and sort last.
@@ -231,46 +232,43 @@ __global__ void __launch_bounds__(lds_chunk<S>()) k_upull_lds(PullCtx pc, const 
   stamp[node] = cycle;
 }
 
-// A later sub-step of the same LB state with moved markers keeps the
-// interval's marker ordering, coverage and gathered velocities.  A marker
-// whose base cell changed since the ordering was built may reach nodes the
-// gather did not cover.  k_moved_markers lists those markers (one thread
-// per marker; the list order does not matter); k_moved_cover then computes
-// the velocity of their stencil nodes not yet gathered for this state (the
-// same pull and moments as k_upull, stamped), one wavefront per listed
-// marker and one lane per stencil node, over a fixed grid that walks the
-// list.  Two markers may compute the same node; they store the same values.
-__global__ void __launch_bounds__(256) k_moved_markers(int n, const int* perm, const double* x, IbmParams ip,
-                                                       Geo g, const Base* bases, int* count, int* list) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  double lx, ly, lz;
-  to_local(ip, g.D, x + 3 * (size_t)perm[p], lx, ly, lz);
-  if (!(isfinite(lx) && isfinite(ly) && isfinite(lz) && fabs(lx) < 1e9 && fabs(ly) < 1e9 && fabs(lz) < 1e9))
-    return;
-  const int i0 = (int)floor(lx), j0 = (int)floor(ly), k0 = (g.D == 3) ? (int)floor(lz) : 0;
-  const Base b = bases[p];
-  if (b.valid && b.i0 == i0 && b.j0 == j0 && b.k0 == k0) return;  // covered when ordered
-  list[atomicAdd(count, 1)] = p;
-}
-
-constexpr int MOVED_BLOCKS = 256;
+// A sub-step with moved markers on a kept marker ordering keeps the
+// ordering's coverage and gathered velocities.  A marker whose base cell
+// changed since the ordering was built may reach nodes the gather did not
+// cover: one lane per marker finds the markers that left their ordered base
+// cell, then the whole wavefront takes them one at a time, one lane per
+// stencil node computing the velocity of a fluid node not yet gathered for
+// this state (the same pull and moments as k_upull, stamped).  Two markers
+// may compute the same node; they store the same values.
 template <int D, typename S>
-__global__ void __launch_bounds__(256) k_moved_cover(const int* perm, const double* x, IbmParams ip,
-                                                     PullCtx pc, const int* count, const int* list,
-                                                     double* u4, int* stamp, int cycle) {
+__global__ void __launch_bounds__(256) k_moved_fix(int n, const int* perm, const double* x, IbmParams ip,
+                                                   PullCtx pc, const Base* bases, double* u4, int* stamp,
+                                                   int cycle) {
   using T = typename Store<S>::T;
   const Geo& g = pc.g;
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
-  const int nm = *count;
-  for (int m = blockIdx.x * 4 + (threadIdx.x >> 6); m < nm; m += MOVED_BLOCKS * 4) {
-    if (lane >= stencil_size<D>()) continue;
-    const int p = list[m];
+  int i0 = 0, j0 = 0, k0 = 0;
+  bool moved = false;
+  if (p < n) {
     double lx, ly, lz;
     to_local(ip, D, x + 3 * (size_t)perm[p], lx, ly, lz);
-    const int i0 = (int)floor(lx), j0 = (int)floor(ly), k0 = (D == 3) ? (int)floor(lz) : 0;
-    const int a = i0 - 1 + (lane & 3), bb = j0 - 1 + ((lane >> 2) & 3);
-    const int c = (D == 3) ? k0 - 1 + (lane >> 4) : 0;
+    if (isfinite(lx) && isfinite(ly) && isfinite(lz) && fabs(lx) < 1e9 && fabs(ly) < 1e9 && fabs(lz) < 1e9) {
+      i0 = (int)floor(lx);
+      j0 = (int)floor(ly);
+      k0 = (D == 3) ? (int)floor(lz) : 0;
+      const Base b = bases[p];
+      moved = !(b.valid && b.i0 == i0 && b.j0 == j0 && b.k0 == k0);  // covered when ordered
+    }
+  }
+  unsigned long long todo = __ballot(moved);
+  while (todo) {
+    const int src = __ffsll((long long)todo) - 1;
+    todo &= todo - 1;
+    const int mi = __shfl(i0, src), mj = __shfl(j0, src), mk = __shfl(k0, src);
+    if (lane >= stencil_size<D>()) continue;
+    const int a = mi - 1 + (lane & 3), bb = mj - 1 + ((lane >> 2) & 3);
+    const int c = (D == 3) ? mk - 1 + (lane >> 4) : 0;
     if (a < 0 || a > g.nx + 1 || bb < 0 || bb > g.ny + 1 || (D == 3 && (c < 0 || c > g.nz + 1))) continue;
     int wd, wsd;
     if (node_type(pc.w, g, a, bb, c, wd, wsd) != 0) continue;
@@ -288,7 +286,9 @@ __global__ void __launch_bounds__(256) k_moved_cover(const int* perm, const doub
 // the whole 4x4 stencil in 2D), IBM::interpolate's rules per node type
 // (couplb_ibm.h:66-86): fluid nodes add vel(node)*d, bounce-back and
 // free-slip nodes add d to the normalisation only, moving-wall nodes add
-// the wall velocity; nodes beyond the ghost layer are skipped.  s = {sum u,
+// the wall velocity; nodes beyond the ghost layer are skipped.  vel(node,
+// slot, un) gives the velocity of fluid node `node`, stencil position
+// slot = 4u + t of the plane.  s = {sum u,
 // sum v, sum w, sum d over counted nodes, sum d over fluid nodes}.
 // wall_vel = false drops the moving-wall velocity term (the weight still
 // counts), which makes the map from node values to the marker value linear.
@@ -319,7 +319,7 @@ __device__ inline void interp_plane(const Geo& g, const Walls& w, bool fin, int 
           const unsigned node = (unsigned)wrap_ghost(a, g.nx) +
                                 (unsigned)g.nx * ((unsigned)wrap_ghost(b, g.ny) +
                                                   (unsigned)g.ny * (unsigned)((D == 3) ? wrap_ghost(c, g.nz) : 0));
-          vel(node, un[h][t]);
+          vel(node, (u0 + h) * 4 + t, un[h][t]);
         }
       }
     }
@@ -375,12 +375,27 @@ __device__ inline void combine_planes(double s[5]) {
 // are combined in plane order, so results are deterministic; they differ
 // from the reference's single running sum by rounding only.
 // fmode 0: f_atom[idx] += F (device f); 1: f_atom[idx] = F (host adds).
+// Velocity cache (moving markers, later sub-steps of an LB interval): with
+// ucache set, each lane keeps the velocities of its 16 stencil nodes (three
+// doubles each, in stencil order) and cbase[p] the base cell and state they
+// belong to; a marker whose base cell and state match reads them there
+// instead of gathering them from u4 again.  The values depend only on the
+// base cell and the state, so the cache stays valid when the marker
+// ordering changes.
+struct VelCache {
+  double* u;   // value c of stencil slot q of lane l at u[(3q + c) * L + l]: one
+               //   wavefront's 64 lanes read or write 64 consecutive doubles
+  int4* base;  // per sorted marker: i0, j0, k0, state
+  long L;      // lanes the cache holds (markers x lanes per marker)
+  int cycle;   // current state
+};
+
 template <int D>
 __global__ void __launch_bounds__(256) k_marker(int n, const int* perm, const double* x,
                                                 const double* v, const double* m, double* f_atom,
                                                 int fmode, IbmParams ip, Geo g, Walls w,
                                                 const double* u4, MarkerRec* rec, double* ufc,
-                                                double* fsrc) {
+                                                double* fsrc, VelCache vc) {
 #pragma clang fp contract(off)
   constexpr int LPM = (D == 3) ? 4 : 1;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -401,12 +416,32 @@ __global__ void __launch_bounds__(256) k_marker(int n, const int* perm, const do
     wy[t] = delta1(ip.kern, ly - (double)(j0 - 1 + t));
   }
   const double wzv = (D == 3) ? delta1(ip.kern, lz - (double)(k0 - 1 + vv)) : 1.0;
+  bool cached = false;
+  double* cl = nullptr;
+  if (vc.u && fin) {
+    const int4 cb = vc.base[p];
+    cached = cb.x == i0 && cb.y == j0 && cb.z == k0 && cb.w == vc.cycle;
+    cl = vc.u + (size_t)p * LPM + vv;
+  }
+  const size_t L = (size_t)vc.L;
   double s[5];
   interp_plane<D>(g, w, fin, i0, j0, k0, vv, wx, wy, wzv,
-                  [&](unsigned node, double* un) {
-                    const double4 q = *reinterpret_cast<const double4*>(u4 + 4 * (size_t)node);
-                    un[0] = q.x; un[1] = q.y; un[2] = q.z;
+                  [&](unsigned node, int slot, double* un) {
+                    if (cached) {
+                      un[0] = cl[(3 * slot) * L];
+                      un[1] = cl[(3 * slot + 1) * L];
+                      un[2] = cl[(3 * slot + 2) * L];
+                    } else {
+                      const double4 q = *reinterpret_cast<const double4*>(u4 + 4 * (size_t)node);
+                      un[0] = q.x; un[1] = q.y; un[2] = q.z;
+                      if (cl) {
+                        cl[(3 * slot) * L] = q.x;
+                        cl[(3 * slot + 1) * L] = q.y;
+                        cl[(3 * slot + 2) * L] = q.z;
+                      }
+                    }
                   }, s);
+  if (cl && !cached && vv == 0) vc.base[p] = int4{i0, j0, k0, vc.cycle};
   combine_planes<LPM>(s);
   double ui = s[0], vi = s[1], wi = s[2], ws = s[3], wsf = s[4];
   double wz[4] = {1.0, 1.0, 1.0, 1.0};

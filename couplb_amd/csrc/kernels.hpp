@@ -48,6 +48,18 @@ struct Slot {
   const MarkerRec* rec;  // one record per marker, in the sorted marker order
 };
 
+// The record slots of an LB interval, as kernel arguments: records of slot
+// s (one per marker, sorted marker order) are entries off[s] .. off[s + 1] - 1
+// of their concatenation.  bases: when set, only the records whose base cell
+// differs from the ordering's (the marker left it) are spread from here.
+struct Base { int i0, j0, k0, valid; };  // a marker's base cell in the ordering
+struct SlotSpan {
+  int nslots;
+  int off[MAX_SLOTS + 1];
+  const MarkerRec* rec[MAX_SLOTS];
+  const Base* bases;
+};
+
 struct Geo {
   int nx, ny, nz;        // interior extents
   int D;
@@ -375,12 +387,36 @@ struct StepArgs {
   // fixed-marker map: instead of F4, the step sums the flagged node's run of
   // (marker, weight) pairs itself: prun[node] = (first pair, count), pp/pw
   // the sorted pairs, fsrc three reaction values per sorted marker
-  int pair_mode;
+  int pair_mode;        // 0 F4 at flagged nodes, 1 fixed-marker map, 2 moving-marker map
   const int2* prun;
   const int* pp;
   const double* pw;
   const double* fsrc;
+  // moving-marker map (pair_mode 2): pcon holds the force of every (marker,
+  // stencil slot) summed over the sub-steps (four doubles each, the last
+  // unused), pidx the sorted pairs' (marker, stencil slot) entries; records
+  // that left the map's base cells were spread into F4 at the nodes flag2
+  // marks
+  const double* pcon;
+  const int* pidx;
+  const unsigned char* flag2;
+  int has_f2;
 };
+
+// Sum of one node's run of the moving-marker map, sum_f con[idx[f]] in pair
+// order (con: four doubles per entry, the last unused).
+__device__ inline void run_sum_con(int s, int len, const int* idx, const double* con, double& ax, double& ay,
+                                   double& az) {
+#pragma clang fp contract(off)
+  ax = ay = az = 0.0;
+  const double4* c4 = reinterpret_cast<const double4*>(con);
+  for (int f = s; f < s + len; f++) {
+    const double4 v = c4[idx[f]];
+    ax += v.x;
+    ay += v.y;
+    az += v.z;
+  }
+}
 
 // Sum of one node's run of the fixed-marker map, sum_f src[pp[f]] * pw[f]
 // in pair order (couplb_ibm.h:125-130: F_node += F_marker * w); src holds
@@ -425,10 +461,12 @@ __device__ inline void run_sum(int s, int len, const int* pp, const double* pw, 
 // lay, dst: the buffer collide_exact (FP64) writes (the reduced formats
 // store through out); rho_in, rho_out: the density array read for gravity
 // and the one written (a.pc.lay, a.dst, a.pc.rho_coll, a.rho_out in the
-// single-step kernels).  PAIR: flagged nodes take their force from their
+// single-step kernels).  PM 1: flagged nodes take their force from their
 // run of the fixed-marker map (a.prun, a.pp, a.pw, a.fsrc) instead of a.F4;
-// a separate instantiation, so the other steps keep their registers.
-template <int D, typename S, bool PAIR = false, typename Out>
+// PM 2: from their run of the moving-marker map (a.pcon), plus a.F4 where
+// a.flag2 is set; separate instantiations, so the other steps keep their
+// registers.
+template <int D, typename S, int PM = 0, typename Out>
 __device__ inline void step_node(const StepArgs& a, const Layout& lay, S* dst, const double* rho_in,
                                  double* rho_out, int i, int j, int k, unsigned node,
                                  typename Store<S>::T (&f)[Lat<D>::Q], Out out) {
@@ -439,8 +477,26 @@ __device__ inline void step_node(const StepArgs& a, const Layout& lay, S* dst, c
   // IBM force spread by the last cycle's markers, present only at flagged
   // nodes: the node's pair run (fixed-marker map) or the tile spread's F4
   T Fx = 0, Fy = 0, Fz = 0;
-  if (a.has_ibm && a.flag[node]) {
-    if constexpr (PAIR) {
+  if constexpr (PM == 2) {
+    if (a.has_ibm) {
+      double ax = 0.0, ay = 0.0, az = 0.0;
+      if (a.flag[node]) {
+        const int2 r = a.prun[node];
+        run_sum_con(r.x, r.y, a.pidx, a.pcon, ax, ay, az);
+      }
+      if (a.has_f2 && a.flag2[node]) {
+#pragma clang fp contract(off)
+        const double4 F = *reinterpret_cast<const double4*>(a.F4 + 4 * (size_t)node);
+        ax += F.x;
+        ay += F.y;
+        az += F.z;
+      }
+      Fx = (T)ax;
+      Fy = (T)ay;
+      Fz = (T)az;
+    }
+  } else if (a.has_ibm && a.flag[node]) {
+    if constexpr (PM == 1) {
       const int2 r = a.prun[node];
       double ax, ay, az;
       // FP64: four pairs in flight (measured faster); the reduced formats
@@ -570,13 +626,13 @@ __device__ inline void step_node(const StepArgs& a, const Layout& lay, S* dst, c
   }
 }
 
-template <int D, typename S, bool PAIR = false, typename Out>
+template <int D, typename S, int PM = 0, typename Out>
 __device__ inline void step_node(const StepArgs& a, int i, int j, int k, unsigned node,
                                  typename Store<S>::T (&f)[Lat<D>::Q], Out out) {
-  step_node<D, S, PAIR>(a, a.pc.lay, static_cast<S*>(a.dst), a.pc.rho_coll, a.rho_out, i, j, k, node, f, out);
+  step_node<D, S, PM>(a, a.pc.lay, static_cast<S*>(a.dst), a.pc.rho_coll, a.rho_out, i, j, k, node, f, out);
 }
 
-template <int D, typename S, bool PAIR = false>
+template <int D, typename S, int PM = 0>
 __global__ void __launch_bounds__(256) k_step(StepArgs a) {
   using L = Lat<D>;
   using T = typename Store<S>::T;
@@ -590,7 +646,7 @@ __global__ void __launch_bounds__(256) k_step(StepArgs a) {
   pull_node<D, S, true>(a.pc, i, j, k, f);
   S* dst = static_cast<S*>(a.dst);
   const size_t at0 = a.pc.lay.at(0, (unsigned)j + (unsigned)g.ny * (unsigned)k, (unsigned)i);
-  step_node<D, S, PAIR>(a, i, j, k, node, f,
+  step_node<D, S, PM>(a, i, j, k, node, f,
                         [&](int q, T v) { Store<S>::store(dst + q * a.pc.lay.pq + at0, v); });
 }
 
@@ -716,7 +772,7 @@ __device__ inline void stage_rows(Geo g, Layout lay, const S* src, S* win, int x
   }
 }
 
-template <int D, typename S, bool PAIR = false>
+template <int D, typename S, int PM = 0>
 __global__ void __launch_bounds__(lds_chunk<S>()) k_step_lds(StepArgs a) {
   constexpr int LDS_XC = lds_chunk<S>();
   using L = Lat<D>;
@@ -748,7 +804,7 @@ __global__ void __launch_bounds__(lds_chunk<S>()) k_step_lds(StepArgs a) {
       return Store<S>::decode(win[q][t + G - L::ex(q)]);
     });
   __syncthreads();
-  if (live) step_node<D, S, PAIR>(a, i, j, k, node, f, [&](int q, T v) { res[q][t] = Store<S>::encode(v); });
+  if (live) step_node<D, S, PM>(a, i, j, k, node, f, [&](int q, T v) { res[q][t] = Store<S>::encode(v); });
   __syncthreads();
   S* dst = static_cast<S*>(a.dst);
   const unsigned row = (unsigned)j + (unsigned)g.ny * (unsigned)k;

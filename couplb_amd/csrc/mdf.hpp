@@ -178,7 +178,7 @@ __global__ void __launch_bounds__(MDF_THREADS) k_mdf_interp(int n, Geo g, Walls 
   }
   double s[5];
   interp_plane<D>(g, w, valid, i0, j0, k0, vv, wx, wy, wzv,
-                  [&](unsigned node, double* un) {
+                  [&](unsigned node, int, double* un) {
                     un[0] = ut[3 * (size_t)node + 0];
                     un[1] = ut[3 * (size_t)node + 1];
                     un[2] = ut[3 * (size_t)node + 2];

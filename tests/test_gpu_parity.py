@@ -260,3 +260,42 @@ def test_more_sub_steps_than_record_slots_match_oracle(md):
     ferr = max(rel_l2(p, q) for p, q in zip(fd, fo))
     print(f"rel-L2(f) = {err:.3e}, rel-L2(F_atom) = {ferr:.3e}")
     assert err <= TOL and ferr <= 1e-11
+
+
+@pytest.mark.parametrize("reorder_every", ["8", "3", "0"])
+def test_moving_markers_across_cells_match_oracle(reorder_every, monkeypatch):
+    """Markers that cross cells and periodic faces at every few MD steps over
+    40 LB intervals (md_per_lb 3): the device keeps a marker ordering for
+    COUPLB_REORDER_EVERY intervals, sums the forces of the markers on their
+    ordered base cells through the node-sorted map of that ordering, and
+    spreads the records of markers that left their base cell tile by tile;
+    the result must be the reference's (fix_couplb.cpp:723-798) to rounding."""
+    monkeypatch.setenv("COUPLB_REORDER_EVERY", reorder_every)
+    rng = np.random.default_rng(31)
+    n = 80
+    nx, ny, nz = 20, 14, 12
+    x0 = np.stack([rng.uniform(0, nx, n), rng.uniform(3, ny - 3, n), rng.uniform(0, nz, n)], 1)
+    vx = rng.uniform(-0.15, 0.15, n)
+    base = atoms_from(x0, rng.normal(0.0, 1e-3, (n, 3)), rng.uniform(0.5, 1.5, n))
+
+    def atoms(step):
+        a = copy_atoms(base)
+        x = x0.copy()
+        x[:, 0] = np.mod(x0[:, 0] + vx * step, nx)
+        x[:, 1] = x0[:, 1] + 1.5 * np.sin(0.07 * step + np.arange(n))
+        x[:, 2] = np.mod(x0[:, 2] + 0.6 * step * np.cos(np.arange(n)), nz)
+        a["x"] = x
+        return a
+
+    cmd = (f"cross all couplb {nx} {ny} {nz} 0.1 1.0 md_per_lb 3 kernel peskin4 wall_y 1 2 "
+           "wall_vel 0.01 0.0 0.0 gravity 1e-5 0.0 0.0")
+    print("case moving markers across cells, COUPLB_REORDER_EVERY", reorder_every, flush=True)
+    steps = 120
+    dev, fd = run_device(cmd, steps, dimension=3, dt=1.0 / 3, atoms_fn=atoms)
+    ref, fo = run_oracle(cmd, steps, dimension=3, dt=1.0 / 3, atoms_fn=atoms)
+    a = interior(dev.device_state().reshape(19, nz + 2, ny + 2, nx + 2), 3)
+    b = interior(ref.f(), 3)
+    err = rel_l2(a, b)
+    ferr = max(rel_l2(p, q) for p, q in zip(fd, fo))
+    print(f"rel-L2(f) = {err:.3e}, rel-L2(F_atom) = {ferr:.3e}")
+    assert err <= TOL and ferr <= 1e-11

@@ -158,6 +158,7 @@ struct couplb_ctx {
   double* pcon = nullptr;
   long pcon_cap = 0;
   unsigned char* flag2 = nullptr;
+  int f2_tag = 0;                  // tag of the current step's flag2 marks
   unsigned char* tflag = nullptr;  // tiles reached by records that left their cell
   int* tlist = nullptr;            // their list, then its length
   int ntiles = 0;
@@ -1321,6 +1322,7 @@ int row_spread(couplb_ctx* c, double* F4, unsigned char* flag, int fold, int acc
     hipLaunchKernelGGL(k_tile_list, dim3((unsigned)((ntiles + tb - 1) / tb)), dim3(tb), 0, c->stream, ntiles,
                        c->tflag, c->tlist, c->tlist + ntiles);
   TileSpreadArgs ta;
+  ta.tag = (unsigned char)(moved_only ? c->f2_tag : 1);
   ta.tiles = moved_only ? c->tlist : nullptr;
   ta.ntiles = moved_only ? c->tlist + ntiles : nullptr;
   ta.g = c->geo;
@@ -1483,9 +1485,13 @@ int step_impl(couplb_ctx* c, const double g[3], int flags, bool timed) {
     // the pair forces of the records on their ordered base cell (the same
     // kernel bins the others), then those others spread into F4 where flag2
     // is set
+    // flag2 marks with a tag that changes every step (1..255), so it is
+    // cleared only when the tags come round again
     if (!c->flag2) HIPCHK(c, hipMalloc(&c->flag2, c->N));
-    hipLaunchKernelGGL(k_fill_u8, dim3(fill_u8_blocks(c->N)), dim3(256), 0, c->stream, c->flag2, c->N,
-                       (unsigned char)0);
+    c->f2_tag = (c->f2_tag % 255) + 1;
+    if (c->f2_tag == 1)
+      hipLaunchKernelGGL(k_fill_u8, dim3(fill_u8_blocks(c->N)), dim3(256), 0, c->stream, c->flag2, c->N,
+                         (unsigned char)0);
     const std::function<int(const SlotSpan&, const RecBin&)> con = [c, S](const SlotSpan& ss, const RecBin& rb) {
       const long nt = (long)c->order_n * ((c->D == 3) ? 4 : 1);  // threads: one per stencil plane
       const int tb = 256;
@@ -1505,7 +1511,7 @@ int step_impl(couplb_ctx* c, const double g[3], int flags, bool timed) {
     a.pcon = c->pcon;
     a.pidx = c->pval_s;
     a.flag2 = c->flag2;
-    a.has_f2 = 1;
+    a.has_f2 = c->f2_tag;
   } else if (fixed) {
     if (timed) {
       const int rc = timer_begin(c, 2, &tspread);

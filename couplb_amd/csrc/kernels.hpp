@@ -395,12 +395,12 @@ struct StepArgs {
   // moving-marker map (pair_mode 2): pcon holds the force of every (marker,
   // stencil slot) summed over the sub-steps (four doubles each, the last
   // unused), pidx the sorted pairs' (marker, stencil slot) entries; records
-  // that left the map's base cells were spread into F4 at the nodes flag2
-  // marks
+  // that left the map's base cells were spread into F4 at the nodes where
+  // flag2 holds the step's tag
   const double* pcon;
   const int* pidx;
   const unsigned char* flag2;
-  int has_f2;
+  int has_f2;           // pair_mode 2: F4 applies where flag2 holds this tag (1..255)
 };
 
 // Sum of one node's run of the moving-marker map, sum_f con[idx[f]] in pair
@@ -484,7 +484,7 @@ __device__ inline void step_node(const StepArgs& a, const Layout& lay, S* dst, c
         const int2 r = a.prun[node];
         run_sum_con(r.x, r.y, a.pidx, a.pcon, ax, ay, az);
       }
-      if (a.has_f2 && a.flag2[node]) {
+      if (a.has_f2 && a.flag2[node] == (unsigned char)a.has_f2) {
 #pragma clang fp contract(off)
         const double4 F = *reinterpret_cast<const double4*>(a.F4 + 4 * (size_t)node);
         ax += F.x;

@@ -240,7 +240,8 @@ struct TileSpreadArgs {
   int serial;            // 1: a periodic axis of fewer than 4 nodes (one record may reach
                          //    one node twice): lanes add one at a time
   const int* tiles;      // when set: the tiles to spread (tile list, *ntiles of them);
-  const int* ntiles;     //   F4 and flag are written only where some record contributes
+  const int* ntiles;     //   F4 written and flag set to tag only where some record contributes
+  unsigned char tag;
 };
 
 // Base-cell values of one axis that a tile of ghost-inclusive nodes lo..hi
@@ -492,7 +493,7 @@ __global__ void __launch_bounds__(64) k_spread_tiles(TileSpreadArgs a) {
       if (a.tiles) {
         if (h) {
           *reinterpret_cast<double4*>(a.F4 + 4 * node) = double4{ax, ay, az, 0.0};
-          a.flag[node] = 1;
+          a.flag[node] = a.tag;
         }
       } else if (a.accumulate) {
         if (h) {

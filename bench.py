@@ -647,9 +647,10 @@ def main():
             dev.ibm_device(n, xm.data_ptr(), tv.data_ptr(), tm.data_ptr(), tf.data_ptr(),
                            positions_unchanged=False, **ibm_kw)
 
-    mv_steps = max(4, args.steps // 4)
+    mv_steps = max(8, args.steps // 2)
     dev.set_timer_mask(1 << 0)
-    one_step_moving()
+    for _ in range(2):
+        one_step_moving()
     dev.sync()
     t2 = time.perf_counter()
     for _ in range(mv_steps):
@@ -666,13 +667,17 @@ def main():
     hm = torch.from_numpy(m).pin_memory().numpy()
     hf = torch.zeros((n, 3), dtype=torch.float64).pin_memory().numpy()
 
+    # the forces are cleared before every sub-step, as LAMMPS clears them
+    # before its force computation, and the fix is told so (f is then not
+    # sent to the device); the masses do not change
     def one_step_host():
         dev.step(g, couplb_amd.RHO_REFRESHED_NEXT)
         for _ in range(md):
             hf[:] = 0.0
-            dev.ibm(hx, hv, hm, hf, positions_unchanged=True, **ibm_kw)
+            dev.ibm(hx, hv, hm, hf, positions_unchanged=True, masses_unchanged=True, forces_zero=True,
+                    **ibm_kw)
 
-    host_steps = max(4, args.steps // 4)
+    host_steps = max(8, args.steps // 4)
     one_step_host()
     dev.sync()
     t3 = time.perf_counter()
@@ -680,6 +685,29 @@ def main():
         one_step_host()
     dev.sync()
     mlups_host = nodes * host_steps / (time.perf_counter() - t3) / 1e6
+
+    # moving particles with the marker arrays in host memory: the positions
+    # of every MD sub-step (advanced by v * dt_MD as above) are prepared in
+    # pinned buffers beforehand, so the timed steps hold the fix's own work
+    # (uploads, sub-steps, force download), not the host's particle advance
+    mh_steps = 8
+    xs = [torch.from_numpy(x + v * ((k + 1) / md)).pin_memory().numpy() for k in range(md * (mh_steps + 1))]
+
+    def one_step_moving_host(s):
+        dev.step(g, couplb_amd.RHO_REFRESHED_NEXT)
+        for k in range(md):
+            hf[:] = 0.0
+            dev.ibm(xs[s * md + k], hv, hm, hf, positions_unchanged=False, masses_unchanged=True,
+                    forces_zero=True, **ibm_kw)
+
+    one_step_moving_host(0)
+    dev.sync()
+    t4 = time.perf_counter()
+    for st in range(1, mh_steps + 1):
+        one_step_moving_host(st)
+    dev.sync()
+    mlups_moving_host = nodes * mh_steps / (time.perf_counter() - t4) / 1e6
+    del xs
 
     # achievable-bandwidth reference: a plain copy of one population buffer's
     # worth (k_step reads one state and writes the other)
@@ -744,6 +772,7 @@ def main():
                                                  / (HBM_PEAK_GBS * 1e9), 4)},
         "mlups_markers_moving": round(mlups_moving, 1),
         "mlups_marker_arrays_on_host": round(mlups_host, 1),
+        "mlups_moving_marker_arrays_on_host": round(mlups_moving_host, 1),
         "lb_only_mlups": round(lb_only, 1),
         "lb_only_hbm_frac": round(lb_only * 1e6 * b_lup / (HBM_PEAK_GBS * 1e9), 4),
         "mass": diag["mass"],

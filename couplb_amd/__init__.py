@@ -45,7 +45,7 @@ class Params(C.Structure):
 class IbmArgs(C.Structure):
     _fields_ = [("lo", C.c_double * 3), ("inv_dx", C.c_double), ("xi_over_dtLB", C.c_double),
                 ("vel_scale", C.c_double), ("inv_N", C.c_double), ("force_scale", C.c_double),
-                ("positions_unchanged", C.c_int)]
+                ("positions_unchanged", C.c_int), ("masses_unchanged", C.c_int), ("forces_zero", C.c_int)]
 
 
 MDF_RICHARDSON, MDF_CG = 0, 1
@@ -254,13 +254,16 @@ class Device:
         return float(out[0])
 
     def ibm(self, x, v, m, f, lo, inv_dx, xi_over_dtLB, vel_scale, inv_N, force_scale,
-            positions_unchanged=False):
-        """Host-array IBM sub-step; f ([n,3] float64) is added into in place."""
+            positions_unchanged=False, masses_unchanged=False, forces_zero=False):
+        """Host-array IBM sub-step; f ([n,3] float64) is added into in place.
+        masses_unchanged / forces_zero: m equals the previous host sub-step's,
+        f holds zeros (neither is then sent to the device)."""
         a = IbmArgs()
         for d in range(3):
             a.lo[d] = lo[d]
         a.inv_dx, a.xi_over_dtLB, a.vel_scale = inv_dx, xi_over_dtLB, vel_scale
         a.inv_N, a.force_scale, a.positions_unchanged = inv_N, force_scale, int(positions_unchanged)
+        a.masses_unchanged, a.forces_zero = int(masses_unchanged), int(forces_zero)
         x = np.ascontiguousarray(x, np.float64)
         v = np.ascontiguousarray(v, np.float64)
         m = np.ascontiguousarray(m, np.float64)

@@ -103,8 +103,9 @@ struct couplb_ctx {
   int cycle = 0;         // state counter for the u stamps
   size_t fused_nodes = 0;  // grids up to this many nodes: k_steps_fused
   unsigned timer_mask = ~0u;  // timed region kinds (bit per kind)
-  double* hstage = nullptr;   // pinned host staging of the host-array IBM forces
-  size_t hstage_n = 0;
+  hipStream_t cstream = nullptr;  // host-array uploads of the IBM sub-steps
+  hipEvent_t up_done = nullptr;   // end of the latest upload on cstream
+  int hx_n = -1;                  // markers whose host x, m dx, dm hold (-1: none)
   size_t fused_lds = 0;    // its LDS bytes; 0 when steps do not fuse
   hipStream_t stream = nullptr;
   TimerRing tr;
@@ -510,6 +511,7 @@ int ensure_ibm(couplb_ctx* c, int n) {
     sl.rec = nr;
   }
   c->cap = cap;
+  c->hx_n = -1;
   c->order_n = -1;
   c->uf_epoch = -1;
   return COUPLB_OK;
@@ -1706,7 +1708,8 @@ int couplb_destroy(couplb_ctx* c) {
   (void)hipFree(c->dx); (void)hipFree(c->dv); (void)hipFree(c->dm); (void)hipFree(c->df);
   (void)hipFree(c->ufc);
   (void)hipFree(c->staging);
-  if (c->hstage) (void)hipHostFree(c->hstage);
+  if (c->up_done) (void)hipEventDestroy(c->up_done);
+  if (c->cstream) (void)hipStreamDestroy(c->cstream);
   (void)hipFree(c->pkey); (void)hipFree(c->pkey_s); (void)hipFree(c->pval); (void)hipFree(c->pval_s);
   (void)hipFree(c->pp); (void)hipFree(c->pw); (void)hipFree(c->ptmp);
   (void)hipFree(c->pheads); (void)hipFree(c->phlen); (void)hipFree(c->phnode); (void)hipFree(c->phflag);
@@ -1852,6 +1855,7 @@ int couplb_lbm_fuses_steps(const couplb_ctx* c) { return (c && c->fused_lds > 0)
 int couplb_ibm_substep_device(couplb_ctx* c, int n, const double* x, const double* v,
                               const double* m, double* f, const couplb_ibm_args* a) {
   if (!c) return COUPLB_EINVAL;
+  c->hx_n = -1;  // the next host sub-step's "unchanged" refers to these positions
   return ibm_impl(c, n, x, v, m, f, 0, a);
 }
 
@@ -1859,34 +1863,41 @@ int couplb_ibm_mdf_device(couplb_ctx* c, int n, const double* x, const double* V
                           double* f_body, const couplb_mdf_args* a, const couplb_mdf_learn* learn,
                           couplb_mdf_stats* st) {
   if (!c) return COUPLB_EINVAL;
+  c->hx_n = -1;
   return mdf_impl(c, n, x, V, F0, f_body, a, learn, st);
 }
 
 int couplb_ibm_substep(couplb_ctx* c, int n, const double* x, const double* v, const double* m,
                        double* f, const couplb_ibm_args* a) {
-  if (!c) return COUPLB_EINVAL;
+  if (!c || !a) return COUPLB_EINVAL;
   if (n <= 0) return COUPLB_OK;
   int rc = ensure_ibm(c, n);
   if (rc) return rc;
-  HIPCHK(c, hipMemcpyAsync(c->dx, x, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipMemcpyAsync(c->dv, v, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipMemcpyAsync(c->dm, m, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
-  rc = ibm_impl(c, n, c->dx, c->dv, c->dm, c->df, 1, a);
-  if (rc) return rc;
-  // the forces come back through a pinned staging buffer (DMA)
-  if ((size_t)3 * n > c->hstage_n) {
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    if (c->hstage) (void)hipHostFree(c->hstage);
-    c->hstage = nullptr;
-    c->hstage_n = 0;
-    HIPCHK(c, hipHostMalloc(reinterpret_cast<void**>(&c->hstage), sizeof(double) * 3 * (size_t)n,
-                            hipHostMallocDefault));
-    c->hstage_n = (size_t)3 * n;
+  if (!c->cstream) {
+    HIPCHK(c, hipStreamCreateWithFlags(&c->cstream, hipStreamNonBlocking));
+    HIPCHK(c, hipEventCreateWithFlags(&c->up_done, hipEventDisableTiming));
   }
-  double* F = c->hstage;
-  HIPCHK(c, hipMemcpyAsync(F, c->df, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+  // the arrays go up on the copy stream, so that after an LB step they
+  // travel while its kernel runs; positions the caller declares unchanged
+  // are already on the device from the previous host sub-step
+  const bool same = c->hx_n == n;  // dx, dm hold the previous host sub-step's x, m
+  if (!(a->positions_unchanged && same))
+    HIPCHK(c, hipMemcpyAsync(c->dx, x, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->cstream));
+  HIPCHK(c, hipMemcpyAsync(c->dv, v, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->cstream));
+  if (!(a->masses_unchanged && same))
+    HIPCHK(c, hipMemcpyAsync(c->dm, m, sizeof(double) * n, hipMemcpyHostToDevice, c->cstream));
+  if (!a->forces_zero)
+    HIPCHK(c, hipMemcpyAsync(c->df, f, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->cstream));
+  HIPCHK(c, hipEventRecord(c->up_done, c->cstream));
+  HIPCHK(c, hipStreamWaitEvent(c->stream, c->up_done, 0));
+  c->hx_n = -1;
+  // f += F on the device (fix_couplb.cpp:783-785), then back into f; with
+  // f declared zero the device writes F itself
+  rc = ibm_impl(c, n, c->dx, c->dv, c->dm, c->df, a->forces_zero ? 1 : 0, a);
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpyAsync(f, c->df, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  for (size_t i = 0; i < (size_t)3 * n; i++) f[i] += F[i];  // f[i] += F (fix_couplb.cpp:783-785)
+  c->hx_n = n;
   return COUPLB_OK;
 }
 

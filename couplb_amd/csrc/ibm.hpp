@@ -374,7 +374,7 @@ __device__ inline void combine_planes(double s[5]) {
 // (couplb_ibm.h:92-133), stored as a MarkerRec.  The per-plane partial sums
 // are combined in plane order, so results are deterministic; they differ
 // from the reference's single running sum by rounding only.
-// fmode 0: f_atom[idx] += F (device f); 1: f_atom[idx] = F (host adds).
+// fmode 0: f_atom[idx] += F (device f); 1: f_atom[idx] = 0.0 + F (f taken as zero).
 // Velocity cache (moving markers, later sub-steps of an LB interval): with
 // ucache set, each lane keeps the velocities of its 16 stencil nodes (three
 // doubles each, in stencil order) and cbase[p] the base cell and state they
@@ -466,7 +466,7 @@ __global__ void __launch_bounds__(256) k_marker(int n, const int* perm, const do
   const double Fz = mm * ip.xi_over_dtLB * (uf2 * ip.vel_scale - vp[2]);
   double* fa = f_atom + 3 * (size_t)idx;
   if (fmode == 0) { fa[0] += Fx; fa[1] += Fy; fa[2] += Fz; }
-  else { fa[0] = Fx; fa[1] = Fy; fa[2] = Fz; }
+  else { fa[0] = 0.0 + Fx; fa[1] = 0.0 + Fy; fa[2] = 0.0 + Fz; }
   MarkerRec r;
 #pragma unroll
   for (int t = 0; t < 4; t++) { r.wx[t] = wx[t]; r.wy[t] = wy[t]; r.wz[t] = wz[t]; }
@@ -507,7 +507,7 @@ __global__ void __launch_bounds__(256) k_marker_force(int n, const int* perm, co
   const double Fz = mm * ip.xi_over_dtLB * (uf2 * ip.vel_scale - vp[2]);
   double* fa = f_atom + 3 * (size_t)idx;
   if (fmode == 0) { fa[0] += Fx; fa[1] += Fy; fa[2] += Fz; }
-  else { fa[0] = Fx; fa[1] = Fy; fa[2] = Fz; }
+  else { fa[0] = 0.0 + Fx; fa[1] = 0.0 + Fy; fa[2] = 0.0 + Fz; }
   MarkerRec& r = rec[p];
   r.fx = r.fx + -Fx * ip.inv_N / ip.force_scale;
   r.fy = r.fy + -Fy * ip.inv_N / ip.force_scale;

@@ -466,8 +466,12 @@ void FixCoupLBCore::ibm_sub_coupling(const couplb_atoms& a, bool first) {
   }
   check(flush_steps(), "LB steps");
   ibm_has_particles = true;
+  // the group packed in order; f goes out as zeros and the returned F is
+  // added into the atoms' forces below (f += F, fix_couplb.cpp:783-785)
+  const bool same_n = pm.size() == (size_t)ng;
   px.resize(3 * (size_t)ng); pv.resize(3 * (size_t)ng); pm.resize(ng); pf.assign(3 * (size_t)ng, 0.0);
   pidx.resize(ng);
+  bool masses_same = same_n;
   int c = 0;
   for (int i = 0; i < a.nlocal; i++) {
     if (!(a.mask[i] & a.groupbit)) continue;
@@ -475,9 +479,10 @@ void FixCoupLBCore::ibm_sub_coupling(const couplb_atoms& a, bool first) {
     for (int d = 0; d < 3; d++) {
       px[3 * c + d] = a.x[3 * i + d];
       pv[3 * c + d] = a.v[3 * i + d];
-      pf[3 * c + d] = a.f[3 * i + d];
     }
-    pm[c] = a.rmass ? a.rmass[i] : a.mass[a.type[i]];
+    const double mi = a.rmass ? a.rmass[i] : a.mass[a.type[i]];
+    if (masses_same && std::memcmp(&pm[c], &mi, sizeof(double)) != 0) masses_same = false;
+    pm[c] = mi;
     c++;
   }
   couplb_ibm_args ia;
@@ -493,9 +498,12 @@ void FixCoupLBCore::ibm_sub_coupling(const couplb_atoms& a, bool first) {
       (px_prev.size() == px.size() && std::memcmp(px_prev.data(), px.data(), sizeof(double) * px.size()) == 0)
           ? 1 : 0;
   px_prev.assign(px.data(), px.data() + px.size());
+  ia.masses_unchanged = (masses_same && masses_sent) ? 1 : 0;
+  ia.forces_zero = 1;
   check(couplb_ibm_substep(ctx, ng, px.data(), pv.data(), pm.data(), pf.data(), &ia), "IBM sub-step");
+  masses_sent = true;
   for (int k = 0; k < ng; k++)
-    for (int d = 0; d < 3; d++) a.f[3 * pidx[k] + d] = pf[3 * k + d];
+    for (int d = 0; d < 3; d++) a.f[3 * pidx[k] + d] += pf[3 * k + d];
 }
 
 // FixCoupLB::setup, fix_couplb.cpp:551-638

@@ -130,6 +130,7 @@ class FixCoupLBCore {
 
   // packed group atoms in pinned host memory (the sub-step's copies to and
   // from the device run as DMA)
+  bool masses_sent = false;  // pm has been sent to the device by a host sub-step
   PinnedVec px, pv, pm, pf;
   std::vector<double> px_prev;  // positions of the previous sub-step (reuse test)
   std::vector<int> pidx;

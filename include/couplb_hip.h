@@ -151,9 +151,18 @@ typedef struct couplb_ibm_args {
   double force_scale;    /* rho*dx^(D+1)/dt_LBM^2 */
   int positions_unchanged; /* 1: x equals the previous sub-step's x, so the
                               marker ordering may be reused */
+  int masses_unchanged;    /* couplb_ibm_substep only (0 elsewhere): 1 when m
+                              equals the previous host sub-step's m, which is
+                              then not sent again */
+  int forces_zero;         /* couplb_ibm_substep only (0 elsewhere): 1 when
+                              f_inout holds zeros on entry, so it returns F
+                              without sending f to the device */
 } couplb_ibm_args;
 
-/* host arrays (copied over PCIe each call) */
+/* host arrays (copied over PCIe each call; x and m only when the flags above
+ * do not declare them unchanged, f only when not declared zero; the uploads
+ * run on a copy stream of their own, so after couplb_lbm_step they travel
+ * while the LB kernel runs) */
 int couplb_ibm_substep(couplb_ctx* ctx, int n, const double* x, const double* v,
                        const double* m, double* f_inout, const couplb_ibm_args* a);
 /* device arrays (already resident in HBM); f_inout is device memory.

@@ -299,3 +299,43 @@ def test_moving_markers_across_cells_match_oracle(reorder_every, monkeypatch):
     ferr = max(rel_l2(p, q) for p, q in zip(fd, fo))
     print(f"rel-L2(f) = {err:.3e}, rel-L2(F_atom) = {ferr:.3e}")
     assert err <= TOL and ferr <= 1e-11
+
+
+def test_host_array_flags_match_plain_host_calls():
+    """couplb_ibm_substep with positions / masses declared unchanged and the
+    forces declared zero (those arrays are then not sent) gives the same
+    forces and the same state, bit for bit, as the calls that send every
+    array, including a sub-step whose masses change."""
+    import couplb_amd
+    rng = np.random.default_rng(7)
+    n, N, md = 40, 16, 2
+    x0 = rng.uniform(2, N - 2, (n, 3))
+    v = rng.normal(0.0, 1e-3, (n, 3))
+    m = rng.uniform(0.5, 1.5, n)
+    kw = dict(lo=(0.0, 0.0, 0.0), inv_dx=1.0, xi_over_dtLB=1.0, vel_scale=1.0, inv_N=1.0 / md, force_scale=1.0)
+    devs = [couplb_amd.Device(3, N, N, N, tau=0.8, kernel=couplb_amd.PESKIN4, md_per_lb=md) for _ in range(2)]
+    for d in devs:
+        d.init_equilibrium(1.0, (0.01, 0.0, 0.0))
+    prev = None
+    for step in range(12):
+        if step % md == 0:
+            for d in devs:
+                d.step((1e-6, 0.0, 0.0), couplb_amd.RHO_REFRESHED_NEXT)
+        x = x0 + (0.3 * (step // 3)) * v / np.abs(v).max()   # positions change every third sub-step
+        mm = m * (1.5 if step >= 7 else 1.0)                  # masses change once
+        same_x = prev is not None and np.array_equal(prev, x)
+        prev = x.copy()
+        out = []
+        for k, d in enumerate(devs):
+            f = np.zeros((n, 3))
+            if k == 0:
+                same_m = step > 0 and step != 7
+                d.ibm(x, v, mm, f, positions_unchanged=same_x, masses_unchanged=same_m, forces_zero=True, **kw)
+            else:
+                d.ibm(x, v, mm, f, positions_unchanged=same_x, **kw)
+            out.append(f)
+        assert np.array_equal(out[0], out[1]), step
+    a, b = (d.get_state() for d in devs)
+    assert np.array_equal(a, b)
+    for d in devs:
+        d.close()

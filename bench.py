@@ -665,26 +665,29 @@ def main():
     hx = torch.from_numpy(x).pin_memory().numpy()
     hv = torch.from_numpy(v).pin_memory().numpy()
     hm = torch.from_numpy(m).pin_memory().numpy()
-    hf = torch.zeros((n, 3), dtype=torch.float64).pin_memory().numpy()
 
-    # the forces are cleared before every sub-step, as LAMMPS clears them
-    # before its force computation, and the fix is told so (f is then not
-    # sent to the device); the masses do not change
-    def one_step_host():
+    # the forces of every sub-step go into a cleared pinned buffer (LAMMPS
+    # clears its forces itself before each force computation) and the fix is
+    # told so (f is then not sent to the device); the masses do not change.
+    # The buffers are cleared before the timed steps.
+    host_steps = 8
+    hfs = [torch.zeros((n, 3), dtype=torch.float64).pin_memory().numpy() for _ in range(md * (host_steps + 1))]
+
+    def one_step_host(s):
         dev.step(g, couplb_amd.RHO_REFRESHED_NEXT)
-        for _ in range(md):
-            hf[:] = 0.0
-            dev.ibm(hx, hv, hm, hf, positions_unchanged=True, masses_unchanged=True, forces_zero=True,
-                    **ibm_kw)
+        for k in range(md):
+            dev.ibm(hx, hv, hm, hfs[s * md + k], positions_unchanged=True, masses_unchanged=True,
+                    forces_zero=True, **ibm_kw)
 
-    host_steps = max(8, args.steps // 4)
-    one_step_host()
+    one_step_host(0)
     dev.sync()
     t3 = time.perf_counter()
-    for _ in range(host_steps):
-        one_step_host()
+    for st in range(1, host_steps + 1):
+        one_step_host(st)
     dev.sync()
     mlups_host = nodes * host_steps / (time.perf_counter() - t3) / 1e6
+    for b in hfs:
+        b[:] = 0.0
 
     # moving particles with the marker arrays in host memory: the positions
     # of every MD sub-step (advanced by v * dt_MD as above) are prepared in
@@ -696,8 +699,7 @@ def main():
     def one_step_moving_host(s):
         dev.step(g, couplb_amd.RHO_REFRESHED_NEXT)
         for k in range(md):
-            hf[:] = 0.0
-            dev.ibm(xs[s * md + k], hv, hm, hf, positions_unchanged=False, masses_unchanged=True,
+            dev.ibm(xs[s * md + k], hv, hm, hfs[s * md + k], positions_unchanged=False, masses_unchanged=True,
                     forces_zero=True, **ibm_kw)
 
     one_step_moving_host(0)
@@ -707,7 +709,7 @@ def main():
         one_step_moving_host(st)
     dev.sync()
     mlups_moving_host = nodes * mh_steps / (time.perf_counter() - t4) / 1e6
-    del xs
+    del xs, hfs
 
     # achievable-bandwidth reference: a plain copy of one population buffer's
     # worth (k_step reads one state and writes the other)

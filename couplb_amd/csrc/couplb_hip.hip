@@ -106,6 +106,7 @@ struct couplb_ctx {
   hipStream_t cstream = nullptr;  // host-array uploads of the IBM sub-steps
   hipEvent_t up_done = nullptr;   // end of the latest upload on cstream
   int hx_n = -1;                  // markers whose host x, m dx, dm hold (-1: none)
+  int host_cycle = -1;            // state of the latest host sub-step
   size_t fused_lds = 0;    // its LDS bytes; 0 when steps do not fuse
   hipStream_t stream = nullptr;
   TimerRing tr;
@@ -1877,20 +1878,26 @@ int couplb_ibm_substep(couplb_ctx* c, int n, const double* x, const double* v, c
     HIPCHK(c, hipStreamCreateWithFlags(&c->cstream, hipStreamNonBlocking));
     HIPCHK(c, hipEventCreateWithFlags(&c->up_done, hipEventDisableTiming));
   }
-  // the arrays go up on the copy stream, so that after an LB step they
-  // travel while its kernel runs; positions the caller declares unchanged
+  // after an LB step the arrays go up on the copy stream, so that they
+  // travel while its kernel runs (otherwise on the context stream, with no
+  // cross-stream wait); positions and masses the caller declares unchanged
   // are already on the device from the previous host sub-step
   const bool same = c->hx_n == n;  // dx, dm hold the previous host sub-step's x, m
+  const bool overlap = c->cycle != c->host_cycle;
+  hipStream_t up = overlap ? c->cstream : c->stream;
   if (!(a->positions_unchanged && same))
-    HIPCHK(c, hipMemcpyAsync(c->dx, x, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->cstream));
-  HIPCHK(c, hipMemcpyAsync(c->dv, v, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->cstream));
+    HIPCHK(c, hipMemcpyAsync(c->dx, x, sizeof(double) * 3 * n, hipMemcpyHostToDevice, up));
+  HIPCHK(c, hipMemcpyAsync(c->dv, v, sizeof(double) * 3 * n, hipMemcpyHostToDevice, up));
   if (!(a->masses_unchanged && same))
-    HIPCHK(c, hipMemcpyAsync(c->dm, m, sizeof(double) * n, hipMemcpyHostToDevice, c->cstream));
+    HIPCHK(c, hipMemcpyAsync(c->dm, m, sizeof(double) * n, hipMemcpyHostToDevice, up));
   if (!a->forces_zero)
-    HIPCHK(c, hipMemcpyAsync(c->df, f, sizeof(double) * 3 * n, hipMemcpyHostToDevice, c->cstream));
-  HIPCHK(c, hipEventRecord(c->up_done, c->cstream));
-  HIPCHK(c, hipStreamWaitEvent(c->stream, c->up_done, 0));
+    HIPCHK(c, hipMemcpyAsync(c->df, f, sizeof(double) * 3 * n, hipMemcpyHostToDevice, up));
+  if (overlap) {
+    HIPCHK(c, hipEventRecord(c->up_done, c->cstream));
+    HIPCHK(c, hipStreamWaitEvent(c->stream, c->up_done, 0));
+  }
   c->hx_n = -1;
+  c->host_cycle = c->cycle;
   // f += F on the device (fix_couplb.cpp:783-785), then back into f; with
   // f declared zero the device writes F itself
   rc = ibm_impl(c, n, c->dx, c->dv, c->dm, c->df, a->forces_zero ? 1 : 0, a);

@@ -307,7 +307,9 @@ def workload_c2(steps):
 
 def workload_c4(steps, n=384):
     """Taylor-Green vortex n^3 (z-invariant), u0 = 0.03, k = 2 pi / n,
-    nu = 0.02: FP64 against FP32 / FP16 / BF16 deviation storage."""
+    nu = 0.02: FP64 against FP32 / FP16 / BF16 deviation storage and the
+    16-bit fixed point (range 0.0075: the largest |f - w rho0| of this
+    vortex is 0.00515)."""
     import couplb_amd
     nu, u0 = 0.02, 0.03
     k = 2 * np.pi / n
@@ -327,9 +329,10 @@ def workload_c4(steps, n=384):
         e = 0.5 * float((r * (a * a + b * b + cz * cz)).sum())
         return e, np.stack([a, b, cz])
 
-    for storage in ("fp64", "fp32", "fp16", "bf16"):
-        d = couplb_amd.Device(3, n, n, n, tau=3 * nu + 0.5, storage=storage)
+    for storage in ("fp64", "fp32", "fp16", "bf16", "i16"):
+        d = couplb_amd.Device(3, n, n, n, tau=3 * nu + 0.5, storage=storage, storage_range=0.0075)
         d.set_state(f0)
+        m0 = d.diagnostics()["mass"]
         e0, _ = ke(d)
         d.sync()
         t0 = time.perf_counter()
@@ -340,7 +343,7 @@ def workload_c4(steps, n=384):
         mass = d.diagnostics()["mass"]
         d.close()
         r = dict(mlups=round(n ** 3 * steps / dt / 1e6, 1), ke_ratio=e1 / e0,
-                 ke_ratio_rel_err_vs_analytic=(e1 / e0) / decay - 1.0, mass=mass)
+                 ke_ratio_rel_err_vs_analytic=(e1 / e0) / decay - 1.0, mass=mass, mass_drift=mass / m0 - 1.0)
         if ref is None:
             ref = u
         else:

@@ -598,6 +598,8 @@ int fold_slots(couplb_ctx* c);
 int ibm_impl(couplb_ctx* c, int n, const double* x, const double* v, const double* m,
              double* f_dev, int fmode, const couplb_ibm_args* a) {
   if (n < 0 || !a) return fail(c, COUPLB_EINVAL, "bad IBM arguments");
+  // binned records carry the marker index in 24 bits (spread_rows.hpp)
+  if (n >= (1 << REC_SLOT_SHIFT)) return fail(c, COUPLB_EINVAL, "an IBM group holds at most 16777215 atoms");
   if (n == 0) return COUPLB_OK;
   int rc = ensure_ibm(c, n);
   if (rc) return rc;

@@ -403,14 +403,42 @@ struct StepArgs {
   int has_f2;           // pair_mode 2: F4 applies where flag2 holds this tag (1..255)
 };
 
+#ifndef COUPLB_LDS_FORCE_FIRST
+#define COUPLB_LDS_FORCE_FIRST 1
+#endif
+
+// pairs in flight per round of the run sums in the reduced formats' steps
+#ifndef COUPLB_RUN_U_REDUCED
+#define COUPLB_RUN_U_REDUCED 2
+#endif
+constexpr int RUN_U_REDUCED = COUPLB_RUN_U_REDUCED;
+
 // Sum of one node's run of the moving-marker map, sum_f con[idx[f]] in pair
 // order (con: four doubles per entry, the last unused).
+template <int U = 4>
 __device__ inline void run_sum_con(int s, int len, const int* idx, const double* con, double& ax, double& ay,
                                    double& az) {
 #pragma clang fp contract(off)
   ax = ay = az = 0.0;
   const double4* c4 = reinterpret_cast<const double4*>(con);
-  for (int f = s; f < s + len; f++) {
+  int f = s;
+  const int end = s + len;
+  // U entries in flight per round; the sums are taken in pair order either way
+  for (; U > 1 && f + U <= end; f += U) {
+    int q[U];
+    double4 v[U];
+#pragma unroll
+    for (int i = 0; i < U; i++) q[i] = idx[f + i];
+#pragma unroll
+    for (int i = 0; i < U; i++) v[i] = c4[q[i]];
+#pragma unroll
+    for (int i = 0; i < U; i++) {
+      ax += v[i].x;
+      ay += v[i].y;
+      az += v[i].z;
+    }
+  }
+  for (; f < end; f++) {
     const double4 v = c4[idx[f]];
     ax += v.x;
     ay += v.y;
@@ -420,8 +448,8 @@ __device__ inline void run_sum_con(int s, int len, const int* idx, const double*
 
 // Sum of one node's run of the fixed-marker map, sum_f src[pp[f]] * pw[f]
 // in pair order (couplb_ibm.h:125-130: F_node += F_marker * w); src holds
-// three values per sorted marker.  U pairs (1 or 4) are loaded per round;
-// the sums are the same.
+// three values per sorted marker.  U pairs are loaded per round; the sums
+// are the same.
 template <int U = 4>
 __device__ inline void run_sum(int s, int len, const int* pp, const double* pw, const double* src,
                                double& ax, double& ay, double& az) {
@@ -429,17 +457,17 @@ __device__ inline void run_sum(int s, int len, const int* pp, const double* pw, 
   ax = ay = az = 0.0;
   int f = s;
   const int end = s + len;
-  for (; U > 1 && f + 4 <= end; f += 4) {
-    int q[4];
-    double w[4], v[4][3];
+  for (; U > 1 && f + U <= end; f += U) {
+    int q[U];
+    double w[U], v[U][3];
 #pragma unroll
-    for (int i = 0; i < 4; i++) { q[i] = pp[f + i]; w[i] = pw[f + i]; }
+    for (int i = 0; i < U; i++) { q[i] = pp[f + i]; w[i] = pw[f + i]; }
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < U; i++)
 #pragma unroll
       for (int c = 0; c < 3; c++) v[i][c] = src[3 * (size_t)q[i] + c];
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
+    for (int i = 0; i < U; i++) {
       ax += v[i][0] * w[i];
       ay += v[i][1] * w[i];
       az += v[i][2] * w[i];
@@ -466,23 +494,23 @@ __device__ inline void run_sum(int s, int len, const int* pp, const double* pw, 
 // PM 2: from their run of the moving-marker map (a.pcon), plus a.F4 where
 // a.flag2 is set; separate instantiations, so the other steps keep their
 // registers.
-template <int D, typename S, int PM = 0, typename Out>
-__device__ inline void step_node(const StepArgs& a, const Layout& lay, S* dst, const double* rho_in,
-                                 double* rho_out, int i, int j, int k, unsigned node,
-                                 typename Store<S>::T (&f)[Lat<D>::Q], Out out) {
-  using L = Lat<D>;
+// IBM force spread by the last cycle's markers, present only at flagged
+// nodes: the node's pair run (PM 1: fixed-marker map; PM 2: moving-marker
+// map, plus the tile-spread F4 where flag2 holds the step's tag) or the tile
+// spread's F4 (PM 0).
+template <int D, typename S, int PM>
+__device__ inline void ibm_node_force(const StepArgs& a, unsigned node, typename Store<S>::T& Fx,
+                                      typename Store<S>::T& Fy, typename Store<S>::T& Fz) {
   using T = typename Store<S>::T;
-  constexpr int Q = L::Q;
-  const Geo& g = a.pc.g;
-  // IBM force spread by the last cycle's markers, present only at flagged
-  // nodes: the node's pair run (fixed-marker map) or the tile spread's F4
-  T Fx = 0, Fy = 0, Fz = 0;
+  Fx = 0;
+  Fy = 0;
+  Fz = 0;
   if constexpr (PM == 2) {
     if (a.has_ibm) {
       double ax = 0.0, ay = 0.0, az = 0.0;
       if (a.flag[node]) {
         const int2 r = a.prun[node];
-        run_sum_con(r.x, r.y, a.pidx, a.pcon, ax, ay, az);
+        run_sum_con<Store<S>::deviation ? RUN_U_REDUCED : 4>(r.x, r.y, a.pidx, a.pcon, ax, ay, az);
       }
       if (a.has_f2 && a.flag2[node] == (unsigned char)a.has_f2) {
 #pragma clang fp contract(off)
@@ -500,8 +528,8 @@ __device__ inline void step_node(const StepArgs& a, const Layout& lay, S* dst, c
       const int2 r = a.prun[node];
       double ax, ay, az;
       // FP64: four pairs in flight (measured faster); the reduced formats
-      // keep their occupancy with one
-      run_sum<Store<S>::deviation ? 1 : 4>(r.x, r.y, a.pp, a.pw, a.fsrc, ax, ay, az);
+      // keep their occupancy with fewer
+      run_sum<Store<S>::deviation ? RUN_U_REDUCED : 4>(r.x, r.y, a.pp, a.pw, a.fsrc, ax, ay, az);
       Fx = (T)ax;
       Fy = (T)ay;
       Fz = (T)az;
@@ -511,6 +539,26 @@ __device__ inline void step_node(const StepArgs& a, const Layout& lay, S* dst, c
       Fy = (T)F.y;
       Fz = (T)F.z;
     }
+  }
+}
+
+template <int D, typename S, int PM = 0, typename Out>
+__device__ inline void step_node(const StepArgs& a, const Layout& lay, S* dst, const double* rho_in,
+                                 double* rho_out, int i, int j, int k, unsigned node,
+                                 typename Store<S>::T (&f)[Lat<D>::Q], Out out,
+                                 const typename Store<S>::T* Fpre = nullptr) {
+  using L = Lat<D>;
+  using T = typename Store<S>::T;
+  constexpr int Q = L::Q;
+  const Geo& g = a.pc.g;
+  // IBM force (ibm_node_force), or the one the caller loaded beforehand
+  T Fx, Fy, Fz;
+  if (Fpre) {
+    Fx = Fpre[0];
+    Fy = Fpre[1];
+    Fz = Fpre[2];
+  } else {
+    ibm_node_force<D, S, PM>(a, node, Fx, Fy, Fz);
   }
   // density (deviation form for the reduced formats: A = rho - base).  The
   // reduced formats' populations arrive in their stored scale s (a power of
@@ -628,8 +676,9 @@ __device__ inline void step_node(const StepArgs& a, const Layout& lay, S* dst, c
 
 template <int D, typename S, int PM = 0, typename Out>
 __device__ inline void step_node(const StepArgs& a, int i, int j, int k, unsigned node,
-                                 typename Store<S>::T (&f)[Lat<D>::Q], Out out) {
-  step_node<D, S, PM>(a, a.pc.lay, static_cast<S*>(a.dst), a.pc.rho_coll, a.rho_out, i, j, k, node, f, out);
+                                 typename Store<S>::T (&f)[Lat<D>::Q], Out out,
+                                 const typename Store<S>::T* Fpre = nullptr) {
+  step_node<D, S, PM>(a, a.pc.lay, static_cast<S*>(a.dst), a.pc.rho_coll, a.rho_out, i, j, k, node, f, out, Fpre);
 }
 
 template <int D, typename S, int PM = 0>
@@ -792,19 +841,30 @@ __global__ void __launch_bounds__(lds_chunk<S>()) k_step_lds(StepArgs a) {
   const Layout lay = a.pc.lay;
   const int x0 = blockIdx.x * LDS_XC, j = blockIdx.y, k = blockIdx.z;
   const int xc = (g.nx - x0 < LDS_XC) ? g.nx - x0 : LDS_XC;
-  stage_rows<D, S, LDS_XC>(g, lay, static_cast<const S*>(a.pc.src), lds, x0, xc, j, k);
-  __syncthreads();
   const int t = threadIdx.x;
   const bool live = t < xc;
   const int i = x0 + t;
   const unsigned node = (unsigned)i + (unsigned)g.nx * ((unsigned)j + (unsigned)g.ny * (unsigned)k);
+#if COUPLB_LDS_FORCE_FIRST
+  // the IBM force's dependent loads go first; the row staging follows
+  T Fp[3] = {0, 0, 0};
+  if (PM != 0 && live) ibm_node_force<D, S, PM>(a, node, Fp[0], Fp[1], Fp[2]);
+#endif
+  stage_rows<D, S, LDS_XC>(g, lay, static_cast<const S*>(a.pc.src), lds, x0, xc, j, k);
+  __syncthreads();
   T f[Q];
   if (live)
     pull_node_with<D, S, true>(a.pc, i, j, k, f, [&](int q, int, int, int) {
       return Store<S>::decode(win[q][t + G - L::ex(q)]);
     });
   __syncthreads();
+#if COUPLB_LDS_FORCE_FIRST
+  if (live)
+    step_node<D, S, PM>(a, i, j, k, node, f, [&](int q, T v) { res[q][t] = Store<S>::encode(v); },
+                        PM != 0 ? Fp : nullptr);
+#else
   if (live) step_node<D, S, PM>(a, i, j, k, node, f, [&](int q, T v) { res[q][t] = Store<S>::encode(v); });
+#endif
   __syncthreads();
   S* dst = static_cast<S*>(a.dst);
   const unsigned row = (unsigned)j + (unsigned)g.ny * (unsigned)k;

@@ -144,7 +144,9 @@ struct couplb_ctx {
   int pair_epoch = -1;       // ordering the pairs (and the node flags) were built for
   int *pheads = nullptr, *phlen = nullptr;  // node runs of the sorted pairs
   unsigned* phnode = nullptr;
-  int2* prun = nullptr;            // per node: (first pair, count) of its run
+  unsigned short* plen = nullptr;  // per node: pair count of its run (0: none)
+  int* pseg = nullptr;             // per 64-node segment of an x-row: its first pair
+  int runs_long = 0;               // a run of the map is longer than plen holds
   unsigned char* phflag = nullptr;
   int* pcount = nullptr;           // device count of runs
   int pnh = 0;                     // number of runs
@@ -704,7 +706,9 @@ int ensure_pairs(couplb_ctx* c, long ne) {
 }
 
 // Node runs of the sorted pairs: first pair of each run, its length and
-// node, and the per-node run index prun.
+// node; for the LB step the run length of every node and the first pair of
+// every 64-node segment of an x-row (kernels.hpp pair_run).  Sets runs_long
+// where a node has more pairs than plen holds (then the map is not used).
 int build_runs(couplb_ctx* c, long ne) {
   const int tb = 256;
   const unsigned nb = (unsigned)((ne + tb - 1) / tb);
@@ -719,7 +723,7 @@ int build_runs(couplb_ctx* c, long ne) {
     HIPCHK(c, hipMalloc(&c->phlen, sizeof(int) * ne));
     HIPCHK(c, hipMalloc(&c->phnode, sizeof(unsigned) * ne));
     HIPCHK(c, hipMalloc(&c->phflag, ne));
-    if (!c->pcount) HIPCHK(c, hipMalloc(&c->pcount, sizeof(int)));
+    if (!c->pcount) HIPCHK(c, hipMalloc(&c->pcount, 2 * sizeof(int)));  // run count, long-run flag
     size_t bytes = 0;
     HIPCHK(c, rocprim::select(nullptr, bytes, rocprim::counting_iterator<int>(0), c->phflag, c->pheads,
                               c->pcount, (size_t)ne, c->stream));
@@ -727,24 +731,32 @@ int build_runs(couplb_ctx* c, long ne) {
     c->seltmp_bytes = bytes;
     c->pcap_runs = ne;
   }
+  const int nsx = (c->geo.nx + 63) / 64;
+  const long nseg = (long)nsx * (c->N / c->geo.nx);
+  if (!c->plen) {
+    HIPCHK(c, hipMalloc(&c->plen, sizeof(unsigned short) * c->N));
+    HIPCHK(c, hipMalloc(&c->pseg, sizeof(int) * nseg));
+  }
   hipLaunchKernelGGL(k_pair_heads, dim3(nb), dim3(tb), 0, c->stream, ne, (const unsigned*)c->pkey_s, sentinel,
                      c->phflag);
   size_t sbytes = c->seltmp_bytes;
   HIPCHK(c, rocprim::select(c->seltmp, sbytes, rocprim::counting_iterator<int>(0), c->phflag, c->pheads,
                             c->pcount, (size_t)ne, c->stream));
-  int nh = 0;
-  HIPCHK(c, hipMemcpyAsync(&nh, c->pcount, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  hipLaunchKernelGGL(k_fill_u8, dim3(fill_u8_blocks(sizeof(unsigned short) * c->N)), dim3(256), 0, c->stream,
+                     reinterpret_cast<unsigned char*>(c->plen), sizeof(unsigned short) * c->N, (unsigned char)0);
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, c->stream, c->pcount + 1, (size_t)1, 0);
+  // the runs number at most ne: the kernel reads their count on the device
+  hipLaunchKernelGGL(k_head_info, dim3(nb), dim3(tb), 0, c->stream, ne, (const int*)c->pcount,
+                     (const int*)c->pheads, (const unsigned*)c->pkey_s, ne, c->phlen, c->phnode, c->plen,
+                     c->pcount + 1);
+  hipLaunchKernelGGL(k_seg_start, dim3((unsigned)((nseg + tb - 1) / tb)), dim3(tb), 0, c->stream, nseg, nsx,
+                     c->geo.nx, (const unsigned*)c->pkey_s, ne, c->pseg);
+  HIPCHK(c, hipGetLastError());
+  int info[2] = {0, 0};
+  HIPCHK(c, hipMemcpyAsync(info, c->pcount, sizeof(info), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  if (nh > 0)
-    hipLaunchKernelGGL(k_head_info, dim3((nh + tb - 1) / tb), dim3(tb), 0, c->stream, nh,
-                       (const int*)c->pheads, (const unsigned*)c->pkey_s, ne, c->phlen, c->phnode);
-  HIPCHK(c, hipGetLastError());
-  if (!c->prun) HIPCHK(c, hipMalloc(&c->prun, sizeof(int2) * c->N));
-  if (nh > 0)
-    hipLaunchKernelGGL(k_run_index, dim3((nh + tb - 1) / tb), dim3(tb), 0, c->stream, nh,
-                       (const int*)c->pheads, (const int*)c->phlen, (const unsigned*)c->phnode, c->prun);
-  HIPCHK(c, hipGetLastError());
-  c->pnh = nh;
+  c->pnh = info[0];
+  c->runs_long = info[1];
   c->pne = ne;
   return COUPLB_OK;
 }
@@ -1387,7 +1399,8 @@ StepArgs step_args(couplb_ctx* c, const double g[3], int flags) {
   a.flag = c->flag;
   a.F4 = c->F4;
   a.pair_mode = 0;
-  a.prun = nullptr;
+  a.plen = nullptr;
+  a.pseg = nullptr;
   a.pp = nullptr;
   a.pw = nullptr;
   a.fsrc = nullptr;
@@ -1469,7 +1482,7 @@ int step_impl(couplb_ctx* c, const double g[3], int flags, bool timed) {
   const int dst = c->cur ^ 1;
   const int S = (c->D == 3) ? stencil_size<3>() : stencil_size<2>();
   // markers at the positions of the previous LB interval: constant map
-  const bool fixed = !c->f4_partial && c->nslots == 1 && c->slots[0].epoch == c->prev_step_epoch &&
+  bool fixed = !c->f4_partial && c->nslots == 1 && c->slots[0].epoch == c->prev_step_epoch &&
                      !c->kept_move && (long)c->slots[0].n * S < (long)INT32_MAX;
   // moving markers whose sub-steps all kept one ordering: the moving-marker
   // map of that ordering, the records that left their base cell spread apart
@@ -1478,15 +1491,21 @@ int step_impl(couplb_ctx* c, const double g[3], int flags, bool timed) {
   for (int s = 0; moving && s < c->nslots; s++)
     moving = c->slots[s].epoch == c->pos_epoch && c->slots[s].n == c->order_n;
   int tspread = -1;
+  if ((moving || fixed) && timed) {
+    const int rc = timer_begin(c, 2, &tspread);
+    if (rc) return rc;
+  }
+  if (moving && c->mpair_epoch != c->pos_epoch) {
+    const int rc = build_moving_map(c, c->order_n);
+    if (rc) return rc;
+  } else if (fixed && c->pair_epoch != c->slots[0].epoch) {
+    const int rc = build_pairs(c, c->slots[0]);
+    if (rc) return rc;
+  }
+  // a node with more pairs than a run length holds (some thousand markers
+  // per cell): spread through the node rows instead
+  if ((moving || fixed) && c->runs_long) moving = fixed = false;
   if (moving) {
-    if (timed) {
-      const int rc = timer_begin(c, 2, &tspread);
-      if (rc) return rc;
-    }
-    if (c->mpair_epoch != c->pos_epoch) {
-      const int rc = build_moving_map(c, c->order_n);
-      if (rc) return rc;
-    }
     // the pair forces of the records on their ordered base cell (the same
     // kernel bins the others), then those others spread into F4 where flag2
     // is set
@@ -1512,23 +1531,17 @@ int step_impl(couplb_ctx* c, const double g[3], int flags, bool timed) {
     const int rc = row_spread(c, c->F4, c->flag2, 1, 0, true, &con);
     if (rc) return rc;
     a.pair_mode = 2;
-    a.prun = c->prun;
+    a.plen = c->plen;
+    a.pseg = c->pseg;
     a.pcon = c->pcon;
     a.pidx = c->pval_s;
     a.flag2 = c->flag2;
     a.has_f2 = c->f2_tag;
   } else if (fixed) {
-    if (timed) {
-      const int rc = timer_begin(c, 2, &tspread);
-      if (rc) return rc;
-    }
-    if (c->pair_epoch != c->slots[0].epoch) {
-      const int rc = build_pairs(c, c->slots[0]);
-      if (rc) return rc;
-    }
-    // the step sums each flagged node's pair run itself (no node-force pass)
+    // the step sums each node's pair run itself (no node-force pass)
     a.pair_mode = 1;
-    a.prun = c->prun;
+    a.plen = c->plen;
+    a.pseg = c->pseg;
     a.pp = c->pp;
     a.pw = c->pw;
     a.fsrc = c->fsrc;
@@ -1536,7 +1549,7 @@ int step_impl(couplb_ctx* c, const double g[3], int flags, bool timed) {
     // spread the pending marker records into the per-node force
     c->pair_epoch = -1;  // the spreading rewrites the node flags
     c->mpair_epoch = -1;
-    if (timed) {
+    if (timed && tspread < 0) {
       const int rc = timer_begin(c, 2, &tspread);
       if (rc) return rc;
     }
@@ -1716,7 +1729,8 @@ int couplb_destroy(couplb_ctx* c) {
   (void)hipFree(c->pkey); (void)hipFree(c->pkey_s); (void)hipFree(c->pval); (void)hipFree(c->pval_s);
   (void)hipFree(c->pp); (void)hipFree(c->pw); (void)hipFree(c->ptmp);
   (void)hipFree(c->pheads); (void)hipFree(c->phlen); (void)hipFree(c->phnode); (void)hipFree(c->phflag);
-  (void)hipFree(c->pcount); (void)hipFree(c->seltmp); (void)hipFree(c->fsrc); (void)hipFree(c->prun);
+  (void)hipFree(c->pcount); (void)hipFree(c->seltmp); (void)hipFree(c->fsrc); (void)hipFree(c->plen);
+  (void)hipFree(c->pseg);
   (void)hipFree(c->ut); (void)hipFree(c->mdf_v); (void)hipFree(c->mdf_S);
   (void)hipFree(c->mdf_F); (void)hipFree(c->mdf_R); (void)hipFree(c->mdf_Ap); (void)hipFree(c->mdf_part);
   (void)hipFree(c->mlp_x); (void)hipFree(c->mlp_y); (void)hipFree(c->mlp_sig);

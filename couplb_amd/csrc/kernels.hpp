@@ -384,11 +384,14 @@ struct StepArgs {
   int has_ibm;          // spread force of the last cycle present
   const unsigned char* flag;
   const double* F4;    // node force (fx, fy, fz, unused) where flag is set
-  // fixed-marker map: instead of F4, the step sums the flagged node's run of
-  // (marker, weight) pairs itself: prun[node] = (first pair, count), pp/pw
-  // the sorted pairs, fsrc three reaction values per sorted marker
+  // fixed-marker map: instead of F4, the step sums the node's run of
+  // (marker, weight) pairs itself: the runs of a 64-node segment of an x-row
+  // start at pseg[segment] and follow each other in node order, plen giving
+  // their lengths (pair_run); pp/pw the sorted pairs, fsrc three reaction
+  // values per sorted marker
   int pair_mode;        // 0 F4 at flagged nodes, 1 fixed-marker map, 2 moving-marker map
-  const int2* prun;
+  const unsigned short* plen;  // pair run length per node (0: no pairs)
+  const int* pseg;             // first pair of each 64-node segment of an x-row
   const int* pp;
   const double* pw;
   const double* fsrc;
@@ -402,10 +405,6 @@ struct StepArgs {
   const unsigned char* flag2;
   int has_f2;           // pair_mode 2: F4 applies where flag2 holds this tag (1..255)
 };
-
-#ifndef COUPLB_LDS_FORCE_FIRST
-#define COUPLB_LDS_FORCE_FIRST 1
-#endif
 
 // pairs in flight per round of the run sums in the reduced formats' steps
 #ifndef COUPLB_RUN_U_REDUCED
@@ -490,16 +489,37 @@ __device__ inline void run_sum(int s, int len, const int* pp, const double* pw, 
 // store through out); rho_in, rho_out: the density array read for gravity
 // and the one written (a.pc.lay, a.dst, a.pc.rho_coll, a.rho_out in the
 // single-step kernels).  PM 1: flagged nodes take their force from their
-// run of the fixed-marker map (a.prun, a.pp, a.pw, a.fsrc) instead of a.F4;
+// run of the fixed-marker map (pair_run, a.pp, a.pw, a.fsrc) instead of a.F4;
 // PM 2: from their run of the moving-marker map (a.pcon), plus a.F4 where
 // a.flag2 is set; separate instantiations, so the other steps keep their
 // registers.
+// The pair run (first pair, length) of interior node (i, row) of a pair map:
+// its segment's first pair plus the lengths of the runs of the lanes before
+// it.  Wavefront-collective: every lane of the wavefront calls it, the 64
+// lanes hold 64 consecutive x of one row starting at a multiple of 64, and
+// lanes past the row's end pass live = false.
+__device__ inline int2 pair_run(const StepArgs& a, int i, unsigned row, bool live) {
+  const Geo& g = a.pc.g;
+  const unsigned nsx = ((unsigned)g.nx + 63u) >> 6;
+  const int len = live ? (int)a.plen[(size_t)i + (size_t)g.nx * row] : 0;
+  int incl = len;
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(incl, o);
+    if (lane >= o) incl += y;
+  }
+  const int i0 = i & ~63;  // the same segment for every lane
+  const int first = (i0 < g.nx) ? a.pseg[(size_t)row * nsx + (unsigned)(i0 >> 6)] : 0;
+  return int2{first + incl - len, len};
+}
+
 // IBM force spread by the last cycle's markers, present only at flagged
 // nodes: the node's pair run (PM 1: fixed-marker map; PM 2: moving-marker
 // map, plus the tile-spread F4 where flag2 holds the step's tag) or the tile
 // spread's F4 (PM 0).
 template <int D, typename S, int PM>
-__device__ inline void ibm_node_force(const StepArgs& a, unsigned node, typename Store<S>::T& Fx,
+__device__ inline void ibm_node_force(const StepArgs& a, unsigned node, int2 r, typename Store<S>::T& Fx,
                                       typename Store<S>::T& Fy, typename Store<S>::T& Fz) {
   using T = typename Store<S>::T;
   Fx = 0;
@@ -508,10 +528,7 @@ __device__ inline void ibm_node_force(const StepArgs& a, unsigned node, typename
   if constexpr (PM == 2) {
     if (a.has_ibm) {
       double ax = 0.0, ay = 0.0, az = 0.0;
-      if (a.flag[node]) {
-        const int2 r = a.prun[node];
-        run_sum_con<Store<S>::deviation ? RUN_U_REDUCED : 4>(r.x, r.y, a.pidx, a.pcon, ax, ay, az);
-      }
+      if (r.y > 0) run_sum_con<Store<S>::deviation ? RUN_U_REDUCED : 4>(r.x, r.y, a.pidx, a.pcon, ax, ay, az);
       if (a.has_f2 && a.flag2[node] == (unsigned char)a.has_f2) {
 #pragma clang fp contract(off)
         const double4 F = *reinterpret_cast<const double4*>(a.F4 + 4 * (size_t)node);
@@ -523,9 +540,8 @@ __device__ inline void ibm_node_force(const StepArgs& a, unsigned node, typename
       Fy = (T)ay;
       Fz = (T)az;
     }
-  } else if (a.has_ibm && a.flag[node]) {
-    if constexpr (PM == 1) {
-      const int2 r = a.prun[node];
+  } else if constexpr (PM == 1) {
+    if (a.has_ibm && r.y > 0) {
       double ax, ay, az;
       // FP64: four pairs in flight (measured faster); the reduced formats
       // keep their occupancy with fewer
@@ -533,12 +549,12 @@ __device__ inline void ibm_node_force(const StepArgs& a, unsigned node, typename
       Fx = (T)ax;
       Fy = (T)ay;
       Fz = (T)az;
-    } else {
-      const double4 F = *reinterpret_cast<const double4*>(a.F4 + 4 * (size_t)node);
-      Fx = (T)F.x;
-      Fy = (T)F.y;
-      Fz = (T)F.z;
     }
+  } else if (a.has_ibm && a.flag[node]) {
+    const double4 F = *reinterpret_cast<const double4*>(a.F4 + 4 * (size_t)node);
+    Fx = (T)F.x;
+    Fy = (T)F.y;
+    Fz = (T)F.z;
   }
 }
 
@@ -546,7 +562,7 @@ template <int D, typename S, int PM = 0, typename Out>
 __device__ inline void step_node(const StepArgs& a, const Layout& lay, S* dst, const double* rho_in,
                                  double* rho_out, int i, int j, int k, unsigned node,
                                  typename Store<S>::T (&f)[Lat<D>::Q], Out out,
-                                 const typename Store<S>::T* Fpre = nullptr) {
+                                 const typename Store<S>::T* Fpre = nullptr, int2 run = int2{0, 0}) {
   using L = Lat<D>;
   using T = typename Store<S>::T;
   constexpr int Q = L::Q;
@@ -558,7 +574,7 @@ __device__ inline void step_node(const StepArgs& a, const Layout& lay, S* dst, c
     Fy = Fpre[1];
     Fz = Fpre[2];
   } else {
-    ibm_node_force<D, S, PM>(a, node, Fx, Fy, Fz);
+    ibm_node_force<D, S, PM>(a, node, run, Fx, Fy, Fz);
   }
   // density (deviation form for the reduced formats: A = rho - base).  The
   // reduced formats' populations arrive in their stored scale s (a power of
@@ -677,8 +693,9 @@ __device__ inline void step_node(const StepArgs& a, const Layout& lay, S* dst, c
 template <int D, typename S, int PM = 0, typename Out>
 __device__ inline void step_node(const StepArgs& a, int i, int j, int k, unsigned node,
                                  typename Store<S>::T (&f)[Lat<D>::Q], Out out,
-                                 const typename Store<S>::T* Fpre = nullptr) {
-  step_node<D, S, PM>(a, a.pc.lay, static_cast<S*>(a.dst), a.pc.rho_coll, a.rho_out, i, j, k, node, f, out, Fpre);
+                                 const typename Store<S>::T* Fpre = nullptr, int2 run = int2{0, 0}) {
+  step_node<D, S, PM>(a, a.pc.lay, static_cast<S*>(a.dst), a.pc.rho_coll, a.rho_out, i, j, k, node, f, out, Fpre,
+                      run);
 }
 
 template <int D, typename S, int PM = 0>
@@ -689,14 +706,19 @@ __global__ void __launch_bounds__(256) k_step(StepArgs a) {
   const Geo& g = a.pc.g;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y, k = blockIdx.z;
+  const unsigned row = (unsigned)j + (unsigned)g.ny * (unsigned)k;
+  // the node's pair run, computed by the whole wavefront before its lanes
+  // past the row's end leave
+  int2 run = int2{0, 0};
+  if (PM != 0 && a.has_ibm) run = pair_run(a, i, row, i < g.nx);
   if (i >= g.nx) return;
-  const unsigned node = (unsigned)i + (unsigned)g.nx * ((unsigned)j + (unsigned)g.ny * (unsigned)k);
+  const unsigned node = (unsigned)i + (unsigned)g.nx * row;
   T f[Q];
   pull_node<D, S, true>(a.pc, i, j, k, f);
   S* dst = static_cast<S*>(a.dst);
-  const size_t at0 = a.pc.lay.at(0, (unsigned)j + (unsigned)g.ny * (unsigned)k, (unsigned)i);
+  const size_t at0 = a.pc.lay.at(0, row, (unsigned)i);
   step_node<D, S, PM>(a, i, j, k, node, f,
-                        [&](int q, T v) { Store<S>::store(dst + q * a.pc.lay.pq + at0, v); });
+                        [&](int q, T v) { Store<S>::store(dst + q * a.pc.lay.pq + at0, v); }, nullptr, run);
 }
 
 // nsteps LB steps of a small grid in one launch of one workgroup, with the
@@ -845,11 +867,13 @@ __global__ void __launch_bounds__(lds_chunk<S>()) k_step_lds(StepArgs a) {
   const bool live = t < xc;
   const int i = x0 + t;
   const unsigned node = (unsigned)i + (unsigned)g.nx * ((unsigned)j + (unsigned)g.ny * (unsigned)k);
-#if COUPLB_LDS_FORCE_FIRST
   // the IBM force's dependent loads go first; the row staging follows
   T Fp[3] = {0, 0, 0};
-  if (PM != 0 && live) ibm_node_force<D, S, PM>(a, node, Fp[0], Fp[1], Fp[2]);
-#endif
+  if (PM != 0) {
+    int2 run = int2{0, 0};
+    if (a.has_ibm) run = pair_run(a, i, (unsigned)j + (unsigned)g.ny * (unsigned)k, live);
+    if (live) ibm_node_force<D, S, PM>(a, node, run, Fp[0], Fp[1], Fp[2]);
+  }
   stage_rows<D, S, LDS_XC>(g, lay, static_cast<const S*>(a.pc.src), lds, x0, xc, j, k);
   __syncthreads();
   T f[Q];
@@ -858,13 +882,9 @@ __global__ void __launch_bounds__(lds_chunk<S>()) k_step_lds(StepArgs a) {
       return Store<S>::decode(win[q][t + G - L::ex(q)]);
     });
   __syncthreads();
-#if COUPLB_LDS_FORCE_FIRST
   if (live)
     step_node<D, S, PM>(a, i, j, k, node, f, [&](int q, T v) { res[q][t] = Store<S>::encode(v); },
                         PM != 0 ? Fp : nullptr);
-#else
-  if (live) step_node<D, S, PM>(a, i, j, k, node, f, [&](int q, T v) { res[q][t] = Store<S>::encode(v); });
-#endif
   __syncthreads();
   S* dst = static_cast<S*>(a.dst);
   const unsigned row = (unsigned)j + (unsigned)g.ny * (unsigned)k;

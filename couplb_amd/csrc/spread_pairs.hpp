@@ -166,25 +166,41 @@ __global__ void __launch_bounds__(256) k_pair_heads(long ne, const unsigned* ske
   hflag[e] = (k != sentinel && (e == 0 || skey[e - 1] != k)) ? 1 : 0;
 }
 
-// node and length of each run (heads in increasing pair order)
-__global__ void __launch_bounds__(256) k_head_info(int nh, const int* heads, const unsigned* skey,
-                                                   long ne, int* hlen, unsigned* hnode) {
-  const int h = blockIdx.x * blockDim.x + threadIdx.x;
-  if (h >= nh) return;
+// node and length of each run (heads in increasing pair order), for the
+// *nh runs the selection counted (launched over a bound on their number);
+// the per-node run length plen[node] for the LB step (StepArgs::plen, which
+// the caller has cleared), and over[0] = 1 where a run is too long for it
+__global__ void __launch_bounds__(256) k_head_info(long bound, const int* nh, const int* heads,
+                                                   const unsigned* skey, long ne, int* hlen,
+                                                   unsigned* hnode, unsigned short* plen, int* over) {
+  const long h = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (h >= bound || h >= (long)*nh) return;
   const long s = heads[h];
   const unsigned k = skey[s];
   long f = s + 1;
   while (f < ne && skey[f] == k) f++;
   hlen[h] = (int)(f - s);
   hnode[h] = k;
+  if (f - s > 0xffff) over[0] = 1;
+  plen[k] = (unsigned short)(f - s > 0xffff ? 0xffff : f - s);
 }
 
-// run of every flagged node: prun[node] = (first pair, pair count), read by
-// the LB step (StepArgs::pair_mode)
-__global__ void k_run_index(int nh, const int* heads, const int* hlen, const unsigned* hnode, int2* prun) {
-  const int h = blockIdx.x * blockDim.x + threadIdx.x;
-  if (h >= nh) return;
-  prun[hnode[h]] = int2{heads[h], hlen[h]};
+// first pair of each 64-node segment of an x-row (segment sg = row * nsx +
+// x / 64, nsx = ceil(nx / 64)): the first sorted pair whose node is at or
+// past the segment's first node; with the run lengths plen it locates every
+// node's run (kernels.hpp pair_run)
+__global__ void __launch_bounds__(256) k_seg_start(long nseg, int nsx, int nx, const unsigned* skey, long ne,
+                                                   int* pseg) {
+  const long sg = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (sg >= nseg) return;
+  const unsigned n0 = (unsigned)(sg / nsx) * (unsigned)nx + 64u * (unsigned)(sg % nsx);
+  long lo = 0, hi = ne;  // lower bound of n0 in skey[0..ne)
+  while (lo < hi) {
+    const long m = (lo + hi) >> 1;
+    if (skey[m] < n0) lo = m + 1;
+    else hi = m;
+  }
+  pseg[sg] = (int)lo;
 }
 
 // p[0..n) = v, 16 bytes per thread (p 16-byte aligned, as device

@@ -48,7 +48,7 @@ class IbmArgs(C.Structure):
                 ("positions_unchanged", C.c_int), ("masses_unchanged", C.c_int), ("forces_zero", C.c_int)]
 
 
-MDF_RICHARDSON, MDF_CG = 0, 1
+MDF_RICHARDSON, MDF_CG, MDF_PCG = 0, 1, 2
 
 
 class MdfArgs(C.Structure):
@@ -286,7 +286,8 @@ class Device:
                    tol=1e-6, relax=1.0, max_iter=200, positions_unchanged=False, method="cg",
                    warm_start=None, features_ptr=None, targets_ptr=None):
         """Multi-direct-forcing solve for markers with prescribed velocity V
-        (device pointers; F0_ptr may be None); method "cg" or "richardson";
+        (device pointers; F0_ptr may be None); method "cg", "pcg" (block-Jacobi
+        preconditioned) or "richardson";
         warm_start an Mlp; features_ptr / targets_ptr optional float32 device
         outputs for training.  Returns the solver statistics."""
         a = MdfArgs()
@@ -296,7 +297,10 @@ class Device:
         a.ibm.xi_over_dtLB, a.ibm.inv_N = 0.0, 1.0
         a.ibm.positions_unchanged = int(positions_unchanged)
         a.tol, a.relax, a.max_iter = tol, relax, max_iter
-        a.method = MDF_CG if method == "cg" else MDF_RICHARDSON
+        methods = {"richardson": MDF_RICHARDSON, "cg": MDF_CG, "pcg": MDF_PCG}
+        if method not in methods:
+            raise ValueError(f"unknown MDF method {method!r}: richardson, cg or pcg")
+        a.method = methods[method]
         st = MdfStats()
         learn = None
         if warm_start is not None or features_ptr or targets_ptr:

@@ -176,6 +176,7 @@ struct couplb_ctx {
   double* ut = nullptr;            // operator node values at pair nodes, 3 per node
   double* mdf_v = nullptr;         // target velocity per sorted marker (3*cap)
   double *mdf_F = nullptr, *mdf_R = nullptr, *mdf_Ap = nullptr;  // force, residual, A p
+  double* mdf_Z = nullptr;         // preconditioned residual (COUPLB_MDF_PCG)
   double* mdf_part = nullptr;      // block partials
   int mdf_cap = 0;
   MdfScalars* mdf_S = nullptr;
@@ -195,6 +196,11 @@ struct couplb_ctx {
   int *mm_ent = nullptr, *mm_elen = nullptr, *mm_erow = nullptr, *mm_col = nullptr;
   int* mm_rowptr = nullptr;
   double* mm_aval = nullptr;  // entries for the current density
+  // block-Jacobi preconditioner of the explicit operator (mdf_matrix.hpp)
+  bool bj_ok = false;         // blocks built for the current matrix
+  int bj_npad = 0;            // markers rounded up to whole blocks
+  int *bj_perm = nullptr, *bj_pos = nullptr;  // block order <-> sorted marker
+  double* bj_inv = nullptr;   // block inverses, BJ x BJ each
   int ugather_cycle = -1; // state counter of the last velocity gather
   std::vector<SlotHost> slots;
   int nslots = 0;
@@ -847,6 +853,11 @@ void mm_free(couplb_ctx* c) {
   c->mm_ent = c->mm_elen = c->mm_erow = c->mm_col = c->mm_rowptr = nullptr;
   c->mm_aval = nullptr;
   c->mm_ok = false;
+  (void)hipFree(c->bj_perm); (void)hipFree(c->bj_pos); (void)hipFree(c->bj_inv);
+  c->bj_perm = c->bj_pos = nullptr;
+  c->bj_inv = nullptr;
+  c->bj_ok = false;
+  c->bj_npad = 0;
   c->mm_nnz = 0;
   c->mm_T = 0;
 }
@@ -955,12 +966,53 @@ int mm_build(couplb_ctx* c, int n) {
   return COUPLB_OK;
 }
 
+// Blocks of the preconditioner for the explicit operator of n sorted markers
+// with records rec: Morton order of their base cells, BJ markers a block.
+int bj_build(couplb_ctx* c, int n, const MarkerRec* rec) {
+  const int tb = 256;
+  const int npad = (n + BJ - 1) / BJ * BJ;
+  unsigned long long *key = nullptr, *key_s = nullptr;
+  int *val = nullptr, *val_s = nullptr;
+  void* tmp = nullptr;
+  auto release = [&]() {
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(key); (void)hipFree(key_s); (void)hipFree(val); (void)hipFree(val_s); (void)hipFree(tmp);
+  };
+  if (hipMalloc(&key, 8 * (size_t)n) != hipSuccess || hipMalloc(&key_s, 8 * (size_t)n) != hipSuccess ||
+      hipMalloc(&val, 4 * (size_t)n) != hipSuccess || hipMalloc(&val_s, 4 * (size_t)n) != hipSuccess) {
+    release();
+    return fail(c, COUPLB_ENOMEM, "out of device memory for the MDF preconditioner");
+  }
+  size_t bytes = 0;
+  hipError_t e = rocprim::radix_sort_pairs(nullptr, bytes, key, key_s, val, val_s, (size_t)n, 0u, 63u,
+                                           c->stream);
+  if (e == hipSuccess) e = hipMalloc(&tmp, bytes);
+  if (e == hipSuccess) e = hipMalloc(&c->bj_perm, 4 * (size_t)npad);
+  if (e == hipSuccess) e = hipMalloc(&c->bj_pos, 4 * (size_t)n);
+  if (e == hipSuccess) e = hipMalloc(&c->bj_inv, sizeof(double) * BJ * (size_t)npad);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_bj_keys, dim3((unsigned)((n + tb - 1) / tb)), dim3(tb), 0, c->stream, n, rec, key,
+                       val);
+    e = rocprim::radix_sort_pairs(tmp, bytes, key, key_s, val, val_s, (size_t)n, 0u, 63u, c->stream);
+  }
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_bj_perm, dim3((unsigned)((npad + tb - 1) / tb)), dim3(tb), 0, c->stream, n, npad,
+                       (const int*)val_s, c->bj_perm, c->bj_pos);
+    e = hipGetLastError();
+  }
+  release();
+  if (e != hipSuccess) return fail(c, COUPLB_EDEVICE, hipGetErrorString(e));
+  c->bj_npad = npad;
+  c->bj_ok = true;
+  return COUPLB_OK;
+}
+
 int mdf_impl(couplb_ctx* c, int n, const double* x, const double* V, const double* F0,
              double* f_body, const couplb_mdf_args* a, const couplb_mdf_learn* learn,
              couplb_mdf_stats* st) {
   if (n < 0 || !a || (n > 0 && (!x || !V || !f_body))) return fail(c, COUPLB_EINVAL, "bad MDF arguments");
   if (!(a->tol > 0) || a->max_iter < 1 || !(a->relax > 0) ||
-      (a->method != COUPLB_MDF_RICHARDSON && a->method != COUPLB_MDF_CG))
+      (a->method != COUPLB_MDF_RICHARDSON && a->method != COUPLB_MDF_CG && a->method != COUPLB_MDF_PCG))
     return fail(c, COUPLB_EINVAL, "MDF needs tol > 0, max_iter >= 1, relax > 0 and a known method");
   if (c->nslots != 0 || c->f4_partial)
     return fail(c, COUPLB_ESTATE, "MDF must be the only IBM call between two LB steps");
@@ -986,6 +1038,8 @@ int mdf_impl(couplb_ctx* c, int n, const double* x, const double* V, const doubl
   if (c->mdf_cap < n) {
     HIPCHK(c, hipStreamSynchronize(c->stream));
     (void)hipFree(c->mdf_F); (void)hipFree(c->mdf_R); (void)hipFree(c->mdf_Ap); (void)hipFree(c->mdf_part);
+    (void)hipFree(c->mdf_Z);
+    c->mdf_Z = nullptr;
     (void)hipFree(c->mm_gam); (void)hipFree(c->mm_B);
     c->mdf_F = c->mdf_R = c->mdf_Ap = c->mdf_part = c->mm_gam = c->mm_B = nullptr;
     c->mdf_cap = 0;
@@ -994,6 +1048,7 @@ int mdf_impl(couplb_ctx* c, int n, const double* x, const double* V, const doubl
     HIPCHK(c, hipMalloc(&c->mdf_F, sizeof(double) * 3 * cap));
     HIPCHK(c, hipMalloc(&c->mdf_R, sizeof(double) * 3 * cap));
     HIPCHK(c, hipMalloc(&c->mdf_Ap, sizeof(double) * 3 * cap));
+    HIPCHK(c, hipMalloc(&c->mdf_Z, sizeof(double) * 3 * cap));
     HIPCHK(c, hipMalloc(&c->mdf_part, sizeof(double) * 4 * nbmax));
     HIPCHK(c, hipMalloc(&c->mm_gam, sizeof(double) * cap));
     HIPCHK(c, hipMalloc(&c->mm_B, sizeof(double) * 3 * cap));
@@ -1037,18 +1092,27 @@ int mdf_impl(couplb_ctx* c, int n, const double* x, const double* V, const doubl
   const unsigned nbp = (unsigned)std::max(1, (c->pnh + 255) / 256);
   // The explicit operator (mdf_matrix.hpp) when the pair map is reused from
   // the previous solve (fixed markers): its build is a sort of all triples,
-  // paid back over the solves of later steps.  COUPLB_MDF_MATRIX=0 keeps the
-  // node path, =1 builds it at the first solve of a pair map.
+  // paid back over the solves of later steps; the preconditioned solve
+  // builds it at once (it saves more passes than the build costs).
+  // COUPLB_MDF_MATRIX=0 keeps the node path (and plain CG), =1 builds it at
+  // the first solve of a pair map.
   const char* mm_env = getenv("COUPLB_MDF_MATRIX");
   const int mm_mode = mm_env ? atoi(mm_env) : -1;
+  const bool pcg_req = a->method == COUPLB_MDF_PCG;
   bool use_mm = false;
-  if (mm_mode != 0 && c->pnh > 0 && (mm_mode == 1 || c->mm_seen == c->pair_epoch)) {
+  if (mm_mode != 0 && c->pnh > 0 && (mm_mode == 1 || pcg_req || c->mm_seen == c->pair_epoch)) {
     if (c->mm_epoch != c->pair_epoch || c->mm_n != n) {
       rc = mm_build(c, n);
       if (rc) return rc;
     }
     use_mm = c->mm_ok;
   }
+  // block-Jacobi preconditioning needs the explicit operator's entries
+  if (pcg_req && use_mm && !c->bj_ok) {
+    rc = bj_build(c, n, sl.rec);
+    if (rc) return rc;
+  }
+  const bool pcg = pcg_req && use_mm && c->bj_ok;
   c->mm_seen = c->pair_epoch;  // pairs rebuilt for one ordering are the same pairs
   const int nbs = (MM_LANES * n + MDF_THREADS - 1) / MDF_THREADS;
   const int nbr = use_mm ? nbs : nbi;  // partials of a residual / p.Ap pass
@@ -1087,6 +1151,11 @@ int mdf_impl(couplb_ctx* c, int n, const double* x, const double* V, const doubl
                        (const unsigned*)c->pkey_s, (const double*)c->u4, (const double*)c->mm_gam,
                        c->mm_aval);
     node_apply(1, nullptr, c->mm_B, Sd, part_a, part_b);
+    if (pcg)
+      hipLaunchKernelGGL(k_bj_invert, dim3((unsigned)(c->bj_npad / BJ)), dim3(64), 0, c->stream,
+                         (const int*)c->bj_perm, (const int*)c->bj_pos, (const int*)c->mm_rowptr,
+                         (const int*)c->mm_col, (const double*)c->mm_aval, (const double*)c->mm_gam,
+                         c->bj_inv);
     HIPCHK(c, hipGetLastError());
   }
   auto scalar = [&](int nblk, int first, int respect, MdfScalars* mirror) {
@@ -1164,19 +1233,34 @@ int mdf_impl(couplb_ctx* c, int n, const double* x, const double* V, const doubl
   }
   int method = a->method;
   while (c->mdf_host->done == 0) {
-    if (method == COUPLB_MDF_CG) {
-      hipLaunchKernelGGL(k_cg_restart, dim3((unsigned)((3 * n + tb - 1) / tb)), dim3(tb), 0, c->stream, n,
-                         (const double*)c->mdf_R, c->fsrc);
+    if (method != COUPLB_MDF_RICHARDSON) {
+      if (pcg) {  // z = M^-1 r, p = z, rz = r.z
+        hipLaunchKernelGGL((k_pcg_step<false>), dim3(nb), dim3(MDF_THREADS), 0, c->stream, c->bj_npad,
+                           (const int*)c->bj_perm, (const double*)c->bj_inv, c->fsrc, (const double*)c->mdf_Ap,
+                           c->mdf_F, c->mdf_R, c->mdf_Z, (const double*)part_x, 0, part_a, part_b, Sd);
+        hipLaunchKernelGGL(k_pcg_rz, dim3(1), dim3(MDF_THREADS), 0, c->stream, (const double*)part_a, nb, Sd);
+      } else {
+        hipLaunchKernelGGL(k_cg_restart, dim3((unsigned)((3 * n + tb - 1) / tb)), dim3(tb), 0, c->stream, n,
+                           (const double*)c->mdf_R, c->fsrc);
+      }
       // pass k reads the scalars Sd[k % 2] and writes Sd[(k + 1) % 2]
       rc = run_chunks([&](MdfScalars* mirror) {
         for (int k = 0; k < CHUNK; k++) {
           MdfScalars* Sk = Sd + (k & 1);
           MdfScalars* Sn = Sd + ((k + 1) & 1);
           apply(0, Sk, part_x, part_y);
-          hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(MDF_THREADS), 0, c->stream, n,
-                             (const double*)c->fsrc, (const double*)c->mdf_Ap, c->mdf_F, c->mdf_R,
-                             (const double*)part_x, nbr, part_a, part_b, Sk);
-          hipLaunchKernelGGL(k_cg_next, dim3(nb), dim3(MDF_THREADS), 0, c->stream, n, (const double*)c->mdf_R,
+          if (pcg)
+            hipLaunchKernelGGL((k_pcg_step<true>), dim3(nb), dim3(MDF_THREADS), 0, c->stream, c->bj_npad,
+                               (const int*)c->bj_perm, (const double*)c->bj_inv, c->fsrc,
+                               (const double*)c->mdf_Ap, c->mdf_F, c->mdf_R, c->mdf_Z, (const double*)part_x,
+                               nbr, part_a, part_b, Sk);
+          else
+            hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(MDF_THREADS), 0, c->stream, n,
+                               (const double*)c->fsrc, (const double*)c->mdf_Ap, c->mdf_F, c->mdf_R,
+                               (const double*)part_x, nbr, part_a, part_b, Sk);
+          // p = r + beta p (CG) or z + beta p (preconditioned)
+          hipLaunchKernelGGL(k_cg_next, dim3(nb), dim3(MDF_THREADS), 0, c->stream, n,
+                             (const double*)(pcg ? c->mdf_Z : c->mdf_R),
                              c->fsrc, (const double*)part_a, (const double*)part_b, nb, a->tol, a->max_iter,
                              (const MdfScalars*)Sk, Sn, k == CHUNK - 1 ? mirror : nullptr);
         }
@@ -1733,6 +1817,7 @@ int couplb_destroy(couplb_ctx* c) {
   (void)hipFree(c->pseg);
   (void)hipFree(c->ut); (void)hipFree(c->mdf_v); (void)hipFree(c->mdf_S);
   (void)hipFree(c->mdf_F); (void)hipFree(c->mdf_R); (void)hipFree(c->mdf_Ap); (void)hipFree(c->mdf_part);
+  (void)hipFree(c->mdf_Z);
   (void)hipFree(c->mlp_x); (void)hipFree(c->mlp_y); (void)hipFree(c->mlp_sig);
   (void)hipFree(c->mm_gam); (void)hipFree(c->mm_B);
   mm_free(c);

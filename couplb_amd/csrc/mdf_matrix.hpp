@@ -14,6 +14,17 @@
 // values depend on the density and are summed per solve from those lists
 // in their fixed order.  A CG pass is then one sparse matrix-vector product
 // instead of a spreading and an interpolation over the grid.
+//
+// Block-Jacobi preconditioner (COUPLB_MDF_PCG): the markers are grouped in
+// blocks of BJ along a Morton curve of their base cells, so that markers
+// whose stencils overlap mostly share a block.  With about one marker per
+// dx and a 4-dx stencil the directions plain CG resolves slowest are forces
+// alternating between neighbouring markers; they live inside the blocks.
+// Per solve each block's diagonal part of the symmetric factor
+// Ah = G^-1 A (Ah_kj = sum_x pw_kx pw_jx / (2 rho_x), G = diag(g_k)) is
+// inverted in LDS, and preconditioned CG applies z = blockdiag(Ah_bb)^-1 r
+// in the pass that updates r.  The preconditioner changes the path of the
+// iteration, not the system it solves.
 #pragma once
 #include "mdf.hpp"
 
@@ -160,6 +171,181 @@ __global__ void __launch_bounds__(MDF_THREADS) k_mm_spmv(int n, const int* rowpt
     part_a[blockIdx.x] = a;
     part_b[blockIdx.x] = b;
   }
+}
+
+// ---- block-Jacobi preconditioner ----
+constexpr int BJ = 32;  // markers per block
+
+// 21 bits of v spread to every third bit
+__device__ inline unsigned long long bj_spread(unsigned v) {
+  unsigned long long m = 0;
+  for (int b = 0; b < 21; b++) m |= (unsigned long long)((v >> b) & 1u) << (3 * b);
+  return m;
+}
+
+// Morton key of each sorted marker's base cell (offset by 2: a base cell
+// can sit on the ghost layer); markers without a stencil go last
+__global__ void __launch_bounds__(256) k_bj_keys(int n, const MarkerRec* rec, unsigned long long* key,
+                                                 int* val) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const MarkerRec& r = rec[k];
+  key[k] = r.valid ? bj_spread((unsigned)(r.i0 + 2)) | (bj_spread((unsigned)(r.j0 + 2)) << 1) |
+                         (bj_spread((unsigned)(r.k0 + 2)) << 2)
+                   : (~0ull >> 1);
+  val[k] = k;
+}
+
+// block order: bperm[t] = marker at position t (-1 past n), bpos = inverse
+__global__ void __launch_bounds__(256) k_bj_perm(int n, int npad, const int* sval, int* bperm, int* bpos) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= npad) return;
+  const int k = t < n ? sval[t] : -1;
+  bperm[t] = k;
+  if (k >= 0) bpos[k] = t;
+}
+
+// One 64-lane block per marker block: gather the block's entries of Ah from
+// the explicit operator (rows with no fluid weight decoupled: identity),
+// invert by Gauss-Jordan elimination in LDS (Ah_bb is symmetric positive
+// definite: no pivoting), write the BJ x BJ inverse row-major.  A pivot
+// that is not positive (a singular block) leaves the block's diagonal
+// inverse instead.
+__global__ void __launch_bounds__(64) k_bj_invert(const int* bperm, const int* bpos, const int* rowptr,
+                                                  const int* col, const double* aval, const double* gam,
+                                                  double* binv) {
+  __shared__ double M[BJ][BJ + 1];
+  __shared__ double dg[BJ];
+  __shared__ int dz[BJ];
+  const int b = blockIdx.x, l = threadIdx.x;
+  for (int e = l; e < BJ * BJ; e += 64) M[e / BJ][e % BJ] = 0.0;
+  __syncthreads();
+  if (l < BJ) {
+    const int k = bperm[b * BJ + l];
+    double d = 0.0;
+    if (k >= 0) {
+      const double gk = gam[k];
+      if (gk > 0.0) {
+        for (int e = rowptr[k]; e < rowptr[k + 1]; e++) {
+          const int pj = bpos[col[e]] - b * BJ;
+          if (pj >= 0 && pj < BJ) M[l][pj] = aval[e] / gk;
+        }
+        d = M[l][l];
+      }
+    }
+    dz[l] = !(d > 0.0);
+  }
+  __syncthreads();
+  for (int e = l; e < BJ * BJ; e += 64) {
+    const int r = e / BJ, q = e % BJ;
+    if (dz[r] || dz[q]) M[r][q] = (r == q) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  if (l < BJ) dg[l] = M[l][l];
+  __syncthreads();
+  const int row = l >> 1, c0 = (l & 1) * (BJ / 2);
+  bool ok = true;
+  for (int p = 0; p < BJ; p++) {
+    const double piv = M[p][p];
+    const double f = M[row][p];
+    __syncthreads();
+    if (!(piv > 1e-12 * dg[p]) || !(piv < 1e300)) {  // the same value in every lane
+      ok = false;
+      break;
+    }
+    if (row == p)
+      for (int j = c0; j < c0 + BJ / 2; j++) M[p][j] = (j == p ? 1.0 : M[p][j]) / piv;
+    __syncthreads();
+    if (row != p)
+      for (int j = c0; j < c0 + BJ / 2; j++) M[row][j] = (j == p ? 0.0 : M[row][j]) - f * M[p][j];
+    __syncthreads();
+  }
+  double* o = binv + (size_t)b * BJ * BJ;
+  for (int e = l; e < BJ * BJ; e += 64) {
+    const int r = e / BJ, q = e % BJ;
+    o[e] = ok ? M[r][q] : (r == q ? 1.0 / dg[r] : 0.0);
+  }
+}
+
+// Preconditioned CG, markers in block order (MDF_THREADS / BJ blocks per
+// workgroup), z = M^-1 r with the block's r staged in LDS:
+//   UPDATE  alpha = rz / p.Ap; F += alpha p, r -= alpha Ap; then z (as
+//           k_cg_update; p.Ap <= 0 leaves everything for k_cg_next)
+//   restart z of the true residual r and p = z
+// Block partials: sum r.z into part_a, max |r| into part_b.
+template <bool UPDATE>
+__global__ void __launch_bounds__(MDF_THREADS) k_pcg_step(int npad, const int* bperm, const double* binv,
+                                                          double* P, const double* Ap, double* Fm, double* R,
+                                                          double* Z, const double* px, int npx,
+                                                          double* part_a, double* part_b, MdfScalars* Sk) {
+  __shared__ double sh[MDF_THREADS / 64];
+  __shared__ double rl[MDF_THREADS][3];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = t < npad ? bperm[t] : -1;
+  double r[3] = {0.0, 0.0, 0.0};
+  if constexpr (UPDATE) {
+    if (Sk->done) return;
+    const double rz = Sk->rr;
+    const double pAp = fold_sum(px, npx, sh);
+    const bool ok = pAp > 0.0;
+    const double al = ok ? rz / pAp : 0.0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      Sk->pAp = pAp;  // fields no block of this kernel reads
+      Sk->alpha = al;
+    }
+    if (!ok) return;
+    if (k >= 0) {
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        Fm[3 * (size_t)k + c] += al * P[3 * (size_t)k + c];
+        r[c] = R[3 * (size_t)k + c] - al * Ap[3 * (size_t)k + c];
+        R[3 * (size_t)k + c] = r[c];
+      }
+    }
+  } else {
+    if (k >= 0)
+#pragma unroll
+      for (int c = 0; c < 3; c++) r[c] = R[3 * (size_t)k + c];
+  }
+#pragma unroll
+  for (int c = 0; c < 3; c++) rl[threadIdx.x][c] = r[c];
+  __syncthreads();
+  double a = 0.0, b = 0.0;
+  if (k >= 0) {
+    const int s0 = threadIdx.x & ~(BJ - 1);  // first lane of the block
+    const double2* row = reinterpret_cast<const double2*>(binv + (size_t)t * BJ);
+    double z0 = 0.0, z1 = 0.0, z2 = 0.0;
+#pragma unroll 4
+    for (int u = 0; u < BJ / 2; u++) {
+      const double2 w = row[u];
+      z0 += w.x * rl[s0 + 2 * u][0] + w.y * rl[s0 + 2 * u + 1][0];
+      z1 += w.x * rl[s0 + 2 * u][1] + w.y * rl[s0 + 2 * u + 1][1];
+      z2 += w.x * rl[s0 + 2 * u][2] + w.y * rl[s0 + 2 * u + 1][2];
+    }
+    Z[3 * (size_t)k + 0] = z0;
+    Z[3 * (size_t)k + 1] = z1;
+    Z[3 * (size_t)k + 2] = z2;
+    if constexpr (!UPDATE) {
+      P[3 * (size_t)k + 0] = z0;
+      P[3 * (size_t)k + 1] = z1;
+      P[3 * (size_t)k + 2] = z2;
+    }
+    a = r[0] * z0 + r[1] * z1 + r[2] * z2;
+    b = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+  }
+  a = block_reduce(a, 0, sh);
+  b = block_reduce(b, 1, sh);
+  if (threadIdx.x == 0) {
+    part_a[blockIdx.x] = a;
+    part_b[blockIdx.x] = b;
+  }
+}
+
+// one block: r.z of the restart into the scalars
+__global__ void __launch_bounds__(MDF_THREADS) k_pcg_rz(const double* part_a, int nblk, MdfScalars* S) {
+  __shared__ double sh[MDF_THREADS / 64];
+  const double rz = fold_sum(part_a, nblk, sh);
+  if (threadIdx.x == 0) S->rr = rz;
 }
 
 }  // namespace couplb

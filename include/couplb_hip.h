@@ -179,9 +179,9 @@ int couplb_ibm_substep_device(couplb_ctx* ctx, int n, const double* x, const dou
  *   slip(F) = V/vel_scale - I(u* + S(F)/(2 rho)) = 0   to max|slip| < tol,
  * with u* = j/rho of the current state, I = IBM::interpolate and
  * S = IBM::spread (couplb_ibm.h:57-133), by Richardson steps
- * F += 2*relax*slip (classic multi-direct forcing) or by conjugate gradients
- * (restarted from the true slip; falls back to Richardson if it breaks
- * down).  The converged F is spread into the
+ * F += 2*relax*slip (classic multi-direct forcing) or by conjugate gradients,
+ * plain or block-Jacobi preconditioned (restarted from the true slip; falls
+ * back to Richardson if it breaks down).  The converged F is spread into the
  * next LB step.  It must be the only IBM call between two LB steps.
  * x, V (physical units), F0 (initial F in lattice units, caller order; NULL
  * starts from zero) and f_body are device arrays [n][3]; f_body receives the
@@ -190,12 +190,15 @@ int couplb_ibm_substep_device(couplb_ctx* ctx, int n, const double* x, const dou
  * iterations.  stats->iterations counts interpolation passes. */
 #define COUPLB_MDF_RICHARDSON 0  /* F += 2*relax*slip, the classic iteration */
 #define COUPLB_MDF_CG 1          /* conjugate gradients on the same fixed point */
+#define COUPLB_MDF_PCG 2         /* conjugate gradients preconditioned by the inverses
+                                    of 32-marker blocks of neighbouring markers (on the
+                                    explicit operator; plain CG where it is not built) */
 typedef struct couplb_mdf_args {
   couplb_ibm_args ibm;  /* lo, inv_dx, vel_scale, force_scale, positions_unchanged */
   double tol;           /* slip tolerance, lattice velocity units */
   double relax;         /* omega of the Richardson iteration */
   int max_iter;         /* operator applications (interpolation passes) */
-  int method;           /* COUPLB_MDF_RICHARDSON or COUPLB_MDF_CG */
+  int method;           /* COUPLB_MDF_RICHARDSON, COUPLB_MDF_CG or COUPLB_MDF_PCG */
 } couplb_mdf_args;
 typedef struct couplb_mdf_stats {
   int iterations;       /* interpolation passes, including the one at F = 0

@@ -148,6 +148,40 @@ def test_mdf_cg_converges_like_oracle(kernel, operator, monkeypatch):
     assert rel_l2(F_dev, F_ref) < 1e-6
 
 
+@pytest.mark.parametrize("kernel", ["roma", "peskin4"])
+def test_mdf_pcg_reaches_the_oracle_solution_in_fewer_passes(kernel, monkeypatch):
+    # block-Jacobi preconditioned CG solves the same system: the oracle's CG
+    # solution to the tolerance, in fewer operator applications; on the node
+    # operator (COUPLB_MDF_MATRIX=0) it is plain CG
+    import couplb_amd
+    kern = couplb_amd.ROMA if kernel == "roma" else couplb_amd.PESKIN4
+    x, V = markers()
+    n = len(x)
+    F_ref, it, s0, s1 = None, None, None, None
+    passes = {}
+    for mode, env in (("pcg", "-1"), ("pcg_node", "0"), ("cg", "1")):
+        monkeypatch.setenv("COUPLB_MDF_MATRIX", env)
+        d = developed_device(kern)
+        if F_ref is None:
+            F_ref, it, s0, s1 = oracle.mdf_solve(_oracle_grid(d), x, V, (0, 0, 0), 1.0, kern, 1e-10,
+                                                 max_iter=400, method="cg")
+        dx, dV, fb = DeviceArray(x), DeviceArray(V), DeviceArray(np.zeros((n, 3)))
+        print("case MDF", mode, kernel, flush=True)
+        st = d.mdf_device(n, dx.addr, dV.addr, None, fb.addr, (0, 0, 0), 1.0, 1.0, 1.0,
+                          tol=1e-10, max_iter=400, method="cg" if mode == "cg" else "pcg")
+        F_dev = -fb.download()
+        print("  ", st, "oracle CG passes", it, "rel F", rel_l2(F_dev, F_ref))
+        assert st["converged"] and st["slip_last"] < 1e-10
+        assert st["slip_first"] == pytest.approx(s0, rel=1e-12)
+        # slip < 1e-10 fixes the forces along the directions A barely sees
+        # only loosely, and the preconditioned iteration takes another path
+        # there: a few 1e-6 (a different operator would differ by O(1))
+        assert rel_l2(F_dev, F_ref) < (5e-6 if mode == "pcg" else 1e-6)
+        passes[mode] = st["iterations"]
+    assert abs(passes["pcg_node"] - it) <= max(3, it // 5)
+    assert passes["pcg"] <= 0.8 * passes["cg"]
+
+
 def test_mdf_one_iteration_equals_penalty_step():
     import couplb_amd
     x, V = markers()

@@ -83,6 +83,9 @@ struct SlotHost {
 
 }  // namespace
 
+// chunks of a host-array sub-step that only recomputes forces
+constexpr int HOST_CHUNKS_MAX = 8;
+
 struct couplb_ctx {
   couplb_params p{};
   int D = 3, Q = 19;
@@ -105,6 +108,10 @@ struct couplb_ctx {
   unsigned timer_mask = ~0u;  // timed region kinds (bit per kind)
   hipStream_t cstream = nullptr;  // host-array uploads of the IBM sub-steps
   hipEvent_t up_done = nullptr;   // end of the latest upload on cstream
+  hipStream_t dstream = nullptr;  // host-array downloads of chunked sub-steps
+  hipEvent_t ev_up[HOST_CHUNKS_MAX] = {}, ev_cmp[HOST_CHUNKS_MAX] = {};  // per chunk
+  int* iperm = nullptr;           // sorted position of each atom
+  int iperm_epoch = -1;           // ordering iperm belongs to
   int hx_n = -1;                  // markers whose host x, m dx, dm hold (-1: none)
   int host_cycle = -1;            // state of the latest host sub-step
   size_t fused_lds = 0;    // its LDS bytes; 0 when steps do not fuse
@@ -483,7 +490,9 @@ int ensure_ibm(couplb_ctx* c, int n) {
   const int cap = std::max(n, 2 * c->cap);
   (void)hipFree(c->keys); (void)hipFree(c->skeys); (void)hipFree(c->vals); (void)hipFree(c->perm); (void)hipFree(c->bases);
   (void)hipFree(c->sort_tmp); (void)hipFree(c->dx); (void)hipFree(c->dv); (void)hipFree(c->dm); (void)hipFree(c->df);
-  (void)hipFree(c->ufc);
+  (void)hipFree(c->ufc); (void)hipFree(c->iperm);
+  c->iperm = nullptr;
+  c->iperm_epoch = -1;
   HIPCHK(c, hipMalloc(&c->keys, sizeof(unsigned) * cap));
   HIPCHK(c, hipMalloc(&c->skeys, sizeof(unsigned) * cap));
   HIPCHK(c, hipMalloc(&c->vals, sizeof(int) * cap));
@@ -494,6 +503,7 @@ int ensure_ibm(couplb_ctx* c, int n) {
   HIPCHK(c, hipMalloc(&c->dm, sizeof(double) * cap));
   HIPCHK(c, hipMalloc(&c->df, sizeof(double) * 3 * cap));
   HIPCHK(c, hipMalloc(&c->ufc, sizeof(double) * 3 * cap));
+  HIPCHK(c, hipMalloc(&c->iperm, sizeof(int) * cap));
   (void)hipFree(c->fsrc);
   c->fsrc = nullptr;
   HIPCHK(c, hipMalloc(&c->fsrc, sizeof(double) * 3 * cap));
@@ -603,6 +613,17 @@ int prepare_markers(couplb_ctx* c, int n, const double* x, const IbmParams& ip,
 
 int fold_slots(couplb_ctx* c);
 
+// Same positions and ordering, and the same LB state as the previous
+// sub-step of this interval: u_f is known and only the force is new
+// (k_marker_force); prepare_markers then has nothing to do.
+bool force_only(const couplb_ctx* c, int n, int positions_unchanged, const IbmParams& ip) {
+  const double key[5] = {ip.lo[0], ip.lo[1], ip.lo[2], ip.inv_dx, (double)c->p.kernel};
+  return positions_unchanged && c->nslots > 0 && c->order_n == n &&
+         std::memcmp(key, c->order_key, sizeof(key)) == 0 &&
+         c->slots[c->nslots - 1].epoch == c->pos_epoch && c->slots[c->nslots - 1].n == n &&
+         c->uf_cycle == c->cycle && c->uf_epoch == c->pos_epoch;
+}
+
 int ibm_impl(couplb_ctx* c, int n, const double* x, const double* v, const double* m,
              double* f_dev, int fmode, const couplb_ibm_args* a) {
   if (n < 0 || !a) return fail(c, COUPLB_EINVAL, "bad IBM arguments");
@@ -622,8 +643,13 @@ int ibm_impl(couplb_ctx* c, int n, const double* x, const double* v, const doubl
   // built its moving-marker map (markers that leave their base cell are
   // spread apart until the next ordering)
   const bool later = c->nslots > 0 || c->f4_partial;
-  const bool keep = later || (c->mpair_epoch == c->pos_epoch && c->order_age < c->reorder_every);
-  rc = prepare_markers(c, n, x, ip, a->positions_unchanged, keep, &reorder);
+  const bool young = c->order_age < c->reorder_every;
+  const bool keep = later || (c->mpair_epoch == c->pos_epoch && young);
+  // a kept ordering that markers moved on is renewed at the start of an
+  // interval once it is old, also when the positions no longer change, so
+  // that markers which stopped get the fixed-marker path back
+  const int unchanged = a->positions_unchanged && !(c->kept_move && !later && !young);
+  rc = prepare_markers(c, n, x, ip, unchanged, keep, &reorder);
   if (rc) return rc;
   // Same positions and same LB state as the previous sub-step of this
   // interval: u_f is known and only the force is new (k_marker_force).
@@ -1669,6 +1695,78 @@ int step_impl(couplb_ctx* c, const double g[3], int flags, bool timed) {
   return COUPLB_OK;
 }
 
+// A host-array sub-step that only recomputes forces (force_only): v (and
+// m, f where needed) go up, k_marker_force runs over the atoms in caller
+// order, and the forces come back.  By default all of it is on the context
+// stream: a kernel waiting on another stream's event has started up to
+// 0.4 ms after the event here (measured), more than the copies take.
+// COUPLB_HOST_CHUNKS = K > 1 instead splits the atoms into K chunks with
+// uploads on the copy stream and downloads on the download stream, each
+// chunk's kernel waiting for its upload (overlap of the copies, at the
+// price of those waits and of 35-40 us of host calls per chunk).
+int host_force_chunks(couplb_ctx* c, int n, const double* v, const double* m, double* f,
+                      const couplb_ibm_args* a, const IbmParams& ip) {
+  static const int env_chunks = [] {
+    const char* e = getenv("COUPLB_HOST_CHUNKS");
+    return e ? atoi(e) : 1;
+  }();
+  const int K = n < 8192 ? 1 : std::max(1, std::min(env_chunks, HOST_CHUNKS_MAX));
+  if (K > 1 && !c->dstream) {
+    HIPCHK(c, hipStreamCreateWithFlags(&c->dstream, hipStreamNonBlocking));
+    for (int k = 0; k < HOST_CHUNKS_MAX; k++) {
+      HIPCHK(c, hipEventCreateWithFlags(&c->ev_up[k], hipEventDisableTiming));
+      HIPCHK(c, hipEventCreateWithFlags(&c->ev_cmp[k], hipEventDisableTiming));
+    }
+  }
+  hipStream_t up = K > 1 ? c->cstream : c->stream, down = K > 1 ? c->dstream : c->stream;
+  const int tb = 256;
+  if (c->iperm_epoch != c->pos_epoch) {
+    hipLaunchKernelGGL(k_inv_perm, dim3((unsigned)((n + tb - 1) / tb)), dim3(tb), 0, c->stream, n,
+                       (const int*)c->perm, c->iperm);
+    HIPCHK(c, hipGetLastError());
+    c->iperm_epoch = c->pos_epoch;
+  }
+  const bool send_m = !a->masses_unchanged;  // dm holds the previous sub-step's masses
+  for (int k = 0; k < K; k++) {
+    const size_t lo = (size_t)n * k / K, cnt = (size_t)n * (k + 1) / K - lo;
+    HIPCHK(c, hipMemcpyAsync(c->dv + 3 * lo, v + 3 * lo, sizeof(double) * 3 * cnt, hipMemcpyHostToDevice, up));
+    if (send_m)
+      HIPCHK(c, hipMemcpyAsync(c->dm + lo, m + lo, sizeof(double) * cnt, hipMemcpyHostToDevice, up));
+    if (!a->forces_zero)
+      HIPCHK(c, hipMemcpyAsync(c->df + 3 * lo, f + 3 * lo, sizeof(double) * 3 * cnt, hipMemcpyHostToDevice,
+                               up));
+    if (K > 1) HIPCHK(c, hipEventRecord(c->ev_up[k], up));
+  }
+  c->hx_n = -1;
+  c->host_cycle = c->cycle;
+  int tslot = 0;
+  int rc = timer_begin(c, 1, &tslot);
+  if (rc) return rc;
+  SlotHost& sl = c->slots[c->nslots - 1];
+  double* fsrc = c->nslots == 1 ? c->fsrc : nullptr;
+  for (int k = 0; k < K; k++) {
+    const size_t lo = (size_t)n * k / K, cnt = (size_t)n * (k + 1) / K - lo;
+    if (K > 1) HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_up[k], 0));
+    hipLaunchKernelGGL(k_marker_force, dim3((unsigned)((cnt + tb - 1) / tb)), dim3(tb), 0, c->stream, (int)cnt,
+                       (const int*)c->perm, (const double*)c->dv, (const double*)c->dm, c->df,
+                       a->forces_zero ? 1 : 0, ip, (const double*)c->ufc, sl.rec, fsrc, (int)lo,
+                       (const int*)c->iperm);
+    HIPCHK(c, hipGetLastError());
+    if (K > 1) {
+      HIPCHK(c, hipEventRecord(c->ev_cmp[k], c->stream));
+      HIPCHK(c, hipStreamWaitEvent(down, c->ev_cmp[k], 0));
+    }
+    HIPCHK(c, hipMemcpyAsync(f + 3 * lo, c->df + 3 * lo, sizeof(double) * 3 * cnt, hipMemcpyDeviceToHost,
+                             down));
+  }
+  rc = timer_end(c, tslot);
+  if (rc) return rc;
+  c->rho_fresh = 1;  // compute_macroscopic(false) at fix_couplb.cpp:758
+  HIPCHK(c, hipStreamSynchronize(down));
+  c->hx_n = n;
+  return COUPLB_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1810,6 +1908,12 @@ int couplb_destroy(couplb_ctx* c) {
   (void)hipFree(c->staging);
   if (c->up_done) (void)hipEventDestroy(c->up_done);
   if (c->cstream) (void)hipStreamDestroy(c->cstream);
+  for (int k = 0; k < HOST_CHUNKS_MAX; k++) {
+    if (c->ev_up[k]) (void)hipEventDestroy(c->ev_up[k]);
+    if (c->ev_cmp[k]) (void)hipEventDestroy(c->ev_cmp[k]);
+  }
+  if (c->dstream) (void)hipStreamDestroy(c->dstream);
+  (void)hipFree(c->iperm);
   (void)hipFree(c->pkey); (void)hipFree(c->pkey_s); (void)hipFree(c->pval); (void)hipFree(c->pval_s);
   (void)hipFree(c->pp); (void)hipFree(c->pw); (void)hipFree(c->ptmp);
   (void)hipFree(c->pheads); (void)hipFree(c->phlen); (void)hipFree(c->phnode); (void)hipFree(c->phflag);
@@ -1979,11 +2083,14 @@ int couplb_ibm_substep(couplb_ctx* c, int n, const double* x, const double* v, c
     HIPCHK(c, hipStreamCreateWithFlags(&c->cstream, hipStreamNonBlocking));
     HIPCHK(c, hipEventCreateWithFlags(&c->up_done, hipEventDisableTiming));
   }
+  const bool same = c->hx_n == n;  // dx, dm hold the previous host sub-step's x, m
+  const IbmParams ip = make_ibm_params(c, a);
+  if (same && n < (1 << REC_SLOT_SHIFT) && force_only(c, n, a->positions_unchanged, ip))
+    return host_force_chunks(c, n, v, m, f, a, ip);
   // after an LB step the arrays go up on the copy stream, so that they
   // travel while its kernel runs (otherwise on the context stream, with no
   // cross-stream wait); positions and masses the caller declares unchanged
   // are already on the device from the previous host sub-step
-  const bool same = c->hx_n == n;  // dx, dm hold the previous host sub-step's x, m
   const bool overlap = c->cycle != c->host_cycle;
   hipStream_t up = overlap ? c->cstream : c->stream;
   if (!(a->positions_unchanged && same))

@@ -491,14 +491,21 @@ __global__ void __launch_bounds__(256) k_marker(int n, const int* perm, const do
 // reaction is added into the first sub-step's record; spreading the summed
 // reaction once equals spreading each sub-step's reaction (couplb_ibm.h:
 // 125-130 is linear in the marker force) up to summation order.
+//
+// Sorted order (iperm null): thread p is sorted marker p, atom perm[p].
+// Caller order (iperm: sorted position of each atom): thread t < n is atom
+// lo + t, so that a range of atoms can run as soon as its arrays are on the
+// device (the host-array sub-steps send them in chunks).
 __global__ void __launch_bounds__(256) k_marker_force(int n, const int* perm, const double* v,
                                                       const double* m, double* f_atom, int fmode,
                                                       IbmParams ip, const double* ufc,
-                                                      MarkerRec* rec, double* fsrc) {
+                                                      MarkerRec* rec, double* fsrc,
+                                                      int lo = 0, const int* iperm = nullptr) {
 #pragma clang fp contract(off)
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  const int idx = perm[p];
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int idx = iperm ? lo + t : perm[t];
+  const int p = iperm ? iperm[idx] : t;
   const double uf0 = ufc[3 * (size_t)p + 0], uf1 = ufc[3 * (size_t)p + 1], uf2 = ufc[3 * (size_t)p + 2];
   const double* vp = v + 3 * (size_t)idx;
   const double mm = m[idx];
@@ -517,6 +524,12 @@ __global__ void __launch_bounds__(256) k_marker_force(int n, const int* perm, co
     fsrc[3 * (size_t)p + 1] = r.fy;
     fsrc[3 * (size_t)p + 2] = r.fz;
   }
+}
+
+// iperm[perm[p]] = p
+__global__ void __launch_bounds__(256) k_inv_perm(int n, const int* perm, int* iperm) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) iperm[perm[p]] = p;
 }
 
 }  // namespace couplb

@@ -262,14 +262,16 @@ def test_more_sub_steps_than_record_slots_match_oracle(md):
     assert err <= TOL and ferr <= 1e-11
 
 
-@pytest.mark.parametrize("reorder_every", ["8", "3", "0"])
-def test_moving_markers_across_cells_match_oracle(reorder_every, monkeypatch):
+@pytest.mark.parametrize("reorder_every,stop", [("8", None), ("3", None), ("0", None), ("8", 45)])
+def test_moving_markers_across_cells_match_oracle(reorder_every, stop, monkeypatch):
     """Markers that cross cells and periodic faces at every few MD steps over
     40 LB intervals (md_per_lb 3): the device keeps a marker ordering for
     COUPLB_REORDER_EVERY intervals, sums the forces of the markers on their
     ordered base cells through the node-sorted map of that ordering, and
     spreads the records of markers that left their base cell tile by tile;
-    the result must be the reference's (fix_couplb.cpp:723-798) to rounding."""
+    the result must be the reference's (fix_couplb.cpp:723-798) to rounding.
+    stop: the markers stand still from that MD step on (the kept ordering is
+    renewed once it is old and the fixed-marker path takes over)."""
     monkeypatch.setenv("COUPLB_REORDER_EVERY", reorder_every)
     rng = np.random.default_rng(31)
     n = 80
@@ -281,6 +283,8 @@ def test_moving_markers_across_cells_match_oracle(reorder_every, monkeypatch):
     def atoms(step):
         a = copy_atoms(base)
         x = x0.copy()
+        if stop is not None:
+            step = min(step, stop)
         x[:, 0] = np.mod(x0[:, 0] + vx * step, nx)
         x[:, 1] = x0[:, 1] + 1.5 * np.sin(0.07 * step + np.arange(n))
         x[:, 2] = np.mod(x0[:, 2] + 0.6 * step * np.cos(np.arange(n)), nz)
@@ -289,7 +293,7 @@ def test_moving_markers_across_cells_match_oracle(reorder_every, monkeypatch):
 
     cmd = (f"cross all couplb {nx} {ny} {nz} 0.1 1.0 md_per_lb 3 kernel peskin4 wall_y 1 2 "
            "wall_vel 0.01 0.0 0.0 gravity 1e-5 0.0 0.0")
-    print("case moving markers across cells, COUPLB_REORDER_EVERY", reorder_every, flush=True)
+    print("case moving markers across cells, COUPLB_REORDER_EVERY", reorder_every, "stop", stop, flush=True)
     steps = 120
     dev, fd = run_device(cmd, steps, dimension=3, dt=1.0 / 3, atoms_fn=atoms)
     ref, fo = run_oracle(cmd, steps, dimension=3, dt=1.0 / 3, atoms_fn=atoms)
@@ -335,6 +339,48 @@ def test_host_array_flags_match_plain_host_calls():
                 d.ibm(x, v, mm, f, positions_unchanged=same_x, **kw)
             out.append(f)
         assert np.array_equal(out[0], out[1]), step
+    a, b = (d.get_state() for d in devs)
+    assert np.array_equal(a, b)
+    for d in devs:
+        d.close()
+
+
+def test_chunked_host_sub_steps_match_device_arrays():
+    """Host-array sub-steps that only recompute forces (same positions, same
+    LB state) send v (m, f where needed) in chunks and run each chunk as it
+    arrives; forces and state equal the device-array path bit for bit, with
+    forces returned as F or added to the caller's f and masses that change
+    inside an interval."""
+    import couplb_amd
+    from tests.scenarios import DeviceArray
+    rng = np.random.default_rng(11)
+    n, N, md = 12000, 40, 4        # n >= 8192: four chunks
+    kw = dict(lo=(0.0, 0.0, 0.0), inv_dx=1.0, xi_over_dtLB=1.0, vel_scale=1.0, inv_N=1.0 / md, force_scale=1.0)
+    devs = [couplb_amd.Device(3, N, N, N, tau=0.8, kernel=couplb_amd.PESKIN4, md_per_lb=md) for _ in range(2)]
+    for d in devs:
+        d.init_equilibrium(1.0, (0.01, 0.0, 0.0))
+    m = rng.uniform(0.5, 1.5, n)
+    dx, dv, dm, df = (DeviceArray(np.zeros(sh)) for sh in ((n, 3), (n, 3), (n,), (n, 3)))
+    for step in range(12):
+        if step % md == 0:
+            for d in devs:
+                d.step((1e-6, 0.0, 0.0), couplb_amd.RHO_REFRESHED_NEXT)
+            x = rng.uniform(2, N - 2, (n, 3))               # new positions every interval
+        v = rng.normal(0.0, 1e-3, (n, 3))
+        if step == 6:
+            m = m * 1.25                                     # masses change inside an interval
+        zero = step % 3 != 1
+        f_in = np.zeros((n, 3)) if zero else rng.normal(0.0, 1e-4, (n, 3))
+        unchanged = step % md != 0
+        print("case chunked host sub-step", step, "forces_zero", zero, flush=True)
+        fa = f_in.copy()
+        devs[0].ibm(x, v, m, fa, positions_unchanged=unchanged, masses_unchanged=step not in (0, 6),
+                    forces_zero=zero, **kw)
+        for arr, val in ((dx, x), (dv, v), (dm, m), (df, f_in)):
+            arr.upload(val)
+        devs[1].ibm_device(n, dx.addr, dv.addr, dm.addr, df.addr, positions_unchanged=unchanged, **kw)
+        devs[1].sync()
+        assert np.array_equal(fa, df.download()), step
     a, b = (d.get_state() for d in devs)
     assert np.array_equal(a, b)
     for d in devs:

@@ -34,3 +34,17 @@ def test_library_exports_every_declared_symbol():
 def test_version_string():
     import couplb_amd
     assert "gfx950" in couplb_amd.version()
+
+
+def test_python_constants_match_the_header():
+    # the binding's status, storage, kernel, step-flag and MDF method codes
+    # are the values include/couplb_hip.h defines
+    import couplb_amd
+    txt = open(os.path.join(ROOT, "include", "couplb_hip.h")).read()
+    defs = {k: int(v) for k, v in re.findall(r"#define COUPLB_(\w+)\s+(-?\d+)", txt)}
+    pairs = {"OK": "OK", "EINVAL": "EINVAL", "EDEVICE": "EDEVICE", "ENOMEM": "ENOMEM", "ESTATE": "ESTATE",
+             "FP64": "FP64", "FP32": "FP32", "FP16": "FP16", "BF16": "BF16", "I16": "I16",
+             "ROMA": "ROMA", "PESKIN4": "PESKIN4", "STEP_RHO_REFRESHED_NEXT": "RHO_REFRESHED_NEXT",
+             "MDF_RICHARDSON": "MDF_RICHARDSON", "MDF_CG": "MDF_CG", "MDF_PCG": "MDF_PCG"}
+    for h, p in pairs.items():
+        assert defs[h] == getattr(couplb_amd, p), h

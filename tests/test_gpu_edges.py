@@ -1,6 +1,7 @@
 """Edge cases of the device IBM entry points: empty marker sets, markers
 that cannot be placed (non-finite or far outside the box), rows longer than
-1024 nodes, misuse of the multi-direct-forcing call, and density clamps."""
+1024 nodes, node runs longer than the pair maps' run lengths hold, misuse of
+the multi-direct-forcing call, and density clamps."""
 import numpy as np
 import pytest
 
@@ -72,6 +73,31 @@ def test_rows_longer_than_1024_nodes_match_the_oracle():
     ferr = max(rel_l2(p, q) for p, q in zip(fd, fo))
     print(f"  rel-L2(f) {err:.3e} rel-L2(F_atom) {ferr:.3e}")
     assert err <= 1e-12 and ferr <= 1e-11
+
+
+def test_node_runs_longer_than_a_run_length_holds_match_the_oracle():
+    # 70,000 markers in one cell: each of its 64 stencil nodes gets 70,000
+    # pairs, more than the pair maps' 16-bit run lengths hold; the fixed
+    # markers' step then spreads through the node rows instead
+    from tests.scenarios import atoms_from, interior, rel_l2, run_device, run_oracle
+    rng = np.random.default_rng(12)
+    n = 70000
+    x = 5.5 + rng.uniform(-0.4, 0.4, (n, 3))
+    base = atoms_from(x, rng.normal(0, 1e-3, (n, 3)), np.full(n, 1e-6))
+
+    def atoms(step):
+        return {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in base.items()}
+
+    cmd = "dense all couplb 10 10 10 0.1 1.0 kernel peskin4 gravity 1e-5 0.0 0.0"
+    print("case 70,000 markers in one cell", flush=True)
+    dev, fd = run_device(cmd, 4, dimension=3, dt=1.0, atoms_fn=atoms)
+    ref, fo = run_oracle(cmd, 4, dimension=3, dt=1.0, atoms_fn=atoms)
+    a = interior(dev.device_state().reshape(19, 12, 12, 12), 3)
+    b = interior(ref.f(), 3)
+    err = rel_l2(a, b)
+    ferr = max(rel_l2(p, q) for p, q in zip(fd, fo))
+    print(f"  rel-L2(f) {err:.3e} rel-L2(F_atom) {ferr:.3e}")
+    assert err <= 1e-12 and ferr <= 1e-10
 
 
 def test_mdf_must_be_the_only_ibm_call_of_an_interval():

@@ -93,29 +93,6 @@ __device__ inline int wrap_ghost(int a, int n) {
   return a == 0 ? n - 1 : (a == n + 1 ? 0 : a - 1);
 }
 
-// Images of interior coordinate I (ghost-inclusive 1..n) in the ghost-
-// inclusive frame: itself, plus the ghost plane that the self-periodic fold
-// (couplb_streaming.h:217-225: ghost_lo -> hi face, ghost_hi -> lo face)
-// maps onto it.  Faces carrying a wall never receive spread force.
-struct Img {
-  int n, a0, a1, a2;
-};
-// f(a) for each image in order a0, a1, a2 (straight-line code: an indexed
-// pick of a0..a2 would put the images in scratch memory)
-template <typename F>
-__device__ inline void each_image(const Img& im, F&& f) {
-  f(im.a0);
-  if (im.n > 1) f(im.a1);
-  if (im.n > 2) f(im.a2);
-}
-__device__ inline Img images(int I, int n, int tlo, int thi) {
-  Img r{1, I, -9, -9};
-  const bool lo = (I == n && tlo == 0), hi = (I == 1 && thi == 0);
-  if (lo) { r.a1 = 0; r.n = 2; if (hi) { r.a2 = n + 1; r.n = 3; } }
-  else if (hi) { r.a1 = n + 1; r.n = 2; }
-  return r;
-}
-
 // ---------------------------------------------------------------------------
 // Pull of the 19 (9) populations arriving at interior node (i, j, k).
 // Replaces Streaming::stream (couplb_streaming.h:93-132) applied to the

@@ -208,7 +208,8 @@ __global__ void __launch_bounds__(256) k_bj_perm(int n, int npad, const int* sva
 // One 64-lane block per marker block: gather the block's entries of Ah from
 // the explicit operator (rows with no fluid weight decoupled: identity),
 // invert by Gauss-Jordan elimination in LDS (Ah_bb is symmetric positive
-// definite: no pivoting), write the BJ x BJ inverse row-major.  A pivot
+// definite: no pivoting), write the BJ x BJ inverse column-major (element
+// (r, q) at q * BJ + r: the lanes of a block read one column together).  A pivot
 // that is not positive (a singular block) leaves the block's diagonal
 // inverse instead.
 __global__ void __launch_bounds__(64) k_bj_invert(const int* bperm, const int* bpos, const int* rowptr,
@@ -262,7 +263,7 @@ __global__ void __launch_bounds__(64) k_bj_invert(const int* bperm, const int* b
   }
   double* o = binv + (size_t)b * BJ * BJ;
   for (int e = l; e < BJ * BJ; e += 64) {
-    const int r = e / BJ, q = e % BJ;
+    const int q = e / BJ, r = e % BJ;
     o[e] = ok ? M[r][q] : (r == q ? 1.0 / dg[r] : 0.0);
   }
 }
@@ -313,14 +314,15 @@ __global__ void __launch_bounds__(MDF_THREADS) k_pcg_step(int npad, const int* b
   double a = 0.0, b = 0.0;
   if (k >= 0) {
     const int s0 = threadIdx.x & ~(BJ - 1);  // first lane of the block
-    const double2* row = reinterpret_cast<const double2*>(binv + (size_t)t * BJ);
+    // column u of the block's inverse holds row s of it at u * BJ + s
+    const double* col = binv + (size_t)(t - (threadIdx.x - s0)) * BJ + (threadIdx.x - s0);
     double z0 = 0.0, z1 = 0.0, z2 = 0.0;
-#pragma unroll 4
-    for (int u = 0; u < BJ / 2; u++) {
-      const double2 w = row[u];
-      z0 += w.x * rl[s0 + 2 * u][0] + w.y * rl[s0 + 2 * u + 1][0];
-      z1 += w.x * rl[s0 + 2 * u][1] + w.y * rl[s0 + 2 * u + 1][1];
-      z2 += w.x * rl[s0 + 2 * u][2] + w.y * rl[s0 + 2 * u + 1][2];
+#pragma unroll 8
+    for (int u = 0; u < BJ; u++) {
+      const double w = col[u * BJ];
+      z0 += w * rl[s0 + u][0];
+      z1 += w * rl[s0 + u][1];
+      z2 += w * rl[s0 + u][2];
     }
     Z[3 * (size_t)k + 0] = z0;
     Z[3 * (size_t)k + 1] = z1;

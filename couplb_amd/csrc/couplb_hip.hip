@@ -200,6 +200,8 @@ struct couplb_ctx {
   int mm_n = 0, mm_nnz = 0;
   long long mm_T = 0;        // triples
   unsigned long long* mm_val = nullptr;  // sorted triples: pair of k << 32 | pair of j
+  double* mm_q = nullptr;                // per pair: weight / (2 rho) of the current state
+  long mm_q_cap = 0;
   int *mm_ent = nullptr, *mm_elen = nullptr, *mm_erow = nullptr, *mm_col = nullptr;
   int* mm_rowptr = nullptr;
   double* mm_aval = nullptr;  // entries for the current density
@@ -1171,11 +1173,22 @@ int mdf_impl(couplb_ctx* c, int n, const double* x, const double* V, const doubl
   };
   if (use_mm) {
     // entries for this density, and b = V - I(u*) on the node path
-    hipLaunchKernelGGL(k_mm_values, dim3((unsigned)((c->mm_nnz + 255) / 256)), dim3(256), 0, c->stream,
+    if (c->mm_q_cap < c->pne) {
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      (void)hipFree(c->mm_q);
+      c->mm_q = nullptr;
+      c->mm_q_cap = 0;
+      HIPCHK(c, hipMalloc(&c->mm_q, sizeof(double) * (size_t)c->pne));
+      c->mm_q_cap = c->pne;
+    }
+    hipLaunchKernelGGL(k_mm_q, dim3((unsigned)((c->pne + 255) / 256)), dim3(256), 0, c->stream, c->pne,
+                       (const double*)c->pw, (const unsigned*)c->pkey_s, (unsigned)c->N, (const double*)c->u4,
+                       c->mm_q);
+    hipLaunchKernelGGL(k_mm_values, dim3((unsigned)(((long)MV_LANES * c->mm_nnz + 255) / 256)), dim3(256), 0,
+                       c->stream,
                        c->mm_nnz, (const int*)c->mm_ent, (const int*)c->mm_elen, (const int*)c->mm_erow,
-                       (const unsigned long long*)c->mm_val, (const double*)c->pw,
-                       (const unsigned*)c->pkey_s, (const double*)c->u4, (const double*)c->mm_gam,
-                       c->mm_aval);
+                       (const unsigned long long*)c->mm_val, (const double*)c->pw, (const double*)c->mm_q,
+                       (const double*)c->mm_gam, c->mm_aval);
     node_apply(1, nullptr, c->mm_B, Sd, part_a, part_b);
     if (pcg)
       hipLaunchKernelGGL(k_bj_invert, dim3((unsigned)(c->bj_npad / BJ)), dim3(64), 0, c->stream,
@@ -1925,6 +1938,7 @@ int couplb_destroy(couplb_ctx* c) {
   (void)hipFree(c->mlp_x); (void)hipFree(c->mlp_y); (void)hipFree(c->mlp_sig);
   (void)hipFree(c->mm_gam); (void)hipFree(c->mm_B);
   mm_free(c);
+  (void)hipFree(c->mm_q);
   if (c->mdf_host) (void)hipHostFree(c->mdf_host);
   for (auto& e : c->mdf_ev)
     if (e) (void)hipEventDestroy(e);

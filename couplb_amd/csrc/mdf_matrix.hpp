@@ -87,24 +87,46 @@ __global__ void __launch_bounds__(256) k_mm_rowptr(int n, int nnz, const int* er
   rowptr[k] = lo;
 }
 
-// values for the current density: A_kj = g_k sum_t pw[a_t] pw[b_t] / (2 rho(node of a_t))
+// per pair a of the sorted map, its weight over twice the density of its
+// node for the current state: q_a = pw_a (0.5 / rho(node of a))
+__global__ void __launch_bounds__(256) k_mm_q(long ne, const double* pw, const unsigned* pkey_s,
+                                              unsigned sentinel, const double* u4, double* q) {
+  const long a = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= ne) return;
+  const unsigned node = pkey_s[a];
+  q[a] = node != sentinel ? pw[a] * (0.5 / u4[4 * (size_t)node + 3]) : 0.0;
+}
+
+// values for the current density: A_kj = g_k sum_t q[a_t] pw[b_t].
+// MV_LANES lanes per entry: lane l sums the entry's triples l, l + MV_LANES,
+// ... in order (two loaded ahead), and the lanes' sums are added in a fixed
+// tree, so the entries do not depend on scheduling.
+constexpr int MV_LANES = 8;
 __global__ void __launch_bounds__(256) k_mm_values(int nnz, const int* ent, const int* elen,
                                                    const int* erow,
                                                    const unsigned long long* sval,
-                                                   const double* pw, const unsigned* pkey_s,
-                                                   const double* u4, const double* gam,
-                                                   double* aval) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nnz) return;
-  const int s = ent[e];
+                                                   const double* pw, const double* q,
+                                                   const double* gam, double* aval) {
+  const long tid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int e = (int)(tid / MV_LANES), l = (int)(tid % MV_LANES);
   double acc = 0.0;
-  for (int t = 0; t < elen[e]; t++) {
-    const unsigned long long v = sval[s + t];
-    const unsigned a = (unsigned)(v >> 32), b = (unsigned)(v & 0xffffffffull);
-    const double rho = u4[4 * (size_t)pkey_s[a] + 3];
-    acc += pw[a] * pw[b] * (0.5 / rho);
+  if (e < nnz) {
+    const int s = ent[e], len = elen[e];
+    for (int t = l; t < len; t += 2 * MV_LANES) {
+      const bool two = t + MV_LANES < len;
+      const unsigned long long v0 = sval[s + t];
+      const unsigned long long v1 = two ? sval[s + t + MV_LANES] : 0ull;
+      const double x0 = q[(unsigned)(v0 >> 32)], y0 = pw[(unsigned)(v0 & 0xffffffffull)];
+      const double x1 = two ? q[(unsigned)(v1 >> 32)] : 0.0;
+      const double y1 = two ? pw[(unsigned)(v1 & 0xffffffffull)] : 0.0;
+      acc += x0 * y0;
+      if (two) acc += x1 * y1;
+    }
   }
-  aval[e] = gam[erow[e]] * acc;
+  // the lanes' sums in a fixed tree (the lanes of an entry are adjacent in a wave)
+#pragma unroll
+  for (int o = 1; o < MV_LANES; o <<= 1) acc += __shfl_down(acc, o, MV_LANES);
+  if (e < nnz && l == 0) aval[e] = gam[erow[e]] * acc;
 }
 
 // y = A x, eight lanes per row, summed in a fixed tree.

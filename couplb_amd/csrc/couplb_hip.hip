@@ -174,7 +174,14 @@ struct couplb_ctx {
   int* tlist = nullptr;            // their list, then its length
   int ntiles = 0;
   int order_age = 0;         // LB intervals since the marker ordering was built
-  int reorder_every = 8;     // moving markers keep an ordering this many intervals
+  int reorder_every = 32;    // moving markers keep an ordering at most this many intervals
+  // records off the kept ordering's map at the last LB step, reported by the
+  // device into pinned memory and read once its event has completed
+  int* moved_host = nullptr;
+  hipEvent_t moved_ev = nullptr;
+  int moved_pending = 0;
+  int moved_epoch = -1;            // ordering the pending report belongs to
+  long moved_cnt = 0, moved_total = 0;
   int kept_move = 0;         // positions changed on the kept ordering since it was built
   double* vcache = nullptr;  // stencil velocities of moving markers (k_marker VelCache)
   int4* vbase = nullptr;
@@ -586,6 +593,8 @@ int prepare_markers(couplb_ctx* c, int n, const double* x, const IbmParams& ip,
     c->pos_epoch++;
     c->order_age = 0;
     c->kept_move = 0;
+    c->moved_cnt = 0;    // counts of the previous ordering no longer apply
+    c->moved_total = 0;
   } else if (!positions_unchanged) {
     c->kept_move = 1;  // positions changed on the kept ordering
   }
@@ -645,12 +654,26 @@ int ibm_impl(couplb_ctx* c, int n, const double* x, const double* v, const doubl
   // built its moving-marker map (markers that leave their base cell are
   // spread apart until the next ordering)
   const bool later = c->nslots > 0 || c->f4_partial;
-  const bool young = c->order_age < c->reorder_every;
-  const bool keep = later || (c->mpair_epoch == c->pos_epoch && young);
-  // a kept ordering that markers moved on is renewed at the start of an
-  // interval once it is old, also when the positions no longer change, so
-  // that markers which stopped get the fixed-marker path back
-  const int unchanged = a->positions_unchanged && !(c->kept_move && !later && !young);
+  if (c->moved_pending) {
+    const hipError_t q = hipEventQuery(c->moved_ev);
+    if (q == hipSuccess) {
+      if (c->moved_epoch == c->pos_epoch) c->moved_cnt = *c->moved_host;
+      c->moved_pending = 0;
+    } else if (q == hipErrorNotReady) {
+      (void)hipGetLastError();  // not an error: the report is still on its way
+    } else {
+      return fail(c, COUPLB_EDEVICE, hipGetErrorString(q));
+    }
+  }
+  // At the start of an interval a kept ordering that markers moved on is
+  // renewed when it is old, when more than a quarter of the last LB step's
+  // records had left its base cells, or when the positions are declared
+  // unchanged (the markers stopped: the fixed-marker path applies again).
+  const bool drifted = c->kept_move && 4 * c->moved_cnt > c->moved_total;
+  const bool renew = !later && c->kept_move &&
+                     (c->order_age >= c->reorder_every || drifted || a->positions_unchanged);
+  const bool keep = later || (c->mpair_epoch == c->pos_epoch && c->order_age < c->reorder_every && !renew);
+  const int unchanged = a->positions_unchanged && !renew;
   rc = prepare_markers(c, n, x, ip, unchanged, keep, &reorder);
   if (rc) return rc;
   // Same positions and same LB state as the previous sub-step of this
@@ -1458,9 +1481,24 @@ int row_spread(couplb_ctx* c, double* F4, unsigned char* flag, int fold, int acc
                        (const int*)nlarge, (const int*)c->sb_start, c->sb_ent, c->sb_ent2);
     HIPCHK(c, hipGetLastError());
   }
-  if (moved_only)
+  if (moved_only) {
+    // the number of records off the map goes to the host for the decision
+    // to renew the ordering (ibm_impl); no wait for it
+    if (!c->moved_host) {
+      HIPCHK(c, hipHostMalloc((void**)&c->moved_host, sizeof(int)));
+      HIPCHK(c, hipEventCreateWithFlags(&c->moved_ev, hipEventDisableTiming));
+    }
+    const bool report = !c->moved_pending;
     hipLaunchKernelGGL(k_tile_list, dim3((unsigned)((ntiles + tb - 1) / tb)), dim3(tb), 0, c->stream, ntiles,
-                       c->tflag, c->tlist, c->tlist + ntiles);
+                       c->tflag, c->tlist, c->tlist + ntiles, (const int*)(c->sb_start + nb),
+                       report ? c->moved_host : nullptr);
+    if (report) {
+      HIPCHK(c, hipEventRecord(c->moved_ev, c->stream));
+      c->moved_pending = 1;
+      c->moved_epoch = c->pos_epoch;
+      c->moved_total = total;
+    }
+  }
   TileSpreadArgs ta;
   ta.tag = (unsigned char)(moved_only ? c->f2_tag : 1);
   ta.tiles = moved_only ? c->tlist : nullptr;
@@ -1920,6 +1958,8 @@ int couplb_destroy(couplb_ctx* c) {
   (void)hipFree(c->ufc);
   (void)hipFree(c->staging);
   if (c->up_done) (void)hipEventDestroy(c->up_done);
+  if (c->moved_ev) (void)hipEventDestroy(c->moved_ev);
+  if (c->moved_host) (void)hipHostFree(c->moved_host);
   if (c->cstream) (void)hipStreamDestroy(c->cstream);
   for (int k = 0; k < HOST_CHUNKS_MAX; k++) {
     if (c->ev_up[k]) (void)hipEventDestroy(c->ev_up[k]);

@@ -141,8 +141,12 @@ __global__ void __launch_bounds__(256) k_rec_count(SlotSpan ss, Geo g, RecBin bi
 }
 
 // the marked tiles as a list (any order), their marks cleared
-__global__ void __launch_bounds__(256) k_tile_list(int ntiles, unsigned char* tflag, int* list, int* count) {
+// (thread 0 also copies the number of binned records, *nbinned, to the
+// host-visible *report when given)
+__global__ void __launch_bounds__(256) k_tile_list(int ntiles, unsigned char* tflag, int* list, int* count,
+                                                   const int* nbinned = nullptr, int* report = nullptr) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t == 0 && report) *report = *nbinned;
   if (t >= ntiles || !tflag[t]) return;
   list[atomicAdd(count, 1)] = t;
   tflag[t] = 0;

@@ -262,7 +262,8 @@ def test_more_sub_steps_than_record_slots_match_oracle(md):
     assert err <= TOL and ferr <= 1e-11
 
 
-@pytest.mark.parametrize("reorder_every,stop", [("8", None), ("3", None), ("0", None), ("8", 45)])
+@pytest.mark.parametrize("reorder_every,stop", [("8", None), ("3", None), ("0", None), ("8", 45), (None, None),
+                                                (None, 45)])
 def test_moving_markers_across_cells_match_oracle(reorder_every, stop, monkeypatch):
     """Markers that cross cells and periodic faces at every few MD steps over
     40 LB intervals (md_per_lb 3): the device keeps a marker ordering for
@@ -271,8 +272,13 @@ def test_moving_markers_across_cells_match_oracle(reorder_every, stop, monkeypat
     spreads the records of markers that left their base cell tile by tile;
     the result must be the reference's (fix_couplb.cpp:723-798) to rounding.
     stop: the markers stand still from that MD step on (the kept ordering is
-    renewed once it is old and the fixed-marker path takes over)."""
-    monkeypatch.setenv("COUPLB_REORDER_EVERY", reorder_every)
+    renewed once it is old and the fixed-marker path takes over).  None: the
+    default policy (renewed when old or when a quarter of the records left
+    the ordering's base cells)."""
+    if reorder_every is None:
+        monkeypatch.delenv("COUPLB_REORDER_EVERY", raising=False)
+    else:
+        monkeypatch.setenv("COUPLB_REORDER_EVERY", reorder_every)
     rng = np.random.default_rng(31)
     n = 80
     nx, ny, nz = 20, 14, 12

@@ -20,6 +20,7 @@ seconds / 1e6.
 from __future__ import annotations
 
 import argparse
+import gc
 import json
 import math
 import os
@@ -513,6 +514,12 @@ def main():
                          "c4 precision study, c5 learned MDF warm start")
     ap.add_argument("--grid", type=int, default=0, help="grid edge for c4 (default 384)")
     args = ap.parse_args()
+    # the timed regions run without Python's cyclic garbage collector (as
+    # timeit does): a full collection inside a timed region blocked the
+    # host for ~40 ms between two IBM calls.  Everything here is freed by
+    # reference counting.
+    gc.collect()
+    gc.disable()
     if args.cpu_only:
         nz, steps = (int(t) for t in args.cpu_only.split(","))
         print(json.dumps(cpu_baseline(args.n, nz, steps, 4)), flush=True)
@@ -681,7 +688,7 @@ def main():
     # clears its forces itself before each force computation) and the fix is
     # told so (f is then not sent to the device); the masses do not change.
     # The buffers are cleared before the timed steps.
-    host_steps = 8
+    host_steps = 24
     hfs = [torch.zeros((n, 3), dtype=torch.float64).pin_memory().numpy() for _ in range(md * (host_steps + 1))]
 
     def one_step_host(s, first=False):
@@ -710,7 +717,7 @@ def main():
     # of every MD sub-step (advanced by v * dt_MD as above) are prepared in
     # pinned buffers beforehand, so the timed steps hold the fix's own work
     # (uploads, sub-steps, force download), not the host's particle advance
-    mh_steps = 8
+    mh_steps = 24
     xs = [torch.from_numpy(x + v * ((k + 1) / md)).pin_memory().numpy() for k in range(md * (mh_steps + 1))]
 
     def one_step_moving_host(s):

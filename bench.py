@@ -500,7 +500,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--n", type=int, default=256, help="grid edge (interior nodes)")
     ap.add_argument("--particles", type=int, default=100000)
-    ap.add_argument("--storage", default="fp64", choices=["fp64", "fp32", "fp16", "bf16"])
+    ap.add_argument("--storage", default="fp64", choices=["fp64", "fp32", "fp16", "bf16", "i16"])
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--cpu-nz", type=int, default=64)
     ap.add_argument("--cpu-only", default="", metavar="NZ,STEPS",
@@ -611,7 +611,7 @@ def main():
     ga_tot, ga_n = dev.kernel_ms_total(4)
     nodes = N ** 3
     mlups = nodes * args.steps / wall / 1e6
-    s_bytes = {"fp64": 8, "fp32": 4, "fp16": 2, "bf16": 2}[args.storage]
+    s_bytes = {"fp64": 8, "fp32": 4, "fp16": 2, "bf16": 2, "i16": 2}[args.storage]
     b_lup = 2 * Q * s_bytes
     lb_avg = lb_tot / max(lb_n, 1)
     ibm_avg = ibm_tot / max(ibm_n, 1)
@@ -761,7 +761,7 @@ def main():
             cpu["one_thread"] = {"value": one["value"], "cores": one["cores"], "sample": one["sample"]}
 
     dtype = {"fp64": "f64", "fp32": "f32 storage/f32 math", "fp16": "f16 storage/f32 math",
-             "bf16": "bf16 storage/f32 math"}[args.storage]
+             "bf16": "bf16 storage/f32 math", "i16": "16-bit fixed-point storage/f32 math"}[args.storage]
     out = {
         "metric": "D3Q19 MLUPS & %HBM peak at 256^3 w/ IBM; markers/s; err vs FP64 oracle",
         "value": round(mlups, 2),

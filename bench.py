@@ -526,7 +526,7 @@ def main():
         return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
+        print("bench.py: one GPU is measured; multi-GPU runs are not part of it "
               f"(asked for --gpus {args.gpus}, WORLD_SIZE={world})", file=sys.stderr)
         sys.exit(2)
 

@@ -26,7 +26,7 @@ constexpr int MAX_HELD_STEPS = 512;
 constexpr uint32_t CLBK_MAGIC = 0x434C424Bu;  // couplb_io.h:19-23
 constexpr int32_t CLBK_VERSION = 1;
 
-// keyword whitelist, fix_couplb.cpp:22-36, plus this build's extensions
+// keyword whitelist, fix_couplb.cpp:22-36, plus the extensions
 // `storage` (population precision), `storage_range` (range of the 16-bit
 // fixed point) and `wall_vel_face` (per-face wall velocity)
 bool is_keyword(const char* s) {

@@ -1,7 +1,7 @@
 """ctypes bindings for the CPU oracle (TEST INFRASTRUCTURE ONLY).
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
-import this package: it is the checker for the HIP path, never the path.
+import this package: it is the oracle for the HIP path, never the path.
 
   Oracle(ops="oracle")  drives the C restatement in libcouplb_oracle.so
   Oracle(ops="ref")     drives the same orchestration on the reference's own

@@ -1,7 +1,7 @@
 /*
  * couplb_oracle.c -- CPU FP64 restatement of the CoupLB LB/IBM step.
  *
- * TEST INFRASTRUCTURE ONLY (the checker for the HIP path; see the header).
+ * TEST INFRASTRUCTURE ONLY (the oracle for the HIP path; see the header).
  * Built with -ffp-contract=off so every expression rounds as the reference
  * does when compiled the same way; expression order follows the reference so
  * that results agree bit-for-bit with oracle/_ref (the reference headers).

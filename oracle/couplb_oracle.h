@@ -1,7 +1,7 @@
 /*
  * couplb_oracle.h -- CPU FP64 restatement of the CoupLB LB/IBM step.
  *
- * TEST INFRASTRUCTURE ONLY.  This is the checker for the HIP product path:
+ * TEST INFRASTRUCTURE ONLY.  This is the oracle for the HIP product path:
  * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
  * load it.  Nothing in couplb_amd/ links or calls it.
  *

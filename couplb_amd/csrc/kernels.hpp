@@ -238,13 +238,20 @@ __device__ inline void pull_node_with(const PullCtx& pc, int i, int j, int k,
   pull_node_with<D, S, SCALED>(pc, pc.lay, static_cast<const S*>(pc.src), pc.rho_coll, i, j, k, f, fluid);
 }
 
-template <int D, typename S, bool SCALED = false>
+// TX: the x-shifted pulls (e_x != 0) as ordinary loads instead of
+// nontemporal ones, so that the 128-byte line a wavefront shares with its
+// x-neighbour stays in L2 for it.  Measured faster only in the FP64 step
+// with the fixed-marker map (about 1 %); slower in the plain step and the
+// velocity gather.
+template <int D, typename S, bool SCALED = false, bool TX = false>
 __device__ inline void pull_node(const PullCtx& pc, int i, int j, int k,
                                  typename Store<S>::T (&f)[Lat<D>::Q]) {
   const S* src = static_cast<const S*>(pc.src);
   const int ny = pc.g.ny;
   pull_node_with<D, S, SCALED>(pc, i, j, k, f, [&](int q, int si, int sj, int sk) {
-    return Store<S>::load(src + pc.lay.at(q, (unsigned)sj + (unsigned)ny * (unsigned)sk, (unsigned)si));
+    const S* p = src + pc.lay.at(q, (unsigned)sj + (unsigned)ny * (unsigned)sk, (unsigned)si);
+    if (TX && Lat<D>::ex(q) != 0) return Store<S>::decode(*p);
+    return Store<S>::load(p);
   });
 }
 
@@ -691,7 +698,7 @@ __global__ void __launch_bounds__(256) k_step(StepArgs a) {
   if (i >= g.nx) return;
   const unsigned node = (unsigned)i + (unsigned)g.nx * row;
   T f[Q];
-  pull_node<D, S, true>(a.pc, i, j, k, f);
+  pull_node<D, S, true, PM == 1>(a.pc, i, j, k, f);
   S* dst = static_cast<S*>(a.dst);
   const size_t at0 = a.pc.lay.at(0, row, (unsigned)i);
   step_node<D, S, PM>(a, i, j, k, node, f,

@@ -243,19 +243,41 @@ __global__ void __launch_bounds__(64) k_bj_invert(const int* bperm, const int* b
   const int b = blockIdx.x, l = threadIdx.x;
   for (int e = l; e < BJ * BJ; e += 64) M[e / BJ][e % BJ] = 0.0;
   __syncthreads();
-  if (l < BJ) {
-    const int k = bperm[b * BJ + l];
-    double d = 0.0;
-    if (k >= 0) {
-      const double gk = gam[k];
-      if (gk > 0.0) {
-        for (int e = rowptr[k]; e < rowptr[k + 1]; e++) {
-          const int pj = bpos[col[e]] - b * BJ;
-          if (pj >= 0 && pj < BJ) M[l][pj] = aval[e] / gk;
+  {
+    // two lanes per row, each taking every other entry of the row, four
+    // entries in flight (every entry has its own cell: the order of the
+    // stores does not matter)
+    const int r = l >> 1;
+    const int k = bperm[b * BJ + r];
+    const double gk = (k >= 0) ? gam[k] : 0.0;
+    if (k >= 0 && gk > 0.0) {
+      const int e1 = rowptr[k + 1];
+      int e = rowptr[k] + (l & 1);
+      for (; e + 6 < e1; e += 8) {
+        int cj[4];
+        double av[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          cj[u] = col[e + 2 * u];
+          av[u] = aval[e + 2 * u];
         }
-        d = M[l][l];
+        int pj[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) pj[u] = bpos[cj[u]] - b * BJ;
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+          if (pj[u] >= 0 && pj[u] < BJ) M[r][pj[u]] = av[u] / gk;
+      }
+      for (; e < e1; e += 2) {
+        const int pj = bpos[col[e]] - b * BJ;
+        if (pj >= 0 && pj < BJ) M[r][pj] = aval[e] / gk;
       }
     }
+  }
+  __syncthreads();
+  if (l < BJ) {
+    const int k = bperm[b * BJ + l];
+    const double d = (k >= 0 && gam[k] > 0.0) ? M[l][l] : 0.0;
     dz[l] = !(d > 0.0);
   }
   __syncthreads();
@@ -276,11 +298,19 @@ __global__ void __launch_bounds__(64) k_bj_invert(const int* bperm, const int* b
       ok = false;
       break;
     }
-    if (row == p)
-      for (int j = c0; j < c0 + BJ / 2; j++) M[p][j] = (j == p ? 1.0 : M[p][j]) / piv;
+    // the pivot row, one column per lane
+    if (l < BJ) M[p][l] = (l == p ? 1.0 : M[p][l]) / piv;
     __syncthreads();
-    if (row != p)
-      for (int j = c0; j < c0 + BJ / 2; j++) M[row][j] = (j == p ? 0.0 : M[row][j]) - f * M[p][j];
+    if (row != p) {  // half a row per lane, its values and the pivot row's in registers
+      double pr[BJ / 2], mr[BJ / 2];
+#pragma unroll
+      for (int u = 0; u < BJ / 2; u++) {
+        pr[u] = M[p][c0 + u];
+        mr[u] = M[row][c0 + u];
+      }
+#pragma unroll
+      for (int u = 0; u < BJ / 2; u++) M[row][c0 + u] = (c0 + u == p ? 0.0 : mr[u]) - f * pr[u];
+    }
     __syncthreads();
   }
   double* o = binv + (size_t)b * BJ * BJ;

@@ -1022,6 +1022,12 @@ int mm_build(couplb_ctx* c, int n) {
 int bj_build(couplb_ctx* c, int n, const MarkerRec* rec) {
   const int tb = 256;
   const int npad = (n + BJ - 1) / BJ * BJ;
+  if (c->bj_perm || c->bj_pos || c->bj_inv) {  // left by a build that failed part-way
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    (void)hipFree(c->bj_perm); (void)hipFree(c->bj_pos); (void)hipFree(c->bj_inv);
+    c->bj_perm = c->bj_pos = nullptr;
+    c->bj_inv = nullptr;
+  }
   unsigned long long *key = nullptr, *key_s = nullptr;
   int *val = nullptr, *val_s = nullptr;
   void* tmp = nullptr;
